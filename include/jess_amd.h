@@ -1,0 +1,165 @@
+/*
+ * jess_amd.h -- C ABI of the MI355X (gfx950) Jess template-matching library.
+ *
+ * Plain pointers and sizes only. All functions return 0 on success or a
+ * negative JAM_E* code; jam_last_error() then describes the failure (the
+ * message is per thread). Nothing here aborts the process.
+ *
+ * Ownership: every object created by a jam_*_create function is owned by the
+ * caller and released with the matching jam_*_free. Input arrays are copied
+ * during the call; the library keeps no pointer to caller memory.
+ *
+ * Each declaration names the reference interface (althonos/pyjess v0.10.1)
+ * that it replaces; paths are relative to the reference checkout.
+ */
+#ifndef JESS_AMD_H
+#define JESS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JAM_OK 0
+#define JAM_ENOMEM (-1)     /* host or device allocation failed */
+#define JAM_EINVAL (-2)     /* invalid argument */
+#define JAM_EMODE (-3)      /* template uses a match mode this build does not implement */
+#define JAM_EDEVICE (-4)    /* HIP runtime error or no usable gfx950 device */
+#define JAM_ETOOBIG (-5)    /* template larger than JAM_MAX_TEMPLATE_ATOMS */
+
+#define JAM_MAX_TEMPLATE_ATOMS 32
+
+/* ignore_chain values (IgnoreType, include/jess/template.pxd:5-8) */
+#define JAM_IGNORE_NONE 0
+#define JAM_IGNORE_RESIDUES 1
+#define JAM_IGNORE_ATOMS 2
+
+typedef struct jam_context jam_context;
+typedef struct jam_templates jam_templates;
+typedef struct jam_molecules jam_molecules;
+typedef struct jam_result jam_result;
+
+/* One molecule, structure of arrays. Names are 4 bytes per atom with '_' for
+ * blanks (pyjess encode_token, src/pyjess/_jess.pyx:133-140); residue names
+ * are 3 bytes + NUL per atom (Atom.resName, include/jess/atom.pxd:3-19). */
+typedef struct {
+    int32_t n_atoms;
+    const double *xyz;      /* 3 * n_atoms, x y z per atom */
+    const char *names;      /* 4 * n_atoms */
+    const char *resnames;   /* 4 * n_atoms */
+    const char *chain2;     /* n_atoms: second chain byte (chainID2) */
+    const int32_t *resseq;  /* n_atoms */
+    const int32_t *serial;  /* n_atoms */
+} jam_molecule_desc;
+
+/* A list of templates, flattened; template t owns atoms
+ * [atom_off[t], atom_off[t+1]). Per atom: the TessAtom fields
+ * (include/jess/tess_atom.pxd:3-17). names[4*k] for k in
+ * [name_off[a], name_off[a+1]) are the atom names of atom a, resnames
+ * likewise (3 bytes + NUL each). */
+typedef struct {
+    int32_t n_templates;
+    const int32_t *atom_off;     /* n_templates + 1 */
+    const int32_t *mode;         /* per atom: match mode */
+    const int32_t *resseq;
+    const char *chain2;
+    const double *pos;           /* 3 per atom */
+    const double *weight;        /* distance weight */
+    const int32_t *name_off;     /* n_atoms + 1 */
+    const char *names;
+    const int32_t *resname_off;  /* n_atoms + 1 */
+    const char *resnames;
+} jam_template_desc;
+
+typedef struct {
+    double rmsd_threshold;        /* Query.rmsd_threshold */
+    double distance_cutoff;       /* t of Jess_query */
+    double max_dynamic_distance;  /* s of Jess_query */
+    int32_t ignore_chain;         /* JAM_IGNORE_* */
+    int32_t reorder;              /* 0 or 1 */
+} jam_params;
+
+/* ---- device context ------------------------------------------------- */
+
+/* Opens device `device` (HIP ordinal), checks it is gfx950, creates one
+ * stream and a growable workspace. No reference counterpart: pyjess is
+ * CPU-only; this is the device state behind every query. */
+int32_t jam_context_create(int32_t device, jam_context **out);
+void jam_context_free(jam_context *ctx);
+
+/* ---- templates -------------------------------------------------------- */
+
+/* Replaces Jess_create + Jess_addTemplate (include/jess/jess.pxd:9-11,
+ * patches/Jess.c.patch:63-72) and the distance matrix of Template.__init__
+ * (src/pyjess/_jess.pyx:1895-1901, TessAtom_distance patches/TessAtom.c.patch:229-245).
+ * Copies the templates to the device, deduplicates atom types across them
+ * and computes the pairwise template distances. Templates keep the index
+ * they have in `desc`; queries scan them in reverse order, as Jess does. */
+int32_t jam_templates_create(jam_context *ctx, const jam_template_desc *desc, jam_templates **out);
+void jam_templates_free(jam_templates *t);
+
+/* ---- molecules -------------------------------------------------------- */
+
+/* Replaces Molecule.__init__ + ResIndex_create (src/pyjess/_jess.pyx:632-684,
+ * patches/ResIndex.c.patch:34-119): copies `n` molecules to the device as one
+ * batch and builds each residue-name index. */
+int32_t jam_molecules_create(jam_context *ctx, const jam_molecule_desc *mols, int32_t n,
+                             jam_molecules **out);
+void jam_molecules_free(jam_molecules *m);
+
+/* ---- queries ---------------------------------------------------------- */
+
+/* Replaces Jess_query + the JessQuery_next loop + JessQuery_superposition +
+ * Superposition_rmsd/rotation/centroid (include/jess/jess.pxd:11-21,
+ * include/jess/super.pxd:1-14, patches/Jess.c.patch:75-216) for every
+ * (template, molecule) pair of the batch at once. The result holds every
+ * complete match whose RMSD is <= rmsd_threshold ("events": hits, and the
+ * matches whose rotation has a NaN, which pyjess skips with a warning), each
+ * with its pair, its rank in the reference enumeration order of that pair,
+ * the matched atoms (molecule-local indices, template order), RMSD, rotation
+ * and centroids. Query.__next__'s policies (best_match, max_candidates) are
+ * applied by the caller over events sorted by (pair scan order, rank). */
+int32_t jam_query(jam_context *ctx, const jam_templates *t, const jam_molecules *m,
+                  const jam_params *params, jam_result **out);
+
+/* ---- results ---------------------------------------------------------- */
+
+int64_t jam_result_n_events(const jam_result *r);
+int32_t jam_result_stride(const jam_result *r);   /* atoms per event row */
+int32_t jam_result_n_pairs(const jam_result *r);  /* n_molecules * n_templates */
+/* Copies the events out. Any pointer may be NULL to skip that field.
+ * pair = molecule * n_templates + template. flag: 1 hit, 2 NaN rotation. */
+int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
+                        double *rmsd, double *rotation9, double *c0_3, double *c1_3,
+                        int32_t *atoms_stride);
+/* Number of complete matches (before the RMSD threshold) of every pair. */
+int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts);
+/* Kernel durations of the last query, milliseconds: [candidates, kd order,
+ * join, superposition, total device time]. */
+int32_t jam_result_timings(const jam_result *r, double *ms5);
+void jam_result_free(jam_result *r);
+
+/* ---- standalone superposition ---------------------------------------- */
+
+/* Batched least-squares superposition of `n_sets` point-set pairs of `k`
+ * points each, on the device: a (molecule side) and b (template side) are
+ * n_sets*k*3 doubles in host memory. Replaces Superposition_align +
+ * Superposition_rmsd/rotation/centroid (include/jess/super.pxd:1-14) for a
+ * batch. Outputs are n_sets, 9*n_sets, 3*n_sets, 3*n_sets doubles. */
+int32_t jam_superpose_batch(jam_context *ctx, const double *a, const double *b, int32_t k,
+                            int64_t n_sets, double *rmsd, double *rotation9, double *c0_3,
+                            double *c1_3);
+
+/* ---- misc ------------------------------------------------------------- */
+
+const char *jam_last_error(void);
+/* Library version string. */
+const char *jam_version(void);
+/* 1 when a usable gfx950 device is visible, else 0 (never initialises more than needed). */
+int32_t jam_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

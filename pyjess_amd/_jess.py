@@ -1,0 +1,416 @@
+"""`Jess`, `Query` and `Hit`: the drop-in query API (reference ``src/pyjess/_jess.pyx:2090-2869``).
+
+A query runs the whole search on the GPU in one call of the native library
+(every template against the molecule), the first time the `Query` iterator is
+advanced. The device returns "events": complete matches whose RMSD passes the
+threshold (hits, and matches whose rotation has a NaN), each with its rank in
+the reference engine's enumeration order. `Query` then applies the policies of
+the reference's ``Query.__next__`` (``_jess.pyx:2197-2283``) over the events in
+that order, so hits, their order, ``best_match`` and ``max_candidates`` behave
+as in pyjess.
+"""
+
+from __future__ import annotations
+
+import functools
+import io
+import math
+import warnings
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from ._model import Atom, Mat4, Molecule, Template, _apply
+
+__all__ = ["Jess", "Query", "Hit"]
+
+# TessTemplate_logE (patches/TessTemplate.c.patch:164-170):
+# logE = logA + alpha * rmsd + beta * dim + ln(n). logA is visible in the
+# reference; alpha and beta are fixed by the 156 logE values of the
+# 1AMY+1.3.3 fixture (the only pair consistent with all of them at their
+# printed precision).
+_LOG_A = -8.5
+_ALPHA = 2.5
+_BETA = -0.7
+
+_IGNORE = {None: 0, "residues": 1, "atoms": 2}
+_IGNORE_NAMES = {0: None, 1: "residues", 2: "atoms"}
+
+
+def _warn_nan():
+    warnings.warn("Jess returned a superposition matrix with NaN values", UserWarning, stacklevel=3)
+
+
+class Hit:
+    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``)."""
+
+    __slots__ = ("rmsd", "_rotation", "_centre", "_atoms", "_template", "_molecule")
+
+    def __init__(self, rmsd: float, rotation: Sequence[float], c0: Sequence[float], c1: Sequence[float],
+                 atoms: List[Atom], template: Template, molecule: Molecule):
+        self.rmsd = float(rmsd)
+        self._rotation = [float(v) for v in rotation]
+        self._centre = ([float(v) for v in c0], [float(v) for v in c1])
+        self._atoms = atoms
+        self._template = template
+        self._molecule = molecule
+
+    def __reduce__(self):
+        return _restore_hit, (self.rmsd, self._rotation, self._centre[0], self._centre[1],
+                              list(self._atoms), self._template, self._molecule)
+
+    def __repr__(self):
+        return f"<Hit rmsd={self.rmsd!r} template={self._template.id!r}>"
+
+    @property
+    def transformation(self) -> Mat4:
+        """Matrix taking molecule coordinates into the template frame (``_jess.pyx:2343-2377``)."""
+        M = self._rotation
+        c, v = self._centre
+        Mc = np.eye(4)
+        Mc[0:3, 3] = [-c[0], -c[1], -c[2]]
+        Mr = np.eye(4)
+        Mr[0:3, 0:3] = np.array(M).reshape(3, 3)
+        Mv = np.eye(4)
+        Mv[0:3, 3] = v
+        return Mat4._wrap(Mv) @ Mat4._wrap(Mr) @ Mat4._wrap(Mc)
+
+    @property
+    def inverse_transformation(self) -> Mat4:
+        """Matrix taking template coordinates into the molecule frame (``_jess.pyx:2379-2413``)."""
+        M = self._rotation
+        c, v = self._centre
+        Mv = np.eye(4)
+        Mv[0:3, 3] = [-v[0], -v[1], -v[2]]
+        Mr = np.eye(4)
+        Mr[0:3, 0:3] = np.array(M).reshape(3, 3).T
+        Mc = np.eye(4)
+        Mc[0:3, 3] = c
+        return Mat4._wrap(Mc) @ Mat4._wrap(Mr) @ Mat4._wrap(Mv)
+
+    @property
+    def determinant(self) -> float:
+        p = self._rotation
+        det = 0.0
+        det += p[0] * (p[4] * p[8] - p[5] * p[7])
+        det -= p[1] * (p[3] * p[8] - p[5] * p[6])
+        det += p[2] * (p[3] * p[7] - p[4] * p[6])
+        return det
+
+    @property
+    def log_evalue(self) -> float:
+        n = len(self._molecule)
+        return _LOG_A + _ALPHA * self.rmsd + _BETA * float(self._template.dimension) + math.log(float(n))
+
+    @property
+    def evalue(self) -> float:
+        return math.exp(self.log_evalue)
+
+    def atoms(self, transform: bool = True) -> List[Atom]:
+        if not transform:
+            return [a.copy() for a in self._atoms]
+        trans = self.transformation
+        return [a.transform(trans) for a in self._atoms]
+
+    def molecule(self, transform: bool = False) -> Molecule:
+        if not transform:
+            return self._molecule
+        return self._molecule.transform(self.transformation)
+
+    def template(self, transform: bool = False) -> Template:
+        if not transform:
+            return self._template
+        return self._template.transform(self.inverse_transformation)
+
+    def dumps(self, format: str = "pdb", transform: bool = True) -> str:
+        out = io.StringIO()
+        self.dump(out, format=format, transform=transform)
+        return out.getvalue()
+
+    def dump(self, file, format: str = "pdb", transform: bool = True) -> None:
+        if format != "pdb":
+            raise ValueError(f"invalid value for `format` argument: {format!r}")
+        if self._template.id is None:
+            raise RuntimeError("cannot dump `Hit` where `self.template.id` is `None`")
+        if self._molecule.id is None:
+            raise RuntimeError("cannot dump `Hit` where `self.molecule.id` is `None`")
+        file.write("REMARK ")
+        file.write(self._molecule.id)
+        file.write(f" {self.rmsd:5.3f} ")
+        file.write(self._template.id)
+        file.write(f" Det={self.determinant:4,.1f} log(E)~ {self.log_evalue:4.2f}\n")
+        mat = [float(v) for v in np.asarray(self.transformation).reshape(16)]
+        for a in self._atoms:
+            if transform:
+                x, y, z = _apply(mat, a.x, a.y, a.z)
+            else:
+                x, y, z = a.x, a.y, a.z
+            file.write(a._line(x, y, z))
+        file.write("ENDMDL\n")
+
+
+def _restore_hit(rmsd, rotation, c0, c1, atoms, template, molecule):
+    return Hit(rmsd, rotation, c0, c1, atoms, template, molecule)
+
+
+class Query:
+    """An iterator over the hits of a query (``_jess.pyx:2090-2283``)."""
+
+    def __init__(self):
+        raise TypeError("Query objects are created by Jess.query")
+
+    @classmethod
+    def _create(cls, jess: "Jess", molecule: Molecule, rmsd_threshold: float, distance_cutoff: float,
+                max_dynamic_distance: float, max_candidates, ignore_chain, best_match: bool,
+                reorder: bool) -> "Query":
+        q = cls.__new__(cls)
+        q.jess = jess
+        q.molecule = molecule
+        q.rmsd_threshold = float(rmsd_threshold)
+        q.best_match = bool(best_match)
+        q.max_candidates = max_candidates
+        q.ignore_chain = ignore_chain
+        q._distance_cutoff = float(distance_cutoff)
+        q._max_dynamic_distance = float(max_dynamic_distance)
+        q._reorder = bool(reorder)
+        q._hits = None
+        q._pos = 0
+        return q
+
+    @property
+    def ignore_chain(self):
+        return _IGNORE_NAMES[self._ignore_chain]
+
+    @ignore_chain.setter
+    def ignore_chain(self, ignore_chain):
+        if ignore_chain not in _IGNORE:
+            raise ValueError(f"invalid value for `ignore_chain`: {ignore_chain!r}")
+        self._ignore_chain = _IGNORE[ignore_chain]
+
+    @property
+    def max_candidates(self):
+        return None if self._max_candidates == -1 else self._max_candidates
+
+    @max_candidates.setter
+    def max_candidates(self, max_candidates):
+        if max_candidates is None:
+            self._max_candidates = -1
+        elif max_candidates >= 0:
+            self._max_candidates = int(max_candidates)
+        else:
+            raise ValueError(f"invalid value for `max_candidates` argument: {max_candidates!r}")
+
+    def __iter__(self):
+        return self
+
+    def __next__(self) -> Hit:
+        if self._hits is None:
+            self._hits = self._run()
+        while self._pos < len(self._hits):
+            item = self._hits[self._pos]
+            self._pos += 1
+            if item is None:        # a NaN rotation reached in pyjess's iteration order
+                _warn_nan()
+                continue
+            return item
+        raise StopIteration
+
+    def _run(self) -> list:
+        if len(self.jess) == 0 or len(self.molecule) == 0:
+            return []
+        res = self.jess._device_query([self.molecule], self.rmsd_threshold, self._distance_cutoff,
+                                      self._max_dynamic_distance, self._ignore_chain, self._reorder)
+        return _policy(self.jess, [self.molecule], res, self.best_match, self._max_candidates)[0]
+
+
+def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", best_match: bool,
+            max_candidates: int) -> List[list]:
+    """Hits per molecule, in pyjess order, from the device events.
+
+    Templates are scanned in reverse insertion order (``Jess_addTemplate``
+    prepends, ``patches/Jess.c.patch:63-72``). Within a template the events are
+    taken by rank, the position of the complete match in the reference
+    enumeration. The candidate counter of ``Query.__next__`` restarts at every
+    template and after every yielded hit; when it exceeds ``max_candidates``
+    the rest of the template is skipped (after the current match, which is
+    still reported if it is a hit). NaN rotations produce ``None`` entries,
+    turned into warnings when iterated.
+    """
+    T = len(jess)
+    out: List[list] = [[] for _ in molecules]
+    if len(res.rmsd) == 0:
+        return out
+    mc = max_candidates
+    mol_of = res.pair // T
+    tpl_of = res.pair % T
+    # molecule, then scan position of the template (reverse order), then rank
+    order = np.lexsort((res.rank, (T - 1) - tpl_of, mol_of))
+    i = 0
+    n = len(order)
+    while i < n:
+        e0 = order[i]
+        p = res.pair[e0]
+        j = i
+        while j < n and res.pair[order[j]] == p:
+            j += 1
+        events = order[i:j]
+        i = j
+        mi = int(mol_of[e0])
+        t = int(tpl_of[e0])
+        mol = molecules[mi]
+        chosen = []
+        if best_match:
+            best = None
+            best_rmsd = math.inf
+            for e in events:
+                rank = int(res.rank[e])
+                if mc != -1 and rank > mc + 1:
+                    break
+                r = float(res.rmsd[e])
+                if r < best_rmsd:
+                    if res.flag[e] == 2:
+                        chosen.append(None)
+                    else:
+                        best, best_rmsd = e, r
+                if mc != -1 and rank == mc + 1:
+                    break
+            if best is not None:
+                chosen.append(best)
+        else:
+            seg = 0
+            for e in events:
+                c = int(res.rank[e]) - seg
+                if mc != -1 and c > mc + 1:
+                    break
+                if res.flag[e] == 2:
+                    chosen.append(None)
+                else:
+                    chosen.append(e)
+                    seg = int(res.rank[e]) + 1
+                if mc != -1 and c == mc + 1:
+                    break
+        tpl = jess._templates[t]
+        for e in chosen:
+            if e is None:
+                out[mi].append(None)
+                continue
+            n_atoms = len(tpl)
+            atoms = [mol._atom(int(k)) for k in res.atoms[e, :n_atoms]]
+            out[mi].append(Hit(res.rmsd[e], res.rotation[e], res.c0[e], res.c1[e], atoms, tpl, mol))
+    return out
+
+
+class Jess:
+    """A handle to run Jess over a list of templates (``_jess.pyx:2623-2869``)."""
+
+    def __init__(self, templates: Iterable[Template] = ()):
+        self._templates = tuple(templates)
+        for t in self._templates:
+            if not isinstance(t, Template):
+                raise TypeError(f"expected Template, found {type(t).__name__}")
+        self._tset = None
+
+    def __len__(self) -> int:
+        return len(self._templates)
+
+    def __bool__(self) -> bool:
+        return len(self._templates) > 0
+
+    def __getitem__(self, index):
+        if isinstance(index, slice):
+            return type(self)(self._templates[index])
+        return self._templates[index]
+
+    def __eq__(self, other):
+        if not isinstance(other, Jess):
+            return NotImplemented
+        return self._templates == other._templates
+
+    def __hash__(self):
+        return hash((Jess, self._templates))
+
+    def __reduce__(self):
+        return type(self), (self._templates,)
+
+    def __copy__(self):
+        return self.copy()
+
+    def copy(self) -> "Jess":
+        return type(self)(self._templates)
+
+    def _device_templates(self, ctx) -> "_native.TemplateSet":
+        if self._tset is None or self._tset.ctx is not ctx:
+            self._tset = _native.TemplateSet(ctx, self._templates)
+        return self._tset
+
+    def _device_query(self, molecules: Sequence[Molecule], rmsd_threshold, distance_cutoff,
+                      max_dynamic_distance, ignore_chain: int, reorder: bool) -> "_native.Result":
+        ctx = _native.context()
+        tset = self._device_templates(ctx)
+        if len(molecules) == 1:
+            mol = molecules[0]
+            mset = mol._device.get(id(ctx))
+            if mset is None:
+                mset = mol._device[id(ctx)] = _native.MoleculeSet(ctx, [mol])
+        else:
+            mset = _native.MoleculeSet(ctx, molecules)
+        return _native.query(tset, mset, rmsd_threshold, distance_cutoff, max_dynamic_distance,
+                             ignore_chain, reorder)
+
+    def query(self, molecule: Molecule, rmsd_threshold: float, distance_cutoff: float,
+              max_dynamic_distance: float, *, max_candidates=None, ignore_chain=None,
+              best_match: bool = False, reorder: bool = True) -> Query:
+        """Scan for templates matching the given molecule (``_jess.pyx:2758-2869``)."""
+        ignore_chain = _normalize_ignore_chain(ignore_chain)
+        if not isinstance(molecule, Molecule):
+            raise TypeError(f"expected Molecule, found {type(molecule).__name__}")
+        return Query._create(self, molecule, rmsd_threshold, distance_cutoff, max_dynamic_distance,
+                             max_candidates, ignore_chain, best_match, reorder)
+
+    def query_batch(self, molecules: Sequence[Molecule], rmsd_threshold: float, distance_cutoff: float,
+                    max_dynamic_distance: float, *, max_candidates=None, ignore_chain=None,
+                    best_match: bool = False, reorder: bool = True) -> List[List[Hit]]:
+        """All hits of many molecules in one device pass.
+
+        Equivalent to ``[list(self.query(m, ...)) for m in molecules]`` (the
+        ThreadPool-over-molecules pattern of the reference README) but runs
+        every (template, molecule) pair in a single launch chain.
+        """
+        ignore_chain = _normalize_ignore_chain(ignore_chain)
+        molecules = list(molecules)
+        if max_candidates is None:
+            mc = -1
+        elif max_candidates >= 0:
+            mc = int(max_candidates)
+        else:
+            raise ValueError(f"invalid value for `max_candidates` argument: {max_candidates!r}")
+        if not molecules or len(self) == 0:
+            return [[] for _ in molecules]
+        res = self._device_query(molecules, rmsd_threshold, distance_cutoff, max_dynamic_distance,
+                                 _IGNORE[ignore_chain], reorder)
+        out = _policy(self, molecules, res, best_match, mc)
+        hits = []
+        for lst in out:
+            if any(h is None for h in lst):
+                _warn_nan()
+            hits.append([h for h in lst if h is not None])
+        return hits
+
+
+def _normalize_ignore_chain(ignore_chain):
+    if ignore_chain is True:
+        warnings.warn(
+            "`ignore_chain` parameter expects string parameters to specificy the mode since "
+            "PyJess v0.7.0. Use `ignore_chain='atoms'` instead of `ignore_chain=True`",
+            DeprecationWarning, stacklevel=3)
+        return "atoms"
+    if ignore_chain is False:
+        warnings.warn(
+            "`ignore_chain` parameter expects string parameters to specificy the mode since "
+            "PyJess v0.7.0. Use `ignore_chain=None` instead of `ignore_chain=False`",
+            DeprecationWarning, stacklevel=3)
+        return None
+    if ignore_chain not in _IGNORE:
+        raise ValueError(f"invalid value for `ignore_chain`: {ignore_chain!r}")
+    return ignore_chain

@@ -1,0 +1,346 @@
+"""ctypes binding of the native library (``libjess_amd.so``, C ABI in ``include/jess_amd.h``).
+
+The library must be built (``python -m pyjess_amd._build``) and a gfx950 GPU
+must be visible for queries; there is no CPU fallback, every failure raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _build
+
+JAM_OK = 0
+JAM_ENOMEM = -1
+JAM_EINVAL = -2
+JAM_EMODE = -3
+JAM_EDEVICE = -4
+JAM_ETOOBIG = -5
+MAX_TEMPLATE_ATOMS = 32
+
+_SYMBOLS = (
+    "jam_context_create", "jam_context_free", "jam_templates_create", "jam_templates_free",
+    "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
+    "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
+    "jam_result_timings", "jam_result_free", "jam_superpose_batch", "jam_last_error",
+    "jam_version", "jam_device_available",
+)
+
+
+class NativeError(RuntimeError):
+    """An error reported by the native library."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (code {code})")
+        self.code = code
+
+
+class UnsupportedModeError(NativeError, NotImplementedError):
+    """A template uses a match mode that the device path does not implement."""
+
+
+class MoleculeDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_atoms", ctypes.c_int32),
+        ("xyz", ctypes.c_void_p),
+        ("names", ctypes.c_void_p),
+        ("resnames", ctypes.c_void_p),
+        ("chain2", ctypes.c_void_p),
+        ("resseq", ctypes.c_void_p),
+        ("serial", ctypes.c_void_p),
+    ]
+
+
+class TemplateDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_templates", ctypes.c_int32),
+        ("atom_off", ctypes.c_void_p),
+        ("mode", ctypes.c_void_p),
+        ("resseq", ctypes.c_void_p),
+        ("chain2", ctypes.c_void_p),
+        ("pos", ctypes.c_void_p),
+        ("weight", ctypes.c_void_p),
+        ("name_off", ctypes.c_void_p),
+        ("names", ctypes.c_void_p),
+        ("resname_off", ctypes.c_void_p),
+        ("resnames", ctypes.c_void_p),
+    ]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("rmsd_threshold", ctypes.c_double),
+        ("distance_cutoff", ctypes.c_double),
+        ("max_dynamic_distance", ctypes.c_double),
+        ("ignore_chain", ctypes.c_int32),
+        ("reorder", ctypes.c_int32),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def library_path() -> str:
+    return _build.LIB_PATH
+
+
+def lib():
+    """Loads ``libjess_amd.so``; raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(
+                f"native library {path} is missing: build it with `python -m pyjess_amd._build`"
+            )
+        L = ctypes.CDLL(path)
+        vp, i32, i64, P = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER
+        L.jam_context_create.argtypes = [i32, P(vp)]
+        L.jam_context_create.restype = i32
+        L.jam_context_free.argtypes = [vp]
+        L.jam_context_free.restype = None
+        L.jam_templates_create.argtypes = [vp, P(TemplateDesc), P(vp)]
+        L.jam_templates_create.restype = i32
+        L.jam_templates_free.argtypes = [vp]
+        L.jam_templates_free.restype = None
+        L.jam_molecules_create.argtypes = [vp, P(MoleculeDesc), i32, P(vp)]
+        L.jam_molecules_create.restype = i32
+        L.jam_molecules_free.argtypes = [vp]
+        L.jam_molecules_free.restype = None
+        L.jam_query.argtypes = [vp, vp, vp, P(Params), P(vp)]
+        L.jam_query.restype = i32
+        L.jam_result_n_events.argtypes = [vp]
+        L.jam_result_n_events.restype = i64
+        L.jam_result_stride.argtypes = [vp]
+        L.jam_result_stride.restype = i32
+        L.jam_result_n_pairs.argtypes = [vp]
+        L.jam_result_n_pairs.restype = i32
+        L.jam_result_copy.argtypes = [vp] * 9
+        L.jam_result_copy.restype = i32
+        L.jam_result_pair_matches.argtypes = [vp, vp]
+        L.jam_result_pair_matches.restype = i32
+        L.jam_result_timings.argtypes = [vp, vp]
+        L.jam_result_timings.restype = i32
+        L.jam_result_free.argtypes = [vp]
+        L.jam_result_free.restype = None
+        L.jam_superpose_batch.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, vp]
+        L.jam_superpose_batch.restype = i32
+        L.jam_last_error.argtypes = []
+        L.jam_last_error.restype = ctypes.c_char_p
+        L.jam_version.argtypes = []
+        L.jam_version.restype = ctypes.c_char_p
+        L.jam_device_available.argtypes = []
+        L.jam_device_available.restype = i32
+        _lib = L
+        return L
+
+
+def _check(rc: int) -> None:
+    if rc != JAM_OK:
+        msg = lib().jam_last_error().decode("utf-8", "replace")
+        if rc == JAM_EMODE:
+            raise UnsupportedModeError(rc, msg)
+        if rc == JAM_ENOMEM:
+            raise MemoryError(msg)
+        raise NativeError(rc, msg)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# context
+# ---------------------------------------------------------------------------
+
+class Context:
+    """One device context (stream + workspace). Queries on one context are serialised."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.jam_context_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+        self.lock = threading.Lock()
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.jam_context_free(h)
+            self.handle = None
+
+
+_contexts = {}
+_ctx_lock = threading.Lock()
+
+
+def context(device: int = 0) -> Context:
+    with _ctx_lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            ctx = _contexts[device] = Context(device)
+        return ctx
+
+
+# ---------------------------------------------------------------------------
+# templates and molecules
+# ---------------------------------------------------------------------------
+
+class TemplateSet:
+    """Device copy of a list of templates."""
+
+    def __init__(self, ctx: Context, templates: Sequence):
+        off, mode, resseq, chain2, pos, weight = [0], [], [], [], [], []
+        name_off, names, rn_off, rns = [0], [], [0], []
+        for t in templates:
+            if len(t) > MAX_TEMPLATE_ATOMS:
+                raise ValueError(f"templates of more than {MAX_TEMPLATE_ATOMS} atoms are not supported")
+            for a in t:
+                mode.append(a._mode)
+                resseq.append(a._resseq)
+                chain2.append(a._chain2)
+                pos.append((a.x, a.y, a.z))
+                weight.append(a._weight)
+                names.extend(a._names)
+                name_off.append(len(names))
+                rns.extend(a._resnames)
+                rn_off.append(len(rns))
+            off.append(len(mode))
+        self.n_templates = len(off) - 1
+        arrays = dict(
+            atom_off=np.array(off, dtype=np.int32),
+            mode=np.array(mode or [0], dtype=np.int32),
+            resseq=np.array(resseq or [0], dtype=np.int32),
+            chain2=np.array(chain2 or [0], dtype=np.uint8),
+            pos=np.array(pos or [(0.0, 0.0, 0.0)], dtype=np.float64),
+            weight=np.array(weight or [0.0], dtype=np.float64),
+            name_off=np.array(name_off, dtype=np.int32),
+            names=np.frombuffer(b"".join(names) or b"\0\0\0\0", dtype=np.uint8).copy(),
+            resname_off=np.array(rn_off, dtype=np.int32),
+            resnames=np.frombuffer(b"".join(rns) or b"\0\0\0\0", dtype=np.uint8).copy(),
+        )
+        desc = TemplateDesc(self.n_templates, *(_ptr(arrays[k]) for k in (
+            "atom_off", "mode", "resseq", "chain2", "pos", "weight", "name_off", "names",
+            "resname_off", "resnames")))
+        h = ctypes.c_void_p()
+        _check(lib().jam_templates_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.jam_templates_free(h)
+            self.handle = None
+
+
+class MoleculeSet:
+    """Device copy of a batch of molecules."""
+
+    def __init__(self, ctx: Context, molecules: Sequence):
+        descs = (MoleculeDesc * max(1, len(molecules)))()
+        keep = []
+        for i, m in enumerate(molecules):
+            arrs = (
+                np.ascontiguousarray(m.xyz, dtype=np.float64),
+                np.ascontiguousarray(m.names, dtype=np.uint8),
+                np.ascontiguousarray(m.resnames, dtype=np.uint8),
+                np.ascontiguousarray(m.chains[:, 1], dtype=np.uint8),
+                np.ascontiguousarray(m.resseq, dtype=np.int32),
+                np.ascontiguousarray(m.serial, dtype=np.int32),
+            )
+            keep.append(arrs)
+            descs[i] = MoleculeDesc(len(m), *(_ptr(a) for a in arrs))
+        h = ctypes.c_void_p()
+        _check(lib().jam_molecules_create(ctx.handle, descs, len(molecules), ctypes.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+        self.n_molecules = len(molecules)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            _lib.jam_molecules_free(h)
+            self.handle = None
+
+
+class Result:
+    """Events of one device query, as numpy arrays."""
+
+    __slots__ = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms", "pair_matches",
+                 "timings_ms", "n_pairs", "stride")
+
+    def __init__(self, handle):
+        L = lib()
+        n = L.jam_result_n_events(handle)
+        self.stride = L.jam_result_stride(handle)
+        self.n_pairs = L.jam_result_n_pairs(handle)
+        m = max(n, 1)
+        self.pair = np.zeros(m, dtype=np.int32)
+        self.rank = np.zeros(m, dtype=np.int64)
+        self.flag = np.zeros(m, dtype=np.int32)
+        self.rmsd = np.zeros(m, dtype=np.float64)
+        self.rotation = np.zeros((m, 9), dtype=np.float64)
+        self.c0 = np.zeros((m, 3), dtype=np.float64)
+        self.c1 = np.zeros((m, 3), dtype=np.float64)
+        self.atoms = np.zeros((m, max(self.stride, 1)), dtype=np.int32)
+        _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
+                                 _ptr(self.rmsd), _ptr(self.rotation), _ptr(self.c0), _ptr(self.c1),
+                                 _ptr(self.atoms)))
+        for name in ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms"):
+            setattr(self, name, getattr(self, name)[:n])
+        self.pair_matches = np.zeros(max(self.n_pairs, 1), dtype=np.int64)
+        _check(L.jam_result_pair_matches(handle, _ptr(self.pair_matches)))
+        self.pair_matches = self.pair_matches[:self.n_pairs]
+        self.timings_ms = np.zeros(5, dtype=np.float64)
+        _check(L.jam_result_timings(handle, _ptr(self.timings_ms)))
+
+
+def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_cutoff: float,
+          max_dynamic_distance: float, ignore_chain: int, reorder: bool) -> Result:
+    L = lib()
+    ctx = tset.ctx
+    p = Params(float(rmsd_threshold), float(distance_cutoff), float(max_dynamic_distance),
+               int(ignore_chain), int(bool(reorder)))
+    h = ctypes.c_void_p()
+    with ctx.lock:
+        _check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h)))
+    try:
+        return Result(h)
+    finally:
+        L.jam_result_free(h)
+
+
+def superpose_batch(a: np.ndarray, b: np.ndarray, device: int = 0):
+    """Batched superposition of point sets a, b with shape (n_sets, k, 3)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    if a.shape != b.shape or a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("expected two arrays of shape (n_sets, k, 3)")
+    n, k, _ = a.shape
+    rmsd = np.zeros(max(n, 1))
+    rot = np.zeros((max(n, 1), 9))
+    c0 = np.zeros((max(n, 1), 3))
+    c1 = np.zeros((max(n, 1), 3))
+    ctx = context(device)
+    with ctx.lock:
+        _check(lib().jam_superpose_batch(ctx.handle, _ptr(a), _ptr(b), int(k), int(n), _ptr(rmsd),
+                                         _ptr(rot), _ptr(c0), _ptr(c1)))
+    return rmsd[:n], rot[:n].reshape(n, 3, 3), c0[:n], c1[:n]
+
+
+def device_available() -> bool:
+    return bool(lib().jam_device_available())
+
+
+def version() -> str:
+    return lib().jam_version().decode()

@@ -1,0 +1,734 @@
+// jam_api.hip -- host side of the C ABI declared in include/jess_amd.h.
+//
+// A query runs as one asynchronous chain on the context's stream:
+//   k_candidates<count> -> exclusive scan -> k_candidates<fill> -> k_kdorder
+//   -> k_join -> k_super
+// followed by one copy of the counters. Buffers are sized from the previous
+// query; a kernel that would overflow one skips its writes (the counters keep
+// counting) and the host grows the buffer to the exact size and re-runs the
+// chain, so results never depend on capacities.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/jess_amd.h"
+#include "jam_common.h"
+#include "jam_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int32_t fail(int32_t code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(JAM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// A device allocation that grows (never shrinks) and frees itself.
+struct DBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t reserve(size_t n)
+    {
+        if (n <= bytes && ptr) return hipSuccess;
+        release();
+        size_t want = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipSuccess) bytes = want;
+        else ptr = nullptr;
+        return e;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+template <class T>
+hipError_t upload(DBuf &b, const std::vector<T> &v, hipStream_t s)
+{
+    hipError_t e = b.reserve(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    e = hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+uint32_t pack4(const char *p)
+{
+    return (uint32_t)(uint8_t)p[0] | ((uint32_t)(uint8_t)p[1] << 8) | ((uint32_t)(uint8_t)p[2] << 16) |
+           ((uint32_t)(uint8_t)p[3] << 24);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+
+struct jam_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    // per-query workspace
+    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total;
+    DBuf m_item, m_seq, m_atoms, m_count, item_cm;
+    DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms, e_count;
+    DBuf scan_tmp;
+    int64_t set_cap = 0, blk_cap = 0, match_cap = 0, event_cap = 0;
+    int nsets_cap = 0;
+    int match_stride = 0, event_stride = 0;
+    // standalone superposition buffers
+    DBuf sa, sb, srmsd, srot, sc0, sc1;
+};
+
+struct jam_templates {
+    jam_context *ctx = nullptr;
+    int n_templates = 0, n_types = 0, stride = 1;
+    DBuf atom_off, atom_type, chain, resseq, pos, weight, dist_off, dist;
+    DBuf type_mode, type_name_off, type_names, type_rn_off, type_rns;
+    DevTpls dev() const
+    {
+        DevTpls d;
+        d.n_templates = n_templates;
+        d.n_types = n_types;
+        d.stride = stride;
+        d.atom_off = atom_off.as<int>();
+        d.atom_type = atom_type.as<int>();
+        d.chain = chain.as<int>();
+        d.resseq = resseq.as<int>();
+        d.pos = pos.as<double>();
+        d.weight = weight.as<double>();
+        d.dist_off = dist_off.as<int>();
+        d.dist = dist.as<double>();
+        d.type_mode = type_mode.as<int>();
+        d.type_name_off = type_name_off.as<int>();
+        d.type_names = type_names.as<uint32_t>();
+        d.type_rn_off = type_rn_off.as<int>();
+        d.type_rns = type_rns.as<uint32_t>();
+        return d;
+    }
+};
+
+struct jam_molecules {
+    jam_context *ctx = nullptr;
+    int n_mols = 0;
+    int64_t n_atoms = 0;
+    DBuf atom_off, xyz, name, resname, chain, resseq, perm, res_off, res_key, res_begin, res_end;
+    DevMols dev() const
+    {
+        DevMols d;
+        d.n_mols = n_mols;
+        d.atom_off = atom_off.as<int>();
+        d.xyz = xyz.as<double>();
+        d.name = name.as<uint32_t>();
+        d.resname = resname.as<uint32_t>();
+        d.chain = chain.as<int>();
+        d.resseq = resseq.as<int>();
+        d.perm = perm.as<int>();
+        d.res_off = res_off.as<int>();
+        d.res_key = res_key.as<uint32_t>();
+        d.res_begin = res_begin.as<int>();
+        d.res_end = res_end.as<int>();
+        return d;
+    }
+};
+
+struct jam_result {
+    int n_pairs = 0;
+    int stride = 1;
+    std::vector<int32_t> pair, flag, atoms;
+    std::vector<int64_t> rank, pair_cm;
+    std::vector<double> rmsd, rot, c0, c1;
+    double ms[5] = {0, 0, 0, 0, 0};
+};
+
+// ---------------------------------------------------------------------------
+// misc
+// ---------------------------------------------------------------------------
+
+extern "C" const char *jam_last_error(void) { return g_error.c_str(); }
+
+extern "C" const char *jam_version(void) { return "pyjess-amd 0.1.0 (gfx950)"; }
+
+static bool is_gfx950(int dev)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return false;
+    return std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+}
+
+extern "C" int32_t jam_device_available(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    for (int i = 0; i < n; i++)
+        if (is_gfx950(i)) return 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
+{
+    if (!out) return fail(JAM_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(JAM_EDEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(JAM_EINVAL, "device ordinal out of range");
+    if (!is_gfx950(device)) return fail(JAM_EDEVICE, "device is not a gfx950 (MI355X) GPU");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<jam_context> c(new jam_context());
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
+    *out = c.release();
+    return JAM_OK;
+}
+
+extern "C" void jam_context_free(jam_context *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+// ---------------------------------------------------------------------------
+// templates
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_templates_create(jam_context *ctx, const jam_template_desc *d, jam_templates **out)
+{
+    if (!ctx || !d || !out) return fail(JAM_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (d->n_templates < 0) return fail(JAM_EINVAL, "negative template count");
+    const int T = d->n_templates;
+    const int A = T > 0 ? d->atom_off[T] : 0;
+    for (int t = 0; t < T; t++) {
+        int n = d->atom_off[t + 1] - d->atom_off[t];
+        if (n < 0) return fail(JAM_EINVAL, "atom offsets are not increasing");
+        if (n > JAM_MAX_TEMPLATE_ATOMS)
+            return fail(JAM_ETOOBIG, "template " + std::to_string(t) + " has " + std::to_string(n) +
+                                         " atoms, more than JAM_MAX_TEMPLATE_ATOMS");
+    }
+    for (int a = 0; a < A; a++) {
+        int m = d->mode[a];
+        if (m != 0 && m != 1 && m != 3)
+            return fail(JAM_EMODE, "match mode " + std::to_string(m) +
+                                       " is not implemented on the device (implemented: 0, 1, 3)");
+    }
+    // atom types: (mode, names, residue names in order), deduplicated
+    std::map<std::tuple<int, std::vector<uint32_t>, std::vector<uint32_t>>, int> types;
+    std::vector<int> atom_type(A), type_mode, type_name_off{0}, type_rn_off{0};
+    std::vector<uint32_t> type_names, type_rns;
+    for (int a = 0; a < A; a++) {
+        std::vector<uint32_t> nm, rn;
+        for (int k = d->name_off[a]; k < d->name_off[a + 1]; k++) nm.push_back(pack4(d->names + 4 * k));
+        for (int k = d->resname_off[a]; k < d->resname_off[a + 1]; k++)
+            rn.push_back(jam_upper4(pack4(d->resnames + 4 * k)));
+        auto key = std::make_tuple(d->mode[a], nm, rn);
+        auto it = types.find(key);
+        if (it == types.end()) {
+            int id = (int)type_mode.size();
+            types.emplace(key, id);
+            type_mode.push_back(d->mode[a]);
+            type_names.insert(type_names.end(), nm.begin(), nm.end());
+            type_name_off.push_back((int)type_names.size());
+            type_rns.insert(type_rns.end(), rn.begin(), rn.end());
+            type_rn_off.push_back((int)type_rns.size());
+            atom_type[a] = id;
+        } else {
+            atom_type[a] = it->second;
+        }
+    }
+    std::vector<int> atom_off(d->atom_off, d->atom_off + T + 1), chain(A), resseq(A), dist_off(T);
+    std::vector<double> pos(d->pos, d->pos + 3 * (size_t)A), weight(d->weight, d->weight + A), dist;
+    int stride = 1;
+    for (int a = 0; a < A; a++) {
+        chain[a] = (int)(signed char)d->chain2[a];
+        resseq[a] = d->resseq[a];
+    }
+    // template distances: TessAtom_distance, sqrt of squared differences summed x, y, z
+    for (int t = 0; t < T; t++) {
+        int b = atom_off[t], n = atom_off[t + 1] - b;
+        stride = std::max(stride, n);
+        dist_off[t] = (int)dist.size();
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < n; k++) {
+                double s = 0.0;
+                for (int q = 0; q < 3; q++) {
+                    double v = pos[3 * (b + i) + q] - pos[3 * (b + k) + q];
+                    s += v * v;
+                }
+                dist.push_back(i == k ? 0.0 : std::sqrt(s));
+            }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<jam_templates> t(new jam_templates());
+    t->ctx = ctx;
+    t->n_templates = T;
+    t->n_types = (int)type_mode.size();
+    t->stride = stride;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(upload(t->atom_off, atom_off, s));
+    HIP_TRY(upload(t->atom_type, atom_type, s));
+    HIP_TRY(upload(t->chain, chain, s));
+    HIP_TRY(upload(t->resseq, resseq, s));
+    HIP_TRY(upload(t->pos, pos, s));
+    HIP_TRY(upload(t->weight, weight, s));
+    HIP_TRY(upload(t->dist_off, dist_off, s));
+    HIP_TRY(upload(t->dist, dist, s));
+    HIP_TRY(upload(t->type_mode, type_mode, s));
+    HIP_TRY(upload(t->type_name_off, type_name_off, s));
+    HIP_TRY(upload(t->type_names, type_names, s));
+    HIP_TRY(upload(t->type_rn_off, type_rn_off, s));
+    HIP_TRY(upload(t->type_rns, type_rns, s));
+    *out = t.release();
+    return JAM_OK;
+}
+
+extern "C" void jam_templates_free(jam_templates *t)
+{
+    if (!t) return;
+    (void)hipSetDevice(t->ctx->device);
+    delete t;
+}
+
+// ---------------------------------------------------------------------------
+// molecules
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_molecules_create(jam_context *ctx, const jam_molecule_desc *mols, int32_t n,
+                                        jam_molecules **out)
+{
+    if (!ctx || !out || (n > 0 && !mols)) return fail(JAM_EINVAL, "NULL argument");
+    if (n < 0) return fail(JAM_EINVAL, "negative molecule count");
+    *out = nullptr;
+    std::vector<int> atom_off{0}, chain, resseq, perm, res_off{0}, res_begin, res_end;
+    std::vector<double> xyz;
+    std::vector<uint32_t> name, resname, res_key;
+    for (int m = 0; m < n; m++) {
+        const jam_molecule_desc &M = mols[m];
+        if (M.n_atoms < 0) return fail(JAM_EINVAL, "negative atom count");
+        const int N = M.n_atoms;
+        if ((int64_t)atom_off.back() + N > INT32_MAX) return fail(JAM_ETOOBIG, "too many atoms in one batch");
+        xyz.insert(xyz.end(), M.xyz, M.xyz + 3 * (size_t)N);
+        std::vector<uint32_t> upkey(N);
+        for (int i = 0; i < N; i++) {
+            name.push_back(pack4(M.names + 4 * i));
+            uint32_t rn = pack4(M.resnames + 4 * i);
+            resname.push_back(rn);
+            upkey[i] = jam_upper4(rn);
+            chain.push_back((int)(signed char)M.chain2[i]);
+            resseq.push_back(M.resseq[i]);
+        }
+        // residue-name index (ResIndex_create, patches/ResIndex.c.patch:34-119):
+        // atoms grouped by case-insensitive residue name, each group by serial,
+        // ties in input order. Groups are ordered by their packed key value,
+        // the order the device binary search expects; the order of groups has
+        // no effect on candidate lists, only the order inside a group does.
+        std::vector<int> p(N);
+        std::iota(p.begin(), p.end(), 0);
+        std::stable_sort(p.begin(), p.end(), [&](int a, int b) {
+            if (upkey[a] != upkey[b]) return upkey[a] < upkey[b];
+            return M.serial[a] < M.serial[b];
+        });
+        perm.insert(perm.end(), p.begin(), p.end());
+        for (int i = 0; i < N; i++) {
+            if (i == 0 || upkey[p[i]] != upkey[p[i - 1]]) {
+                if (i > 0) res_end.push_back(i);
+                res_key.push_back(upkey[p[i]]);
+                res_begin.push_back(i);
+            }
+        }
+        if (N > 0) res_end.push_back(N);
+        res_off.push_back((int)res_key.size());
+        atom_off.push_back(atom_off.back() + N);
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<jam_molecules> M(new jam_molecules());
+    M->ctx = ctx;
+    M->n_mols = n;
+    M->n_atoms = atom_off.back();
+    hipStream_t s = ctx->stream;
+    HIP_TRY(upload(M->atom_off, atom_off, s));
+    HIP_TRY(upload(M->xyz, xyz, s));
+    HIP_TRY(upload(M->name, name, s));
+    HIP_TRY(upload(M->resname, resname, s));
+    HIP_TRY(upload(M->chain, chain, s));
+    HIP_TRY(upload(M->resseq, resseq, s));
+    HIP_TRY(upload(M->perm, perm, s));
+    HIP_TRY(upload(M->res_off, res_off, s));
+    HIP_TRY(upload(M->res_key, res_key, s));
+    HIP_TRY(upload(M->res_begin, res_begin, s));
+    HIP_TRY(upload(M->res_end, res_end, s));
+    *out = M.release();
+    return JAM_OK;
+}
+
+extern "C" void jam_molecules_free(jam_molecules *m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    delete m;
+}
+
+// ---------------------------------------------------------------------------
+// query
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
+{
+    if (n_sets > c->nsets_cap) {
+        HIP_TRY(c->cnt.reserve(sizeof(int) * (size_t)n_sets));
+        HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
+        c->nsets_cap = n_sets;
+    }
+    const int64_t blk = cap / JAM_WAVE + n_sets + 1;
+    if (cap > c->set_cap) {
+        cap = std::max<int64_t>(cap, 1024);
+        HIP_TRY(c->list.reserve(sizeof(int) * cap));
+        HIP_TRY(c->kidx.reserve(sizeof(int) * cap));
+        HIP_TRY(c->kx.reserve(sizeof(double) * cap));
+        HIP_TRY(c->ky.reserve(sizeof(double) * cap));
+        HIP_TRY(c->kz.reserve(sizeof(double) * cap));
+        HIP_TRY(c->katom.reserve(sizeof(int) * cap));
+        HIP_TRY(c->kres.reserve(sizeof(int) * cap));
+        HIP_TRY(c->kch.reserve(sizeof(int) * cap));
+        c->set_cap = cap;
+    }
+    const int64_t blk_need = c->set_cap / JAM_WAVE + n_sets + 1;
+    if (blk_need > c->blk_cap || blk > c->blk_cap) {
+        c->blk_cap = std::max(blk, blk_need);
+        HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
+    }
+    HIP_TRY(c->total.reserve(sizeof(int64_t)));
+    return JAM_OK;
+}
+
+int32_t grow_matches(jam_context *c, int64_t cap, int stride, int n_items)
+{
+    if (cap > c->match_cap || stride > c->match_stride) {
+        cap = std::max<int64_t>(cap, std::max<int64_t>(c->match_cap, 4096));
+        stride = std::max(stride, c->match_stride);
+        HIP_TRY(c->m_item.reserve(sizeof(int) * cap));
+        HIP_TRY(c->m_seq.reserve(sizeof(int) * cap));
+        HIP_TRY(c->m_atoms.reserve(sizeof(int) * cap * stride));
+        c->match_cap = cap;
+        c->match_stride = stride;
+    }
+    HIP_TRY(c->m_count.reserve(sizeof(unsigned long long)));
+    HIP_TRY(c->item_cm.reserve(sizeof(int) * (size_t)std::max(1, n_items)));
+    return JAM_OK;
+}
+
+int32_t grow_events(jam_context *c, int64_t cap, int stride)
+{
+    if (cap > c->event_cap || stride > c->event_stride) {
+        cap = std::max<int64_t>(cap, std::max<int64_t>(c->event_cap, 1024));
+        stride = std::max(stride, c->event_stride);
+        HIP_TRY(c->e_pair.reserve(sizeof(int) * cap));
+        HIP_TRY(c->e_rank.reserve(sizeof(long long) * cap));
+        HIP_TRY(c->e_flag.reserve(sizeof(int) * cap));
+        HIP_TRY(c->e_rmsd.reserve(sizeof(double) * cap));
+        HIP_TRY(c->e_rot.reserve(sizeof(double) * 9 * cap));
+        HIP_TRY(c->e_c0.reserve(sizeof(double) * 3 * cap));
+        HIP_TRY(c->e_c1.reserve(sizeof(double) * 3 * cap));
+        HIP_TRY(c->e_atoms.reserve(sizeof(int) * cap * stride));
+        c->event_cap = cap;
+        c->event_stride = stride;
+    }
+    HIP_TRY(c->e_count.reserve(sizeof(unsigned long long)));
+    return JAM_OK;
+}
+
+__global__ void k_total(const int *cnt, const int64_t *off, int n, int64_t *total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
+}
+
+}  // namespace
+
+extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_molecules *ms,
+                             const jam_params *p, jam_result **out)
+{
+    if (!c || !tp || !ms || !p || !out) return fail(JAM_EINVAL, "NULL argument");
+    if (tp->ctx != c || ms->ctx != c) return fail(JAM_EINVAL, "objects belong to another context");
+    if (p->ignore_chain < 0 || p->ignore_chain > 2) return fail(JAM_EINVAL, "invalid ignore_chain");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(c->device));
+    std::unique_ptr<jam_result> R(new jam_result());
+    const int T = tp->n_templates, M = ms->n_mols, U = tp->n_types;
+    const int64_t n_pairs64 = (int64_t)T * M;
+    if (n_pairs64 > INT32_MAX / 64) return fail(JAM_ETOOBIG, "too many (template, molecule) pairs");
+    const int n_pairs = (int)n_pairs64;
+    R->n_pairs = n_pairs;
+    R->stride = tp->stride;
+    R->pair_cm.assign(n_pairs, 0);
+    if (n_pairs == 0 || U == 0) {
+        *out = R.release();
+        return JAM_OK;
+    }
+    const int n_sets = M * U;
+    hipStream_t s = c->stream;
+
+    QueryParams qp;
+    qp.rmsd_threshold = p->rmsd_threshold;
+    qp.distance_cutoff = p->distance_cutoff;
+    qp.max_dynamic_distance = p->max_dynamic_distance;
+    qp.ignore_chain = p->ignore_chain;
+    qp.reorder = p->reorder ? 1 : 0;
+    // level-0 ranges per pair: enough waves to fill the device on small queries
+    qp.items_per_pair = (int)std::min<int64_t>(16, std::max<int64_t>(1, 4096 / n_pairs));
+    const int n_items = n_pairs * qp.items_per_pair;
+    const int stride = tp->stride;
+
+    int32_t rc;
+    if ((rc = grow_sets(c, std::max<int64_t>(c->set_cap, 1 << 16), n_sets)) != JAM_OK) return rc;
+    if ((rc = grow_matches(c, std::max<int64_t>(c->match_cap, 1 << 14), stride, n_items)) != JAM_OK) return rc;
+    if ((rc = grow_events(c, std::max<int64_t>(c->event_cap, 1 << 12), stride)) != JAM_OK) return rc;
+
+    DevMols dm = ms->dev();
+    DevTpls dt = tp->dev();
+    size_t scan_bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int *)nullptr, (int64_t *)nullptr,
+                                             n_sets, s));
+    HIP_TRY(c->scan_tmp.reserve(scan_bytes));
+
+    float ms_k[4] = {0, 0, 0, 0};
+    for (int attempt = 0; attempt < 4; attempt++) {
+        DevSets st;
+        st.n_sets = n_sets;
+        st.cap = c->set_cap;
+        st.blk_cap = c->blk_cap;
+        st.cnt = c->cnt.as<int>();
+        st.off = c->off.as<int64_t>();
+        st.list = c->list.as<int>();
+        st.kidx = c->kidx.as<int>();
+        st.kx = c->kx.as<double>();
+        st.ky = c->ky.as<double>();
+        st.kz = c->kz.as<double>();
+        st.katom = c->katom.as<int>();
+        st.kres = c->kres.as<int>();
+        st.kch = c->kch.as<int>();
+        st.bbox = c->bbox.as<double>();
+        st.total = c->total.as<int64_t>();
+        DevMatches mt;
+        mt.cap = c->match_cap;
+        mt.stride = c->match_stride;
+        mt.item = c->m_item.as<int>();
+        mt.seq = c->m_seq.as<int>();
+        mt.atoms = c->m_atoms.as<int>();
+        mt.count = c->m_count.as<unsigned long long>();
+        mt.item_cm = c->item_cm.as<int>();
+        DevEvents ev;
+        ev.cap = c->event_cap;
+        ev.stride = stride;
+        ev.pair = c->e_pair.as<int>();
+        ev.rank = c->e_rank.as<long long>();
+        ev.flag = c->e_flag.as<int>();
+        ev.rmsd = c->e_rmsd.as<double>();
+        ev.rot = c->e_rot.as<double>();
+        ev.c0 = c->e_c0.as<double>();
+        ev.c1 = c->e_c1.as<double>();
+        ev.atoms = c->e_atoms.as<int>();
+        ev.count = c->e_count.as<unsigned long long>();
+
+        HIP_TRY(hipMemsetAsync(mt.count, 0, sizeof(unsigned long long), s));
+        HIP_TRY(hipMemsetAsync(ev.count, 0, sizeof(unsigned long long), s));
+        const int wpb = 4;  // waves per block of the per-set kernels
+        const int set_blocks = (n_sets + wpb - 1) / wpb;
+        HIP_TRY(hipEventRecord(c->ev[0], s));
+        hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_bytes, st.cnt, st.off, n_sets, s));
+        hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
+        hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[1], s));
+        hipLaunchKernelGGL(jam::k_kdorder, dim3(n_sets), dim3(64), 0, s, dm, dt, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[2], s));
+        if (stride <= 8)
+            hipLaunchKernelGGL(jam::k_join<8>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
+        else if (stride <= 16)
+            hipLaunchKernelGGL(jam::k_join<16>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
+        else
+            hipLaunchKernelGGL(jam::k_join<32>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[3], s));
+        unsigned long long counts[2];
+        int64_t total = 0;
+        // matches counted; superposition over min(count, cap) needs the count on the host
+        HIP_TRY(hipMemcpyAsync(&counts[0], mt.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&total, st.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (total > c->set_cap) {
+            if ((rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
+            continue;
+        }
+        if ((int64_t)counts[0] > c->match_cap) {
+            if ((rc = grow_matches(c, (int64_t)counts[0], stride, n_items)) != JAM_OK) return rc;
+            continue;
+        }
+        const long long nm = (long long)counts[0];
+        if (nm > 0) {
+            const int tb = 256;
+            hipLaunchKernelGGL(jam::k_super, dim3((unsigned)((nm + tb - 1) / tb)), dim3(tb), 0, s, dm, dt, mt, ev,
+                               qp, nm);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(c->ev[4], s));
+        HIP_TRY(hipMemcpyAsync(&counts[1], ev.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if ((int64_t)counts[1] > c->event_cap) {
+            if ((rc = grow_events(c, (int64_t)counts[1], stride)) != JAM_OK) return rc;
+            continue;
+        }
+        HIP_TRY(hipEventElapsedTime(&ms_k[0], c->ev[0], c->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&ms_k[1], c->ev[1], c->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&ms_k[2], c->ev[2], c->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&ms_k[3], c->ev[3], c->ev[4]));
+        for (int k = 0; k < 4; k++) R->ms[k] = ms_k[k];
+        float all = 0;
+        HIP_TRY(hipEventElapsedTime(&all, c->ev[0], c->ev[4]));
+        R->ms[4] = all;
+        // copy out
+        const int64_t ne = (int64_t)counts[1];
+        R->pair.resize(ne);
+        R->rank.resize(ne);
+        R->flag.resize(ne);
+        R->rmsd.resize(ne);
+        R->rot.resize(9 * ne);
+        R->c0.resize(3 * ne);
+        R->c1.resize(3 * ne);
+        R->atoms.resize(ne * stride);
+        std::vector<int> icm(n_items);
+        if (ne > 0) {
+            HIP_TRY(hipMemcpyAsync(R->pair.data(), ev.pair, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->rank.data(), ev.rank, sizeof(long long) * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->flag.data(), ev.flag, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->rmsd.data(), ev.rmsd, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->rot.data(), ev.rot, sizeof(double) * 9 * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->c0.data(), ev.c0, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->c1.data(), ev.c1, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(R->atoms.data(), ev.atoms, sizeof(int) * ne * stride, hipMemcpyDeviceToHost, s));
+        }
+        HIP_TRY(hipMemcpyAsync(icm.data(), mt.item_cm, sizeof(int) * n_items, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < n_items; i++) R->pair_cm[i / qp.items_per_pair] += icm[i];
+        *out = R.release();
+        return JAM_OK;
+    }
+    return fail(JAM_EDEVICE, "query did not converge after growing its buffers");
+}
+
+// ---------------------------------------------------------------------------
+// results
+// ---------------------------------------------------------------------------
+
+extern "C" int64_t jam_result_n_events(const jam_result *r) { return r ? (int64_t)r->rmsd.size() : 0; }
+
+extern "C" int32_t jam_result_stride(const jam_result *r) { return r ? r->stride : 0; }
+
+extern "C" int32_t jam_result_n_pairs(const jam_result *r) { return r ? r->n_pairs : 0; }
+
+extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
+                                   double *rmsd, double *rot, double *c0, double *c1, int32_t *atoms)
+{
+    if (!r) return fail(JAM_EINVAL, "NULL result");
+    const size_t n = r->rmsd.size();
+    if (pair) std::memcpy(pair, r->pair.data(), sizeof(int32_t) * n);
+    if (rank) std::memcpy(rank, r->rank.data(), sizeof(int64_t) * n);
+    if (flag) std::memcpy(flag, r->flag.data(), sizeof(int32_t) * n);
+    if (rmsd) std::memcpy(rmsd, r->rmsd.data(), sizeof(double) * n);
+    if (rot) std::memcpy(rot, r->rot.data(), sizeof(double) * 9 * n);
+    if (c0) std::memcpy(c0, r->c0.data(), sizeof(double) * 3 * n);
+    if (c1) std::memcpy(c1, r->c1.data(), sizeof(double) * 3 * n);
+    if (atoms) std::memcpy(atoms, r->atoms.data(), sizeof(int32_t) * r->atoms.size());
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts)
+{
+    if (!r || !counts) return fail(JAM_EINVAL, "NULL argument");
+    std::memcpy(counts, r->pair_cm.data(), sizeof(int64_t) * r->pair_cm.size());
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_result_timings(const jam_result *r, double *ms5)
+{
+    if (!r || !ms5) return fail(JAM_EINVAL, "NULL argument");
+    std::memcpy(ms5, r->ms, sizeof(r->ms));
+    return JAM_OK;
+}
+
+extern "C" void jam_result_free(jam_result *r) { delete r; }
+
+// ---------------------------------------------------------------------------
+// standalone superposition
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_superpose_batch(jam_context *c, const double *a, const double *b, int32_t k,
+                                       int64_t n_sets, double *rmsd, double *rot, double *c0, double *c1)
+{
+    if (!c || !a || !b || !rmsd || !rot || !c0 || !c1) return fail(JAM_EINVAL, "NULL argument");
+    if (k < 1 || n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets == 0) return JAM_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const size_t pts = (size_t)n_sets * k * 3;
+    HIP_TRY(c->sa.reserve(sizeof(double) * pts));
+    HIP_TRY(c->sb.reserve(sizeof(double) * pts));
+    HIP_TRY(c->srmsd.reserve(sizeof(double) * n_sets));
+    HIP_TRY(c->srot.reserve(sizeof(double) * 9 * n_sets));
+    HIP_TRY(c->sc0.reserve(sizeof(double) * 3 * n_sets));
+    HIP_TRY(c->sc1.reserve(sizeof(double) * 3 * n_sets));
+    HIP_TRY(hipMemcpyAsync(c->sa.ptr, a, sizeof(double) * pts, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->sb.ptr, b, sizeof(double) * pts, hipMemcpyHostToDevice, s));
+    const int tb = 256;
+    hipLaunchKernelGGL(jam::k_superpose_batch, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0, s,
+                       c->sa.as<double>(), c->sb.as<double>(), k, (long long)n_sets, c->srmsd.as<double>(),
+                       c->srot.as<double>(), c->sc0.as<double>(), c->sc1.as<double>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(rmsd, c->srmsd.ptr, sizeof(double) * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(rot, c->srot.ptr, sizeof(double) * 9 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c0, c->sc0.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c1, c->sc1.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}

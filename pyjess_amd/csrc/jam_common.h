@@ -1,0 +1,111 @@
+// jam_common.h -- device-side layouts shared by the kernels and the host API.
+//
+// HBM layout (all arrays device resident, structure of arrays):
+//
+//   molecules   concatenated atoms of every molecule of a batch; molecule m owns
+//               atoms [atom_off[m], atom_off[m+1]); per atom xyz (AoS, 24 B),
+//               raw 4-byte name and residue name, chainID2, resSeq; plus the
+//               residue-name index: perm (atoms sorted by upper-cased residue
+//               name, then serial), bucket keys and bucket ranges.
+//   templates   per template atom: atom-type id, chainID2, resSeq, position,
+//               distance weight; per template an n*n distance matrix; per
+//               atom type (deduplicated across templates): mode, names,
+//               upper-cased residue names in template order.
+//   candidate   one set per (molecule, atom type): the candidate list in
+//   sets        reference order, and a copy in kd-tree DFS leaf order as SoA
+//               (x, y, z, atom, resSeq, chain) with one bounding box per 64
+//               consecutive entries.
+#pragma once
+
+#include <stdint.h>
+
+#define JAM_WAVE 64
+
+struct DevMols {
+    int n_mols;
+    const int *atom_off;      // M+1
+    const double *xyz;        // 3 per atom
+    const uint32_t *name;     // raw packed name, byte 0 = first character
+    const uint32_t *resname;  // raw packed residue name
+    const int *chain;         // chainID2 (signed char value)
+    const int *resseq;
+    const int *perm;          // per molecule: local atom indices, residue index order
+    const int *res_off;       // M+1: bucket range of molecule m
+    const uint32_t *res_key;  // per bucket: upper-cased packed residue name
+    const int *res_begin;     // per bucket: first position in perm (molecule local)
+    const int *res_end;       // per bucket: one past the last position
+};
+
+struct DevTpls {
+    int n_templates;
+    int n_types;
+    int stride;               // largest template (atoms per match row)
+    const int *atom_off;      // T+1
+    const int *atom_type;     // per atom
+    const int *chain;         // per atom
+    const int *resseq;        // per atom
+    const double *pos;        // 3 per atom
+    const double *weight;     // per atom
+    const int *dist_off;      // T: offset of the n*n distance matrix
+    const double *dist;
+    const int *type_mode;     // U
+    const int *type_name_off; // U+1
+    const uint32_t *type_names;  // raw packed names
+    const int *type_rn_off;   // U+1
+    const uint32_t *type_rns;    // upper-cased packed residue names, template order
+};
+
+struct DevSets {
+    int n_sets;               // M * U
+    int64_t cap;              // capacity of the entry arrays
+    int64_t blk_cap;          // capacity of the block boxes
+    int *cnt;                 // per set
+    int64_t *off;             // per set: exclusive scan of cnt
+    int *list;                // reference-order candidate list (molecule-local atom)
+    int *kidx;                // scratch: kd permutation
+    double *kx, *ky, *kz;     // kd DFS order
+    int *katom, *kres, *kch;
+    double *bbox;             // 6 per block: lo xyz, hi xyz
+    int64_t *total;           // device scalar: sum of cnt
+};
+
+struct DevMatches {
+    int64_t cap;
+    int stride;
+    int *item;                // work item that found the match
+    int *seq;                 // rank within the work item
+    int *atoms;               // stride per match: molecule-local atom per template atom
+    unsigned long long *count;  // device scalar
+    int *item_cm;             // complete matches per work item
+};
+
+struct DevEvents {
+    int64_t cap;
+    int stride;
+    int *pair;
+    long long *rank;
+    int *flag;
+    double *rmsd, *rot, *c0, *c1;
+    int *atoms;
+    unsigned long long *count;  // device scalar
+};
+
+struct QueryParams {
+    double rmsd_threshold;
+    double distance_cutoff;
+    double max_dynamic_distance;
+    int ignore_chain;
+    int reorder;
+    int items_per_pair;       // G: level-0 ranges per (template, molecule) pair
+};
+
+__host__ __device__ static inline uint32_t jam_upper4(uint32_t v)
+{
+    uint32_t r = 0;
+    for (int i = 0; i < 4; i++) {
+        uint32_t c = (v >> (8 * i)) & 0xFFu;
+        if (c >= 'a' && c <= 'z') c &= ~0x20u;
+        r |= c << (8 * i);
+    }
+    return r;
+}
