@@ -153,6 +153,10 @@ int32_t jam_superpose_batch(jam_context *ctx, const double *a, const double *b, 
 
 /* ---- misc ------------------------------------------------------------- */
 
+/* Waits until all work queued on the context's device has finished
+ * (hipDeviceSynchronize). No reference counterpart; used to bracket timings. */
+int32_t jam_synchronize(jam_context *ctx);
+
 const char *jam_last_error(void);
 /* Library version string. */
 const char *jam_version(void);

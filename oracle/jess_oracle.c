@@ -669,6 +669,9 @@ typedef struct {
     int32_t *scratch;
     char *done;
     double *pa, *pb;
+    /* work counters (jo_work): sum over opened levels L of L * |set inside join box|,
+     * opened levels, complete matches, sum of their sizes */
+    int64_t w_inbox, w_levels, w_complete, w_complete_atoms;
 } scanner;
 
 static void scanner_free(scanner *S)
@@ -699,6 +702,7 @@ static void scanner_free(scanner *S)
 static int scanner_init(scanner *S, int32_t nmax, int32_t natoms, int32_t nres)
 {
     memset(S, 0, sizeof(*S));
+    S->w_inbox = -1;
     S->nmax = nmax;
     S->setcap = natoms > 0 ? natoms : 1;
     S->set = (int32_t **)calloc(nmax, sizeof(int32_t *));
@@ -799,6 +803,13 @@ static int sink_stream(void *ctx, scanner *S, const jo_mol *m, const jo_tpls *T,
     return emit(S, m, T, t, n, (jo_stream *)ctx);
 }
 
+static int sink_none(void *ctx, scanner *S, const jo_mol *m, const jo_tpls *T, int32_t t, int32_t n)
+{
+    S->w_complete++;
+    S->w_complete_atoms += n;
+    return 0;
+}
+
 /* Builds the join of level L: one annulus per earlier level, centred on the
  * atom placed there (patches/Scanner.c.patch:274-294), then starts the query. */
 static void open_level(scanner *S, const jo_mol *m, const jo_tpls *T, int32_t base,
@@ -810,6 +821,20 @@ static void open_level(scanner *S, const jo_mol *m, const jo_tpls *T, int32_t ba
         annulus_set(&S->ann[L][j], m->xyz + 3 * S->atom[S->ord[j]], lo, hi);
     }
     kdq_start(S, L);
+    S->w_levels++;
+    if (S->w_inbox >= 0) {
+        const double *B = S->box[L];
+        const int32_t k = S->ord[L];
+        int64_t in = 0;
+        for (int32_t i = 0; i < S->cnt[k]; i++) {
+            const double *x = m->xyz + 3 * S->set[k][i];
+            int ok = 1;
+            for (int d = 0; d < 3; d++)
+                if (x[d] < B[d] || x[d] > B[3 + d]) ok = 0;
+            in += ok;
+        }
+        S->w_inbox += (int64_t)L * in;
+    }
 }
 
 /* Scanner_reuse + Scanner_next for one template (patches/Scanner.c.patch:115-203,232-334).
@@ -894,6 +919,28 @@ int32_t jo_query(const jo_mol *mol, const jo_tpls *tpls, const jo_params *p,
     int32_t rc = scanner_init(&S, max_template_atoms(tpls), mol->n, R.nres);
     for (int32_t i = 0; rc == JO_OK && i < n_scan; i++)
         rc = scan_template(&S, mol, &R, tpls, scan_order[i], p, sink_stream, *out);
+    scanner_free(&S);
+    resindex_free(&R);
+    return rc;
+}
+
+/* Algorithmic work of a query, counted on the reference algorithm: out[0] =
+ * sum over opened levels L of L * (candidates of the level's set inside the
+ * join box), out[1] = opened levels, out[2] = complete matches, out[3] = sum
+ * of their atom counts. */
+int32_t jo_work(const jo_mol *mol, const jo_tpls *tpls, const jo_params *p, int64_t *out)
+{
+    resindex R;
+    if (resindex_build(mol, &R) != JO_OK) { resindex_free(&R); return JO_ENOMEM; }
+    scanner S;
+    int32_t rc = scanner_init(&S, max_template_atoms(tpls), mol->n, R.nres);
+    S.w_inbox = 0;
+    for (int32_t t = 0; rc == JO_OK && t < tpls->n_templates; t++)
+        rc = scan_template(&S, mol, &R, tpls, t, p, sink_none, NULL);
+    out[0] = S.w_inbox;
+    out[1] = S.w_levels;
+    out[2] = S.w_complete;
+    out[3] = S.w_complete_atoms;
     scanner_free(&S);
     resindex_free(&R);
     return rc;

@@ -90,6 +90,9 @@ int32_t jo_candidates(const jo_mol *mol, const jo_tpls *tpls, int32_t t, int32_t
 /* kd leaf order of a candidate set (positions into the set list). */
 int32_t jo_kd_order(const jo_mol *mol, const int32_t *set_atoms, int32_t n, int32_t *out);
 
+/* Algorithmic work of a query (see jess_oracle.c). out: 4 counters. */
+int32_t jo_work(const jo_mol *mol, const jo_tpls *tpls, const jo_params *p, int64_t *out);
+
 /* Multi-threaded batch: every (template, molecule) pair; per-pair counts of
  * complete matches and of hits (rmsd <= threshold, finite rotation), and an
  * order-independent 64-bit digest of the hit set. Used as the CPU baseline. */

@@ -110,6 +110,8 @@ def lib():
         L.jo_batch.argtypes = [P(JoMol), ctypes.c_int32, P(JoTpls), P(JoParams), ctypes.c_double,
                                ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.jo_batch.restype = ctypes.c_int32
+        L.jo_work.argtypes = [P(JoMol), P(JoTpls), P(JoParams), ctypes.c_void_p]
+        L.jo_work.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -364,3 +366,20 @@ def batch(mol_views: Sequence[MolView], templates, rmsd_threshold, distance_cuto
     if rc != 0:
         raise RuntimeError(f"oracle batch failed with code {rc}")
     return cm[:npairs], hc[:npairs], dg[:npairs]
+
+
+def work(molecule, templates, distance_cutoff, max_dynamic_distance, ignore_chain=None, reorder=True):
+    """Algorithmic work of one query on the reference algorithm: dict with
+    ``inbox_tests`` (sum over opened levels L of L x candidates inside the join
+    box), ``levels`` (opened levels), ``complete`` (complete matches) and
+    ``complete_atoms`` (sum of their sizes)."""
+    L = lib()
+    mv = MolView.of(molecule) if not isinstance(molecule, MolView) else molecule
+    tv = TplView(templates) if not isinstance(templates, TplView) else templates
+    out = np.zeros(4, dtype=np.int64)
+    p = params(distance_cutoff, max_dynamic_distance, ignore_chain, reorder)
+    rc = L.jo_work(ctypes.byref(mv.c), ctypes.byref(tv.c), ctypes.byref(p), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle work count failed with code {rc}")
+    return dict(inbox_tests=int(out[0]), levels=int(out[1]), complete=int(out[2]),
+                complete_atoms=int(out[3]))

@@ -42,23 +42,66 @@ def _warn_nan():
     warnings.warn("Jess returned a superposition matrix with NaN values", UserWarning, stacklevel=3)
 
 
-class Hit:
-    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``)."""
+class _Block:
+    """Row-major result arrays shared by the hits of one query."""
 
-    __slots__ = ("rmsd", "_rotation", "_centre", "_atoms", "_template", "_molecule")
+    __slots__ = ("rotation", "c0", "c1", "atoms")
+
+    def __init__(self, rotation, c0, c1, atoms):
+        self.rotation = rotation
+        self.c0 = c0
+        self.c1 = c1
+        self.atoms = atoms
+
+
+class Hit:
+    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``).
+
+    A hit refers to a row of its query's result arrays; rotation, centroids
+    and matched atoms are read from there when asked for."""
+
+    __slots__ = ("rmsd", "_block", "_row", "_template", "_molecule")
 
     def __init__(self, rmsd: float, rotation: Sequence[float], c0: Sequence[float], c1: Sequence[float],
-                 atoms: List[Atom], template: Template, molecule: Molecule):
+                 atoms, template: Template, molecule: Molecule):
         self.rmsd = float(rmsd)
-        self._rotation = [float(v) for v in rotation]
-        self._centre = ([float(v) for v in c0], [float(v) for v in c1])
-        self._atoms = atoms
+        self._block = _Block(np.asarray(rotation, dtype=np.float64).reshape(1, 9),
+                             np.asarray(c0, dtype=np.float64).reshape(1, 3),
+                             np.asarray(c1, dtype=np.float64).reshape(1, 3),
+                             np.asarray(list(atoms), dtype=np.int64).reshape(1, -1))
+        self._row = 0
         self._template = template
         self._molecule = molecule
 
+    @classmethod
+    def _at(cls, rmsd: float, block: _Block, row: int, template: Template, molecule: Molecule) -> "Hit":
+        h = cls.__new__(cls)
+        h.rmsd = rmsd
+        h._block = block
+        h._row = row
+        h._template = template
+        h._molecule = molecule
+        return h
+
+    @property
+    def _rotation(self) -> List[float]:
+        return self._block.rotation[self._row].tolist()
+
+    @property
+    def _centre(self):
+        return self._block.c0[self._row].tolist(), self._block.c1[self._row].tolist()
+
+    @property
+    def _index(self) -> List[int]:
+        return self._block.atoms[self._row, :len(self._template)].tolist()
+
+    @property
+    def _atoms(self) -> List[Atom]:
+        return [self._molecule._atom(k) for k in self._index]
+
     def __reduce__(self):
         return _restore_hit, (self.rmsd, self._rotation, self._centre[0], self._centre[1],
-                              list(self._atoms), self._template, self._molecule)
+                              list(self._index), self._template, self._molecule)
 
     def __repr__(self):
         return f"<Hit rmsd={self.rmsd!r} template={self._template.id!r}>"
@@ -109,7 +152,7 @@ class Hit:
 
     def atoms(self, transform: bool = True) -> List[Atom]:
         if not transform:
-            return [a.copy() for a in self._atoms]
+            return self._atoms
         trans = self.transformation
         return [a.transform(trans) for a in self._atoms]
 
@@ -246,33 +289,58 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
     tpl_of = res.pair % T
     # molecule, then scan position of the template (reverse order), then rank
     order = np.lexsort((res.rank, (T - 1) - tpl_of, mol_of))
-    i = 0
+    if not best_match and mc == -1:
+        # no policy can drop an event: every event is reported, in order
+        chosen = order
+    else:
+        chosen = _select(res, order, best_match, mc)
+    block = _Block(res.rotation, res.c0, res.c1, res.atoms)
+    rmsd = res.rmsd.tolist()
+    flag = res.flag.tolist()
+    mol_l = mol_of.tolist()
+    tpl_l = tpl_of.tolist()
+    templates = jess._templates
+    make = Hit._at
+    for e in (chosen.tolist() if isinstance(chosen, np.ndarray) else chosen):
+        mi = mol_l[e]
+        if flag[e] == 2:
+            out[mi].append(None)
+            continue
+        out[mi].append(make(rmsd[e], block, e, templates[tpl_l[e]], molecules[mi]))
+    return out
+
+
+def _select(res, order, best_match: bool, mc: int) -> list:
+    """Events that pyjess reaches under ``best_match`` / ``max_candidates``, in
+    order: hits it yields and NaN-rotation matches it warns about."""
+    T_pairs = res.pair
+    ranks = res.rank.tolist()
+    flags = res.flag.tolist()
+    rmsds = res.rmsd.tolist()
+    pairs = T_pairs.tolist()
+    chosen = []
     n = len(order)
+    i = 0
+    order = order.tolist()
     while i < n:
-        e0 = order[i]
-        p = res.pair[e0]
+        p = pairs[order[i]]
         j = i
-        while j < n and res.pair[order[j]] == p:
+        while j < n and pairs[order[j]] == p:
             j += 1
         events = order[i:j]
         i = j
-        mi = int(mol_of[e0])
-        t = int(tpl_of[e0])
-        mol = molecules[mi]
-        chosen = []
         if best_match:
             best = None
             best_rmsd = math.inf
             for e in events:
-                rank = int(res.rank[e])
+                rank = ranks[e]
                 if mc != -1 and rank > mc + 1:
                     break
-                r = float(res.rmsd[e])
-                if r < best_rmsd:
-                    if res.flag[e] == 2:
-                        chosen.append(None)
+                if rmsds[e] < best_rmsd:
+                    if flags[e] == 2:
+                        chosen.append(e)            # warning only
                     else:
-                        best, best_rmsd = e, r
+                        best, best_rmsd = e, rmsds[e]
                 if mc != -1 and rank == mc + 1:
                     break
             if best is not None:
@@ -280,25 +348,15 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         else:
             seg = 0
             for e in events:
-                c = int(res.rank[e]) - seg
+                c = ranks[e] - seg
                 if mc != -1 and c > mc + 1:
                     break
-                if res.flag[e] == 2:
-                    chosen.append(None)
-                else:
-                    chosen.append(e)
-                    seg = int(res.rank[e]) + 1
+                chosen.append(e)
+                if flags[e] != 2:
+                    seg = ranks[e] + 1
                 if mc != -1 and c == mc + 1:
                     break
-        tpl = jess._templates[t]
-        for e in chosen:
-            if e is None:
-                out[mi].append(None)
-                continue
-            n_atoms = len(tpl)
-            atoms = [mol._atom(int(k)) for k in res.atoms[e, :n_atoms]]
-            out[mi].append(Hit(res.rmsd[e], res.rotation[e], res.c0[e], res.c1[e], atoms, tpl, mol))
-    return out
+    return chosen
 
 
 class Jess:
@@ -310,6 +368,7 @@ class Jess:
             if not isinstance(t, Template):
                 raise TypeError(f"expected Template, found {type(t).__name__}")
         self._tset = None
+        self._last_result = None
 
     def __len__(self) -> int:
         return len(self._templates)
@@ -354,9 +413,11 @@ class Jess:
             if mset is None:
                 mset = mol._device[id(ctx)] = _native.MoleculeSet(ctx, [mol])
         else:
-            mset = _native.MoleculeSet(ctx, molecules)
-        return _native.query(tset, mset, rmsd_threshold, distance_cutoff, max_dynamic_distance,
-                             ignore_chain, reorder)
+            mset = _batch_cache(ctx, molecules)
+        res = _native.query(tset, mset, rmsd_threshold, distance_cutoff, max_dynamic_distance,
+                            ignore_chain, reorder)
+        self._last_result = res     # kernel timings and match counts of the latest query
+        return res
 
     def query(self, molecule: Molecule, rmsd_threshold: float, distance_cutoff: float,
               max_dynamic_distance: float, *, max_candidates=None, ignore_chain=None,
@@ -396,6 +457,22 @@ class Jess:
                 _warn_nan()
             hits.append([h for h in lst if h is not None])
         return hits
+
+
+_last_batch = [None, None, None]   # (context, molecules, device copy) of the latest batch
+
+
+def _batch_cache(ctx, molecules: Sequence[Molecule]) -> "_native.MoleculeSet":
+    """Device copy of a batch of molecules; the latest one is kept, so a batch
+    queried again (against other templates, or repeatedly) is not re-uploaded.
+    The cache holds the molecule objects, which keeps their identities valid."""
+    ctx0, mols0, mset0 = _last_batch
+    if ctx0 is ctx and mols0 is not None and len(mols0) == len(molecules) and \
+            all(a is b for a, b in zip(mols0, molecules)):
+        return mset0
+    mset = _native.MoleculeSet(ctx, molecules)
+    _last_batch[:] = [ctx, list(molecules), mset]
+    return mset
 
 
 def _normalize_ignore_chain(ignore_chain):

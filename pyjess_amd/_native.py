@@ -28,7 +28,7 @@ _SYMBOLS = (
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
     "jam_result_timings", "jam_result_free", "jam_superpose_batch", "jam_last_error",
-    "jam_version", "jam_device_available",
+    "jam_version", "jam_device_available", "jam_synchronize",
 )
 
 
@@ -139,6 +139,8 @@ def lib():
         L.jam_version.restype = ctypes.c_char_p
         L.jam_device_available.argtypes = []
         L.jam_device_available.restype = i32
+        L.jam_synchronize.argtypes = [vp]
+        L.jam_synchronize.restype = i32
         _lib = L
         return L
 
@@ -336,6 +338,11 @@ def superpose_batch(a: np.ndarray, b: np.ndarray, device: int = 0):
         _check(lib().jam_superpose_batch(ctx.handle, _ptr(a), _ptr(b), int(k), int(n), _ptr(rmsd),
                                          _ptr(rot), _ptr(c0), _ptr(c1)))
     return rmsd[:n], rot[:n].reshape(n, 3, 3), c0[:n], c1[:n]
+
+
+def synchronize(device: int = 0) -> None:
+    """Waits for all work on the device (bracketing timed regions)."""
+    _check(lib().jam_synchronize(context(device).handle))
 
 
 def device_available() -> bool:

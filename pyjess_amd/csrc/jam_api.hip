@@ -96,7 +96,7 @@ struct jam_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     // per-query workspace
-    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total;
+    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr;
     DBuf m_item, m_seq, m_atoms, m_count, item_cm;
     DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms, e_count;
     DBuf scan_tmp;
@@ -211,6 +211,14 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
     *out = c.release();
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_synchronize(jam_context *ctx)
+{
+    if (!ctx) return fail(JAM_EINVAL, "NULL context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
     return JAM_OK;
 }
 
@@ -429,6 +437,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->katom.reserve(sizeof(int) * cap));
         HIP_TRY(c->kres.reserve(sizeof(int) * cap));
         HIP_TRY(c->kch.reserve(sizeof(int) * cap));
+        HIP_TRY(c->kscr.reserve((size_t)KD_SCRATCH_BYTES_PER_ENTRY * cap));
         c->set_cap = cap;
     }
     const int64_t blk_need = c->set_cap / JAM_WAVE + n_sets + 1;
@@ -580,7 +589,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
-        hipLaunchKernelGGL(jam::k_kdorder, dim3(n_sets), dim3(64), 0, s, dm, dt, st);
+        hipLaunchKernelGGL(jam::k_kdorder, dim3(n_sets), dim3(KD_THREADS), 0, s, dm, dt, st,
+                           c->kscr.as<unsigned char>());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[2], s));
         if (stride <= 8)

@@ -4,11 +4,12 @@
 //                  reference order (TessTemplate_candidates,
 //                  patches/TessTemplate.c.patch:173-235), two passes (count,
 //                  fill) with ballot compaction.
-//   k_kdorder      wave per candidate set: the leaf order of the reference's
-//                  QuickSelect kd-tree (patches/KdTree.c.patch:348-451,
-//                  patches/qselect.h.patch:50-116), reversed because its DFS
-//                  visits right children first; writes the set as SoA in that
-//                  order plus one bounding box per 64 entries.
+//   k_kdorder      workgroup per candidate set: the leaf order of the
+//                  reference's QuickSelect kd-tree (patches/KdTree.c.patch:348-451,
+//                  patches/qselect.h.patch:50-116) built in parallel in LDS,
+//                  reversed because its DFS visits right children first; writes
+//                  the set as SoA in that order plus one bounding box per 64
+//                  entries.
 //   k_join         wave per (template, molecule, level-0 range): the Scanner
 //                  DFS (patches/Scanner.c.patch:232-334) with an explicit
 //                  per-level stack in LDS; each level scans 64-entry blocks of
@@ -129,110 +130,312 @@ __global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevS
 // kd leaf order
 // ---------------------------------------------------------------------------
 
-struct KdCmp {
-    const double *xyz;
-    const int *list;
-    const int *idx;
-    int abase;
-    int axis;
-    __device__ double at(size_t i) const { return xyz[3 * (size_t)(abase + list[idx[i]]) + axis]; }
-    __device__ int cmp(size_t i, size_t j) const
-    {
-        double x = at(i), y = at(j);
-        return x < y ? -1 : (x > y ? 1 : 0);
-    }
+// ---- exact parallel arrangement ------------------------------------------
+//
+// The reference builds the tree by QuickSelect of the median of every range
+// (KdTreeNode_create with qselect_r, patches/KdTree.c.patch:348-451,
+// patches/qselect.h.patch:50-116): median-of-three pivot at
+// mid = (l + r + 1) / 2, moved to r, Lomuto partition sending "<= pivot" to
+// the left, pivot swapped into place, repeat on the side holding the median.
+// Where equal coordinates land depends on that exact swap sequence, and it
+// decides the leaf order, so the kernel reproduces it, in parallel: all ranges
+// of one tree level run their QuickSelect together, one partition round at a
+// time, and each Lomuto partition is computed without its serial scan:
+//   * the "<= pivot" elements keep their order: a stable compaction;
+//   * the others move as a FIFO queue: each is pushed when scanned, and each
+//     "<=" element scanned after the first ">" one pops the front and pushes
+//     it back; push number t of such a re-push is the q-th pop, i.e. push
+//     number q < t. Pointer jumping resolves every push to its element. The
+//     final queue fills [l+k+1, r-1] from its second element on, its front
+//     goes to r, the pivot to l+k.
+
+#define KD_THREADS 256
+#define KD_CAP 1024          // largest set arranged in LDS
+// per set entry: 3 coordinates and 9 ints of work arrays (60 bytes), for sets
+// too large for LDS
+#define KD_SCRATCH_BYTES_PER_ENTRY 64
+
+struct KdWork {
+    double *c[3];            // coordinates by element
+    int *arr[2];             // arrangement, double buffered
+    int *pre1, *pre2;        // exclusive prefixes: "<=" flags, re-push flags
+    int *src;                // push -> element (>= 0) or -(push index) - 1
+    int *rlo, *rhi;          // range of each position at the current level
+    int *wl, *wr;            // QuickSelect window of the range starting here; wr < 0: done
 };
 
-__device__ static inline void iswap(int *v, size_t i, size_t j)
+// in-place exclusive block scan of v[0, n)
+__device__ static void kd_scan(int *v, int n, int *partial)
 {
-    int t = v[i];
-    v[i] = v[j];
-    v[j] = t;
+    const int tid = threadIdx.x;
+    const int per = (n + KD_THREADS - 1) / KD_THREADS;
+    const int b0 = tid * per, b1 = min(n, b0 + per);
+    int cnt = 0;
+    for (int p = b0; p < b1; p++) cnt += v[p];
+    partial[tid] = cnt;
+    __syncthreads();
+    if (tid < JAM_WAVE) {
+        int x[KD_THREADS / JAM_WAVE], sum = 0;
+#pragma unroll
+        for (int q = 0; q < KD_THREADS / JAM_WAVE; q++) {
+            x[q] = partial[(KD_THREADS / JAM_WAVE) * tid + q];
+            sum += x[q];
+        }
+        int incl = sum;
+        for (int sh = 1; sh < JAM_WAVE; sh <<= 1) {
+            int o = __shfl_up(incl, sh, JAM_WAVE);
+            if (tid >= sh) incl += o;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int q = 0; q < KD_THREADS / JAM_WAVE; q++) {
+            partial[(KD_THREADS / JAM_WAVE) * tid + q] = run;
+            run += x[q];
+        }
+    }
+    __syncthreads();
+    int run = partial[tid];
+    for (int p = b0; p < b1; p++) {
+        const int x = v[p];
+        v[p] = run;
+        run += x;
+    }
+    __syncthreads();
 }
 
-// QuickSelect of the k-th element of idx[0, n) by coordinate `axis`:
-// median-of-three pivot (mid = (l + r + 1) / 2), Lomuto partition with
-// "<= pivot goes left". The same swap sequence as the reference is needed
-// because it decides the side of elements equal to the median.
-__device__ static size_t quickselect(KdCmp &c, int *idx, size_t n, size_t k)
+__device__ static inline int kd_cmp3(const double *c, int x, int y)
 {
-    size_t l = 0, r = n - 1;
-    c.idx = idx;
-    for (;;) {
-        if (l == r) return l;
-        size_t mid = (l + r + 1) / 2, p;
-        if ((c.cmp(l, mid) > 0) != (c.cmp(l, r) > 0)) p = l;
-        else if ((c.cmp(mid, l) < 0) != (c.cmp(mid, r) < 0)) p = mid;
-        else p = r;
-        iswap(idx, r, p);
-        size_t store = l;
-        for (size_t i = l; i < r; i++) {
-            if (c.cmp(i, r) <= 0) {
-                iswap(idx, i, store);
-                store++;
+    const double u = c[x], v = c[y];
+    return u < v ? -1 : (u > v ? 1 : 0);
+}
+
+// Arranges elements 0..n-1 into the reference's leaf order, written to out.
+__device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
+{
+    const int tid = threadIdx.x;
+    int cur = 0;
+    for (int p = tid; p < n; p += KD_THREADS) {
+        w.arr[0][p] = p;
+        w.rlo[p] = 0;
+        w.rhi[p] = n;
+    }
+    __syncthreads();
+    for (int d = 0;; d++) {
+        const double *c = w.c[d % 3];
+        if (tid == 0) flag[0] = 0;
+        __syncthreads();
+        for (int p = tid; p < n; p += KD_THREADS) {
+            if (w.rlo[p] != p) continue;
+            const int hi = w.rhi[p];
+            if (hi - p >= 2) {
+                w.wl[p] = p;
+                w.wr[p] = hi - 1;
+                flag[0] = 1;
+            } else {
+                w.wr[p] = -1;
             }
         }
-        iswap(idx, store, r);
-        p = store;
-        if (p == k) return p;
-        if (k < p) r = p - 1;
-        else l = p + 1;
+        __syncthreads();
+        if (!flag[0]) break;
+        for (;;) {
+            int *a = w.arr[cur];
+            int *an = w.arr[cur ^ 1];
+            // pivot: median of three, moved to the right end of the window
+            for (int p = tid; p < n; p += KD_THREADS) {
+                if (w.rlo[p] != p || w.wr[p] < 0) continue;
+                const int l = w.wl[p], r = w.wr[p], mid = (l + r + 1) / 2;
+                int piv;
+                if ((kd_cmp3(c, a[l], a[mid]) > 0) != (kd_cmp3(c, a[l], a[r]) > 0)) piv = l;
+                else if ((kd_cmp3(c, a[mid], a[l]) < 0) != (kd_cmp3(c, a[mid], a[r]) < 0)) piv = mid;
+                else piv = r;
+                const int t = a[r];
+                a[r] = a[piv];
+                a[piv] = t;
+            }
+            __syncthreads();
+            // "<= pivot" flags of the windows
+            for (int i = tid; i < n; i += KD_THREADS) {
+                const int R = w.rlo[i];
+                const int r = w.wr[R];
+                int b = 0;
+                if (r >= 0 && i >= w.wl[R] && i < r) b = c[a[i]] <= c[a[r]] ? 1 : 0;
+                w.pre1[i] = b;
+            }
+            __syncthreads();
+            kd_scan(w.pre1, n, partial);
+            // re-push flags: a "<=" element after the first ">" one
+            for (int i = tid; i < n; i += KD_THREADS) {
+                const int R = w.rlo[i];
+                const int l = w.wl[R], r = w.wr[R];
+                int rot = 0;
+                if (r >= 0 && i >= l && i < r && c[a[i]] <= c[a[r]]) {
+                    const int zeros = (i - l) - (w.pre1[i] - w.pre1[l]);
+                    rot = zeros > 0 ? 1 : 0;
+                }
+                w.pre2[i] = rot;
+            }
+            __syncthreads();
+            kd_scan(w.pre2, n, partial);
+            // pushes: ">" elements are new pushes, re-pushes point to push q
+            for (int i = tid; i < n; i += KD_THREADS) {
+                const int R = w.rlo[i];
+                const int l = w.wl[R], r = w.wr[R];
+                if (r < 0 || i < l || i >= r) continue;
+                const int ones = w.pre1[i] - w.pre1[l];
+                const int zeros = (i - l) - ones;
+                const int rots = w.pre2[i] - w.pre2[l];
+                const int slot = l + zeros + rots;      // push number - 1, absolute
+                if (!(c[a[i]] <= c[a[r]])) w.src[slot] = a[i];
+                else if (zeros > 0) w.src[slot] = -(l + rots) - 1;
+            }
+            __syncthreads();
+            // pointer jumping until every push names an element
+            for (;;) {
+                if (tid == 0) flag[1] = 0;
+                __syncthreads();
+                for (int i = tid; i < n; i += KD_THREADS) {
+                    const int R = w.rlo[i];
+                    const int l = w.wl[R], r = w.wr[R];
+                    if (r < 0 || i < l || i >= r) continue;
+                    // only the m + nrot pushes of this round are valid; each
+                    // re-push points to a lower push, so the chains end
+                    const int pushes = (r - l) - (w.pre1[r] - w.pre1[l]) + (w.pre2[r] - w.pre2[l]);
+                    if (i - l >= pushes) continue;
+                    const int v = w.src[i];
+                    if (v < 0) {
+                        const int u = w.src[-v - 1];
+                        w.src[i] = u;
+                        if (u < 0) flag[1] = 1;
+                    }
+                }
+                __syncthreads();
+                const int again = flag[1];
+                __syncthreads();
+                if (!again) break;
+            }
+            // scatter into the other buffer
+            for (int i = tid; i < n; i += KD_THREADS) {
+                const int R = w.rlo[i];
+                const int l = w.wl[R], r = w.wr[R];
+                if (r < 0 || i < l || i > r) {
+                    an[i] = a[i];
+                    continue;
+                }
+                const int k1 = w.pre1[r] - w.pre1[l];
+                const int m = (r - l) - k1;
+                const int nrot = w.pre2[r] - w.pre2[l];
+                if (i == r) {
+                    an[l + k1] = a[r];                  // the pivot
+                    continue;
+                }
+                if (c[a[i]] <= c[a[r]]) an[l + (w.pre1[i] - w.pre1[l])] = a[i];
+                const int j = i - l;                    // queue slot j+1
+                if (j < m) an[j == 0 ? r : l + k1 + j] = w.src[l + nrot + j];
+            }
+            __syncthreads();
+            cur ^= 1;
+            // next window of each QuickSelect
+            if (tid == 0) flag[1] = 0;
+            __syncthreads();
+            for (int p = tid; p < n; p += KD_THREADS) {
+                if (w.rlo[p] != p || w.wr[p] < 0) continue;
+                const int l = w.wl[p], r = w.wr[p];
+                const int pos = l + (w.pre1[r] - w.pre1[l]);
+                const int k = p + (w.rhi[p] - p) / 2;
+                int nl = l, nr = r;
+                if (pos == k) nr = -1;
+                else if (k < pos) nr = pos - 1;
+                else nl = pos + 1;
+                if (nr >= 0 && nl == nr) nr = -1;
+                w.wl[p] = nl;
+                w.wr[p] = nr;
+                if (nr >= 0) flag[1] = 1;
+            }
+            __syncthreads();
+            const int more = flag[1];
+            __syncthreads();
+            if (!more) break;
+        }
+        // split every range at its median position
+        for (int p = tid; p < n; p += KD_THREADS) {
+            const int lo = w.rlo[p], hi = w.rhi[p];
+            if (hi - lo < 2) continue;
+            const int m = lo + (hi - lo) / 2;
+            if (p < m) w.rhi[p] = m;
+            else w.rlo[p] = m;
+        }
+        __syncthreads();
     }
+    const int *fin = w.arr[cur];
+    for (int p = tid; p < n; p += KD_THREADS) out[p] = fin[p];
+    __syncthreads();
 }
 
-// Recursive median split (left = [0, n/2), right = [n/2, n), axis cycling
-// x, y, z) done with an explicit stack of ranges, on lane 0.
-__device__ static void kd_arrange(KdCmp &c, int *idx, int n)
+// One workgroup per candidate set. Sets of up to KD_CAP entries work in LDS,
+// larger ones in a global scratch area of the same layout (scr).
+__global__ void __launch_bounds__(KD_THREADS) k_kdorder(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr)
 {
-    struct Rng { int lo, n, axis; };
-    Rng stack[64];
-    int sp = 0;
-    stack[sp++] = {0, n, 0};
-    while (sp > 0) {
-        Rng r = stack[--sp];
-        if (r.n <= 1) continue;
-        c.axis = r.axis;
-        size_t split = quickselect(c, idx + r.lo, (size_t)r.n, (size_t)(r.n / 2));
-        int na = (r.axis + 1) % 3;
-        // order of processing does not matter: the ranges are disjoint
-        stack[sp++] = {r.lo + (int)split, r.n - (int)split, na};
-        stack[sp++] = {r.lo, (int)split, na};
-    }
-}
-
-__global__ void __launch_bounds__(64) k_kdorder(DevMols ms, DevTpls tp, DevSets st)
-{
+    __shared__ double lc[3][KD_CAP];
+    __shared__ int lint[9][KD_CAP];
+    __shared__ int partial[KD_THREADS];
+    __shared__ int flag[2];
     const int s = blockIdx.x;
-    const int lane = lane_id();
+    const int tid = threadIdx.x;
     if (s >= st.n_sets) return;
     if (*st.total > st.cap) return;
-    const int cnt = st.cnt[s];
-    if (cnt == 0) return;
+    const int n = st.cnt[s];
+    if (n == 0) return;
     const int64_t off = st.off[s];
     const int m = s / tp.n_types;
     const int abase = ms.atom_off[m];
-    int *idx = st.kidx + off;
-    for (int i = lane; i < cnt; i += JAM_WAVE) idx[i] = i;
-    __syncthreads();
-    if (lane == 0) {
-        KdCmp c{ms.xyz, st.list + off, idx, abase, 0};
-        kd_arrange(c, idx, cnt);
+    double *cb;
+    int *ib;
+    size_t stride;
+    if (n <= KD_CAP) {
+        cb = &lc[0][0];
+        ib = &lint[0][0];
+        stride = KD_CAP;
+    } else {
+        cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
+        ib = (int *)(cb + 3 * (size_t)n);
+        stride = (size_t)n;
+    }
+    KdWork w;
+    for (int a = 0; a < 3; a++) w.c[a] = cb + a * stride;
+    w.arr[0] = ib;
+    w.arr[1] = ib + stride;
+    w.pre1 = ib + 2 * stride;
+    w.pre2 = ib + 3 * stride;
+    w.src = ib + 4 * stride;
+    w.rlo = ib + 5 * stride;
+    w.rhi = ib + 6 * stride;
+    w.wl = ib + 7 * stride;
+    w.wr = ib + 8 * stride;
+    for (int i = tid; i < n; i += KD_THREADS) {
+        const int g = abase + st.list[off + i];
+        w.c[0][i] = ms.xyz[3 * (size_t)g + 0];
+        w.c[1][i] = ms.xyz[3 * (size_t)g + 1];
+        w.c[2][i] = ms.xyz[3 * (size_t)g + 2];
     }
     __syncthreads();
+    int *order = st.kidx + off;
+    kd_exact(w, n, order, partial, flag);
     // DFS visits right subtrees first: position p of the visit order is
-    // element cnt-1-p of the arranged index array.
+    // element n-1-p of the arrangement.
     const int64_t bbase = off / JAM_WAVE + s;
-    for (int b0 = 0; b0 < cnt; b0 += JAM_WAVE) {
+    const int lane = lane_id();
+    const int wave = tid / JAM_WAVE;
+    for (int b0 = wave * JAM_WAVE; b0 < n; b0 += KD_THREADS) {
         const int p = b0 + lane;
         double x = 0.0, y = 0.0, z = 0.0;
-        const bool ok = p < cnt;
+        const bool ok = p < n;
         if (ok) {
-            const int li = idx[cnt - 1 - p];
+            const int li = order[n - 1 - p];
             const int atom = st.list[off + li];
             const int g = abase + atom;
-            x = ms.xyz[3 * (size_t)g + 0];
-            y = ms.xyz[3 * (size_t)g + 1];
-            z = ms.xyz[3 * (size_t)g + 2];
+            x = w.c[0][li];
+            y = w.c[1][li];
+            z = w.c[2][li];
             st.kx[off + p] = x;
             st.ky[off + p] = y;
             st.kz[off + p] = z;

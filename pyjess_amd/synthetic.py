@@ -114,6 +114,41 @@ def ca_walk(rng: np.random.Generator, n: int, step: float = 3.8, avoid: float = 
     return pos
 
 
+def _residue_topology():
+    """Per residue type: atom names in PDB order, parent (local index, -1 for
+    CA) and depth in the bond tree rooted at CA."""
+    topo = {}
+    for rn in RESIDUES:
+        names = ["N", "CA", "C", "O"] + SIDECHAINS[rn]
+        parent = []
+        prev = "CA"
+        for nm in names:
+            if nm == "CA":
+                parent.append(-1)
+                continue
+            if nm in ("N", "C", "CB"):
+                par = "CA"
+            elif nm == "O":
+                par = "C"
+            else:
+                par = _PARENT.get((rn, nm), prev)
+            if nm not in ("N", "C", "O"):
+                prev = nm
+            parent.append(names.index(par))
+        depth = []
+        for i in range(len(names)):
+            d, j = 0, i
+            while parent[j] >= 0:
+                j = parent[j]
+                d += 1
+            depth.append(d)
+        topo[rn] = (names, parent, depth)
+    return topo
+
+
+_TOPO = _residue_topology()
+
+
 def protein(rng: np.random.Generator, n_residues: int, n_chains: int = 1, mol_id: str = "SYN1",
             ca: Optional[np.ndarray] = None) -> Molecule:
     """A synthetic protein of ``n_residues`` residues split into ``n_chains`` chains
@@ -122,46 +157,35 @@ def protein(rng: np.random.Generator, n_residues: int, n_chains: int = 1, mol_id
         ca = ca_walk(rng, n_residues)
     types = rng.integers(0, len(RESIDUES), size=n_residues)
     per_chain = int(math.ceil(n_residues / n_chains))
-    xyz, names, resn, chains, resseq = [], [], [], [], []
-    bonds = _unit(rng, n_residues * 16)
-    b = 0
-    for r in range(n_residues):
-        rn = RESIDUES[types[r]]
-        chain = CHAIN_IDS[min(r // per_chain, len(CHAIN_IDS) - 1)]
-        num = r % per_chain + 1
-        placed = {"CA": ca[r]}
-        order = ["N", "CA", "C", "O"] + SIDECHAINS[rn]
-        prev = "CA"
-        for nm in order:
-            if nm == "CA":
-                pass
-            else:
-                if nm in ("N", "C", "CB"):
-                    parent = "CA"
-                elif nm == "O":
-                    parent = "C"
-                else:
-                    parent = _PARENT.get((rn, nm), prev)
-                placed[nm] = placed[parent] + 1.5 * bonds[b]
-                b += 1
-                if nm not in ("N", "C", "O"):
-                    prev = nm
-            xyz.append(placed[nm])
-            names.append(nm)
-            resn.append(rn)
-            chains.append(chain)
-            resseq.append(num)
-    n = len(xyz)
-    xyz = np.round(np.array(xyz, dtype=np.float64), 3)
-    enc_names = np.frombuffer(b"".join(_enc_name(s) for s in names), dtype=np.uint8).reshape(n, 4)
-    enc_res = np.frombuffer(b"".join(s.encode() + b"\0" for s in resn), dtype=np.uint8).reshape(n, 4)
+    counts = np.array([len(_TOPO[RESIDUES[t]][0]) for t in range(len(RESIDUES))])[types]
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    n = int(counts.sum())
+    res_of = np.repeat(np.arange(n_residues), counts)
+    tlist = [RESIDUES[t] for t in types]
+    names = [nm for rn in tlist for nm in _TOPO[rn][0]]
+    par_flat = np.concatenate([np.array(_TOPO[rn][1]) for rn in tlist])
+    dep_flat = np.concatenate([np.array(_TOPO[rn][2]) for rn in tlist])
+    parent = np.where(par_flat >= 0, first[res_of] + par_flat, -1)
+    xyz = np.zeros((n, 3))
+    is_ca = par_flat < 0
+    xyz[is_ca] = ca[res_of[is_ca]]
+    bonds = _unit(rng, n)
+    for d in range(1, int(dep_flat.max()) + 1):
+        sel = np.nonzero(dep_flat == d)[0]
+        xyz[sel] = xyz[parent[sel]] + 1.5 * bonds[sel]
+    xyz = np.round(xyz, 3)
+    chain_idx = np.minimum(res_of // per_chain, len(CHAIN_IDS) - 1)
+    resseq = (res_of % per_chain + 1).astype(np.int32)
+    enc_names = np.frombuffer(b"".join(_enc_name(nm) for nm in names), dtype=np.uint8).reshape(n, 4)
+    res_codes = np.frombuffer(b"".join(rn.encode() + b"\0" for rn in RESIDUES), dtype=np.uint8).reshape(-1, 4)
+    enc_res = res_codes[types[res_of]]
     ch = np.zeros((n, 2), dtype=np.uint8)
     ch[:, 0] = ord(" ")
-    ch[:, 1] = np.frombuffer("".join(chains).encode(), dtype=np.uint8)
+    ch[:, 1] = np.frombuffer(CHAIN_IDS.encode(), dtype=np.uint8)[chain_idx]
     element = np.zeros((n, 2), dtype=np.uint8)
     element[:, 0] = ord("_")
-    element[:, 1] = np.frombuffer("".join(s[0] for s in names).encode(), dtype=np.uint8)
-    return Molecule.from_arrays(xyz, enc_names, enc_res, ch, np.array(resseq, dtype=np.int32),
+    element[:, 1] = enc_names[:, 1]
+    return Molecule.from_arrays(xyz, enc_names, np.ascontiguousarray(enc_res), ch, resseq,
                                 np.arange(1, n + 1, dtype=np.int32), id=mol_id, element=element)
 
 

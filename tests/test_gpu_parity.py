@@ -193,3 +193,45 @@ def test_repeat_queries_are_deterministic(mol_1amy, tpl_133, tpl_412):
     first = [(h.rmsd, key(mol_1amy, h)) for h in jess.query(mol_1amy, 2, 4, 4)]
     for _ in range(3):
         assert [(h.rmsd, key(mol_1amy, h)) for h in jess.query(mol_1amy, 2, 4, 4)] == first
+
+
+def trp_template(mol, rng, tid):
+    """Mode-3 carbons of a TRP (a set of every TRP carbon: > 1024 entries on a
+    big protein) plus two exact atoms of a residue nearby."""
+    starts = synthetic._residue_starts(mol)
+    trp = [s for s in starts if bytes(mol.resnames[s][:3]) == b"TRP"]
+    s0 = trp[int(rng.integers(len(trp)))]
+    res = synthetic._residue_atoms(mol, s0)
+    ca = mol.xyz[s0 + 1]
+    near = [s for s in starts if s != s0 and np.linalg.norm(mol.xyz[s + 1] - ca) < 9.0]
+    s1 = near[int(rng.integers(len(near)))]
+    atoms = []
+    for a, mode in ((res[1], 3), (res[4], 3), (res[5], 3)):
+        nm = bytes(mol.names[a]).decode().strip("_")
+        atoms.append(TemplateAtom(chain_id="A", residue_number=int(mol.resseq[a]), x=float(mol.xyz[a, 0]),
+                                  y=float(mol.xyz[a, 1]), z=float(mol.xyz[a, 2]), residue_names=["TRP"],
+                                  atom_names=[nm], match_mode=mode))
+    for a in synthetic._residue_atoms(mol, s1)[:2]:
+        nm = bytes(mol.names[a]).decode().strip("_")
+        rn = bytes(mol.resnames[a][:3]).decode()
+        atoms.append(TemplateAtom(chain_id="A", residue_number=int(mol.resseq[a]), x=float(mol.xyz[a, 0]),
+                                  y=float(mol.xyz[a, 1]), z=float(mol.xyz[a, 2]), residue_names=[rn],
+                                  atom_names=[nm], match_mode=0))
+    return Template(atoms, id=tid)
+
+
+@pytest.mark.parametrize("decimals", [3, 0])
+@pytest.mark.parametrize("reorder", [False, True])
+def test_kd_order_large_sets_and_ties(decimals, reorder):
+    """Sets larger than the LDS capacity, and coordinates with many equal values
+    (decimals=0), still enumerate in the reference order."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    mol = synthetic.protein(rng, 2600)
+    if decimals == 0:
+        mol.xyz[:] = np.round(mol.xyz)
+    tpls = [trp_template(mol, rng, f"W{i}") for i in range(6)]
+    tpls += synthetic.template_set(rng, [mol], 20, 0.5, (2, 4), (2, 3), jitter=0.3, p_flexible=0.8)
+    got = list(Jess(tpls).query(mol, 2.0, 1.5, 1.5, reorder=reorder))
+    want = oracle.hits(mol, tpls, 2.0, 1.5, 1.5, reorder=reorder)
+    assert len(want) > 0
+    assert_same_hits(mol, tpls, got, want)
