@@ -97,11 +97,16 @@ struct jam_context {
     hipEvent_t ev[6] = {};
     // per-query workspace
     DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr;
-    DBuf m_item, m_seq, m_atoms, m_count, item_cm;
+    DBuf m_item, m_seq, m_atoms, m_count, item_cm, item_pre;
+    DBuf p_ord, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
+    DBuf f_pair[2], f_pos[2], f_cnt_next, f_cnt_done, f_off_next, f_off_done, f_tot, d_pair, d_pos;
+    int64_t front_cap[2] = {0, 0}, fcnt_cap = 0, done_cap = 0;
+    int front_stride = 0;
     DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms, e_count;
-    DBuf scan_tmp;
+    DBuf scan_tmp, scan_tmp2;
     int64_t set_cap = 0, blk_cap = 0, match_cap = 0, event_cap = 0;
-    int nsets_cap = 0;
+    int nsets_cap = 0, npairs_cap = 0;
+    int64_t items_cap = 0;
     int match_stride = 0, event_stride = 0;
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
@@ -449,7 +454,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
     return JAM_OK;
 }
 
-int32_t grow_matches(jam_context *c, int64_t cap, int stride, int n_items)
+int32_t grow_matches(jam_context *c, int64_t cap, int stride)
 {
     if (cap > c->match_cap || stride > c->match_stride) {
         cap = std::max<int64_t>(cap, std::max<int64_t>(c->match_cap, 4096));
@@ -461,7 +466,82 @@ int32_t grow_matches(jam_context *c, int64_t cap, int stride, int n_items)
         c->match_stride = stride;
     }
     HIP_TRY(c->m_count.reserve(sizeof(unsigned long long)));
-    HIP_TRY(c->item_cm.reserve(sizeof(int) * (size_t)std::max(1, n_items)));
+    return JAM_OK;
+}
+
+int32_t grow_pairs(jam_context *c, int n_pairs)
+{
+    if (n_pairs <= c->npairs_cap) return JAM_OK;
+    HIP_TRY(c->p_ord.reserve((size_t)JAM_MAX_LEVELS * n_pairs));
+    HIP_TRY(c->p_nchunk.reserve(sizeof(int) * (size_t)n_pairs));
+    HIP_TRY(c->p_base.reserve(sizeof(long long) * (size_t)n_pairs));
+    HIP_TRY(c->p_cm.reserve(sizeof(long long) * (size_t)n_pairs));
+    HIP_TRY(c->p_first.reserve(sizeof(long long) * (size_t)n_pairs));
+    HIP_TRY(c->p_n1.reserve(sizeof(int) * (size_t)n_pairs));
+    HIP_TRY(c->n_items.reserve(sizeof(long long)));
+    c->npairs_cap = n_pairs;
+    return JAM_OK;
+}
+
+int32_t grow_items(jam_context *c, int64_t n_items)
+{
+    if (n_items <= c->items_cap) return JAM_OK;
+    n_items = std::max<int64_t>(n_items, 4096);
+    HIP_TRY(c->i_pair.reserve(sizeof(int) * (size_t)n_items));
+    c->items_cap = n_items;
+    return JAM_OK;
+}
+
+// partial-match rows of buffer b (pair + stride positions each)
+int32_t grow_front(jam_context *c, int b, int64_t rows, int stride)
+{
+    if (rows <= c->front_cap[b] && stride <= c->front_stride) return JAM_OK;
+    rows = std::max<int64_t>(std::max<int64_t>(rows, c->front_cap[b]), 4096);
+    stride = std::max(stride, c->front_stride);
+    HIP_TRY(c->f_pair[b].reserve(sizeof(int) * (size_t)rows));
+    HIP_TRY(c->f_pos[b].reserve(sizeof(int) * (size_t)rows * stride));
+    c->front_cap[b] = rows;
+    if (stride > c->front_stride) {
+        // the other buffer must follow the wider rows too
+        c->front_stride = stride;
+        c->front_cap[b ^ 1] = 0;
+    }
+    return JAM_OK;
+}
+
+int32_t grow_fcnt(jam_context *c, int64_t rows)
+{
+    if (rows <= c->fcnt_cap) return JAM_OK;
+    rows = std::max<int64_t>(rows, 4096);
+    HIP_TRY(c->f_cnt_next.reserve(sizeof(int) * (size_t)rows));
+    HIP_TRY(c->f_cnt_done.reserve(sizeof(int) * (size_t)rows));
+    HIP_TRY(c->f_off_next.reserve(sizeof(long long) * (size_t)rows));
+    HIP_TRY(c->f_off_done.reserve(sizeof(long long) * (size_t)rows));
+    HIP_TRY(c->f_tot.reserve(sizeof(long long) * 2));
+    c->fcnt_cap = rows;
+    return JAM_OK;
+}
+
+int32_t grow_done(jam_context *c, int64_t rows, int stride, int64_t keep)
+{
+    // keep: rows already written that must survive the growth
+    if (rows <= c->done_cap && (size_t)stride * sizeof(int) * std::max<int64_t>(rows, 1) <= c->d_pos.bytes)
+        return JAM_OK;
+    rows = std::max<int64_t>(std::max<int64_t>(rows, 2 * c->done_cap), 4096);
+    DBuf np, nq;
+    HIP_TRY(np.reserve(sizeof(int) * (size_t)rows));
+    HIP_TRY(nq.reserve(sizeof(int) * (size_t)rows * stride));
+    if (keep > 0) {
+        HIP_TRY(hipMemcpyAsync(np.ptr, c->d_pair.ptr, sizeof(int) * keep, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(nq.ptr, c->d_pos.ptr, sizeof(int) * keep * stride, hipMemcpyDeviceToDevice,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    std::swap(c->d_pair.ptr, np.ptr);
+    std::swap(c->d_pair.bytes, np.bytes);
+    std::swap(c->d_pos.ptr, nq.ptr);
+    std::swap(c->d_pos.bytes, nq.bytes);
+    c->done_cap = rows;
     return JAM_OK;
 }
 
@@ -490,6 +570,45 @@ __global__ void k_total(const int *cnt, const int64_t *off, int n, int64_t *tota
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
 }
 
+__global__ void k_total_items(const int *cnt, const long long *off, long long n, long long *total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
+}
+
+// complete matches per pair: the matches of a pair are contiguous
+__global__ void k_pair_matches(const int *done_pair, long long n, const long long *first, long long *out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == n - 1 || done_pair[i + 1] != done_pair[i]) out[done_pair[i]] = i - first[done_pair[i]] + 1;
+}
+
+
+__global__ void k_items_any(DevPairs pr)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pr.n_pairs) return;
+    const int c = pr.nchunk[p];
+    const long long b = pr.item_base[p];
+    for (int i = 0; i < c; i++) pr.item_pair[b + i] = p;
+}
+
+void launch_expand(int stride, bool fill, int64_t n_in, hipStream_t s, const DevMols &dm, const DevTpls &dt,
+                   const DevSets &st, const DevPairs &pr, const DevFront &fr, const QueryParams &qp)
+{
+    const dim3 g((unsigned)n_in), b(64);
+    if (stride <= 8) {
+        if (fill) hipLaunchKernelGGL((jam::k_expand<8, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+        else hipLaunchKernelGGL((jam::k_expand<8, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+    } else if (stride <= 16) {
+        if (fill) hipLaunchKernelGGL((jam::k_expand<16, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+        else hipLaunchKernelGGL((jam::k_expand<16, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+    } else {
+        if (fill) hipLaunchKernelGGL((jam::k_expand<32, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+        else hipLaunchKernelGGL((jam::k_expand<32, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+    }
+}
+
 }  // namespace
 
 extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_molecules *ms,
@@ -504,6 +623,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     const int T = tp->n_templates, M = ms->n_mols, U = tp->n_types;
     const int64_t n_pairs64 = (int64_t)T * M;
     if (n_pairs64 > INT32_MAX / 64) return fail(JAM_ETOOBIG, "too many (template, molecule) pairs");
+    if ((int64_t)M * U > INT32_MAX / 64) return fail(JAM_ETOOBIG, "too many (molecule, atom type) sets");
     const int n_pairs = (int)n_pairs64;
     R->n_pairs = n_pairs;
     R->stride = tp->stride;
@@ -514,31 +634,29 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     }
     const int n_sets = M * U;
     hipStream_t s = c->stream;
-
     QueryParams qp;
     qp.rmsd_threshold = p->rmsd_threshold;
     qp.distance_cutoff = p->distance_cutoff;
     qp.max_dynamic_distance = p->max_dynamic_distance;
     qp.ignore_chain = p->ignore_chain;
     qp.reorder = p->reorder ? 1 : 0;
-    // level-0 ranges per pair: enough waves to fill the device on small queries
-    qp.items_per_pair = (int)std::min<int64_t>(16, std::max<int64_t>(1, 4096 / n_pairs));
-    const int n_items = n_pairs * qp.items_per_pair;
+    qp.chunk = 1;
     const int stride = tp->stride;
 
     int32_t rc;
     if ((rc = grow_sets(c, std::max<int64_t>(c->set_cap, 1 << 16), n_sets)) != JAM_OK) return rc;
-    if ((rc = grow_matches(c, std::max<int64_t>(c->match_cap, 1 << 14), stride, n_items)) != JAM_OK) return rc;
     if ((rc = grow_events(c, std::max<int64_t>(c->event_cap, 1 << 12), stride)) != JAM_OK) return rc;
-
+    if ((rc = grow_pairs(c, n_pairs)) != JAM_OK) return rc;
     DevMols dm = ms->dev();
     DevTpls dt = tp->dev();
-    size_t scan_bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int *)nullptr, (int64_t *)nullptr,
+    size_t scan_sets = 0, scan_pairs = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_sets, (const int *)nullptr, (int64_t *)nullptr,
                                              n_sets, s));
-    HIP_TRY(c->scan_tmp.reserve(scan_bytes));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_pairs, (const int *)nullptr, (long long *)nullptr,
+                                             n_pairs, s));
+    HIP_TRY(c->scan_tmp.reserve(std::max(scan_sets, scan_pairs)));
+    const int pair_blocks = (n_pairs + 255) / 256;
 
-    float ms_k[4] = {0, 0, 0, 0};
     for (int attempt = 0; attempt < 4; attempt++) {
         DevSets st;
         st.n_sets = n_sets;
@@ -556,14 +674,114 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         st.kch = c->kch.as<int>();
         st.bbox = c->bbox.as<double>();
         st.total = c->total.as<int64_t>();
-        DevMatches mt;
-        mt.cap = c->match_cap;
-        mt.stride = c->match_stride;
-        mt.item = c->m_item.as<int>();
-        mt.seq = c->m_seq.as<int>();
-        mt.atoms = c->m_atoms.as<int>();
-        mt.count = c->m_count.as<unsigned long long>();
-        mt.item_cm = c->item_cm.as<int>();
+        DevPairs pr;
+        pr.n_pairs = n_pairs;
+        pr.n_items = 0;
+        pr.ord = c->p_ord.as<unsigned char>();
+        pr.nchunk = c->p_nchunk.as<int>();
+        pr.n1 = c->p_n1.as<int>();
+        pr.item_base = c->p_base.as<long long>();
+        pr.item_pair = c->i_pair.as<int>();
+
+        const int wpb = 4;  // waves per block of the per-set candidate kernels
+        const int set_blocks = (n_sets + wpb - 1) / wpb;
+        HIP_TRY(hipEventRecord(c->ev[0], s));
+        hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_sets, st.cnt, st.off, n_sets, s));
+        hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
+        hipLaunchKernelGGL(jam::k_pairs, dim3(pair_blocks), dim3(256), 0, s, dt, st, pr, qp);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_pairs, pr.nchunk, pr.item_base, n_pairs,
+                                                 s));
+        hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, pr.nchunk, pr.item_base, (long long)n_pairs,
+                           c->n_items.as<long long>());
+        int64_t totals[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(&totals[0], st.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&totals[1], c->n_items.ptr, sizeof(long long), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (totals[0] > c->set_cap) {
+            if ((rc = grow_sets(c, totals[0], n_sets)) != JAM_OK) return rc;
+            continue;
+        }
+        const int64_t n0 = totals[1];
+        if ((rc = grow_items(c, n0)) != JAM_OK) return rc;
+        pr.n_items = n0;
+        pr.item_pair = c->i_pair.as<int>();
+        hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[1], s));
+        {
+            // small sets: one wave each; larger: 256 threads; grid-stride loops
+            const int g_small = std::min(n_sets, 256 * 40);
+            const int g_big = std::min(n_sets, 256 * 2);
+            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(g_small), dim3(JAM_WAVE), 0, s, dm,
+                               dt, st, c->kscr.as<unsigned char>(), 0, KD_SMALL);
+            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(g_big), dim3(KD_THREADS), 0, s, dm,
+                               dt, st, c->kscr.as<unsigned char>(), KD_SMALL, INT32_MAX);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipEventRecord(c->ev[2], s));
+
+        // ---- the join, one level at a time
+        int64_t n_done = 0;
+        if (n0 > 0) {
+            if ((rc = grow_front(c, 0, n0, stride)) != JAM_OK) return rc;
+            hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
+            DevFront fr{};
+            fr.stride = c->front_stride;
+            fr.out_pair = c->f_pair[0].as<int>();
+            fr.out_pos = c->f_pos[0].as<int>();
+            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, pr, fr);
+            HIP_TRY(hipGetLastError());
+            int cur = 0;
+            int64_t n_in = n0;
+            for (int L = 1; n_in > 0 && L <= std::max(1, stride - 1); L++) {
+                if ((rc = grow_fcnt(c, n_in)) != JAM_OK) return rc;
+                fr.level = L;
+                fr.stride = c->front_stride;
+                fr.n_in = n_in;
+                fr.in_pair = c->f_pair[cur].as<int>();
+                fr.in_pos = c->f_pos[cur].as<int>();
+                fr.cnt_next = c->f_cnt_next.as<int>();
+                fr.cnt_done = c->f_cnt_done.as<int>();
+                fr.off_next = c->f_off_next.as<long long>();
+                fr.off_done = c->f_off_done.as<long long>();
+                fr.done_base = n_done;
+                launch_expand(stride, false, n_in, s, dm, dt, st, pr, fr, qp);
+                HIP_TRY(hipGetLastError());
+                size_t sb = 0;
+                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, fr.cnt_next, fr.off_next, n_in, s));
+                HIP_TRY(c->scan_tmp2.reserve(sb));
+                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp2.ptr, sb, fr.cnt_next, fr.off_next, n_in, s));
+                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp2.ptr, sb, fr.cnt_done, fr.off_done, n_in, s));
+                long long *tot = c->f_tot.as<long long>();
+                hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, fr.cnt_next, fr.off_next, n_in, tot);
+                hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, fr.cnt_done, fr.off_done, n_in, tot + 1);
+                long long nt[2];
+                HIP_TRY(hipMemcpyAsync(nt, tot, sizeof(long long) * 2, hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+                if ((rc = grow_front(c, cur ^ 1, nt[0], stride)) != JAM_OK) return rc;
+                if ((rc = grow_done(c, n_done + nt[1], c->front_stride, n_done)) != JAM_OK) return rc;
+                fr.stride = c->front_stride;
+                fr.in_pos = c->f_pos[cur].as<int>();
+                fr.in_pair = c->f_pair[cur].as<int>();
+                fr.out_pair = c->f_pair[cur ^ 1].as<int>();
+                fr.out_pos = c->f_pos[cur ^ 1].as<int>();
+                fr.done_pair = c->d_pair.as<int>();
+                fr.done_pos = c->d_pos.as<int>();
+                launch_expand(stride, true, n_in, s, dm, dt, st, pr, fr, qp);
+                HIP_TRY(hipGetLastError());
+                n_done += nt[1];
+                n_in = nt[0];
+                cur ^= 1;
+            }
+        }
+        HIP_TRY(hipEventRecord(c->ev[3], s));
+        DevFront fd{};
+        fd.stride = c->front_stride;
+        fd.done_pair = c->d_pair.as<int>();
+        fd.done_pos = c->d_pos.as<int>();
         DevEvents ev;
         ev.cap = c->event_cap;
         ev.stride = stride;
@@ -576,69 +794,31 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         ev.c1 = c->e_c1.as<double>();
         ev.atoms = c->e_atoms.as<int>();
         ev.count = c->e_count.as<unsigned long long>();
-
-        HIP_TRY(hipMemsetAsync(mt.count, 0, sizeof(unsigned long long), s));
         HIP_TRY(hipMemsetAsync(ev.count, 0, sizeof(unsigned long long), s));
-        const int wpb = 4;  // waves per block of the per-set kernels
-        const int set_blocks = (n_sets + wpb - 1) / wpb;
-        HIP_TRY(hipEventRecord(c->ev[0], s));
-        hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_bytes, st.cnt, st.off, n_sets, s));
-        hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
-        hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[1], s));
-        hipLaunchKernelGGL(jam::k_kdorder, dim3(n_sets), dim3(KD_THREADS), 0, s, dm, dt, st,
-                           c->kscr.as<unsigned char>());
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[2], s));
-        if (stride <= 8)
-            hipLaunchKernelGGL(jam::k_join<8>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
-        else if (stride <= 16)
-            hipLaunchKernelGGL(jam::k_join<16>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
-        else
-            hipLaunchKernelGGL(jam::k_join<32>, dim3(n_items), dim3(64), 0, s, dm, dt, st, mt, qp, n_items);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[3], s));
-        unsigned long long counts[2];
-        int64_t total = 0;
-        // matches counted; superposition over min(count, cap) needs the count on the host
-        HIP_TRY(hipMemcpyAsync(&counts[0], mt.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&total, st.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (total > c->set_cap) {
-            if ((rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
-            continue;
-        }
-        if ((int64_t)counts[0] > c->match_cap) {
-            if ((rc = grow_matches(c, (int64_t)counts[0], stride, n_items)) != JAM_OK) return rc;
-            continue;
-        }
-        const long long nm = (long long)counts[0];
-        if (nm > 0) {
-            const int tb = 256;
-            hipLaunchKernelGGL(jam::k_super, dim3((unsigned)((nm + tb - 1) / tb)), dim3(tb), 0, s, dm, dt, mt, ev,
-                               qp, nm);
+        if (n_done > 0) {
+            const unsigned gb = (unsigned)((n_done + 255) / 256);
+            hipLaunchKernelGGL(jam::k_first, dim3(gb), dim3(256), 0, s, fd.done_pair, (long long)n_done,
+                               c->p_first.as<long long>());
+            hipLaunchKernelGGL(jam::k_super, dim3(gb), dim3(256), 0, s, dm, dt, st, pr, fd, ev, qp,
+                               (long long)n_done, (const long long *)c->p_first.as<long long>());
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[4], s));
-        HIP_TRY(hipMemcpyAsync(&counts[1], ev.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        unsigned long long ne_dev = 0;
+        HIP_TRY(hipMemcpyAsync(&ne_dev, ev.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        if ((int64_t)counts[1] > c->event_cap) {
-            if ((rc = grow_events(c, (int64_t)counts[1], stride)) != JAM_OK) return rc;
+        if ((int64_t)ne_dev > c->event_cap) {
+            if ((rc = grow_events(c, (int64_t)ne_dev, stride)) != JAM_OK) return rc;
             continue;
         }
+        float ms_k[5] = {0, 0, 0, 0, 0};
         HIP_TRY(hipEventElapsedTime(&ms_k[0], c->ev[0], c->ev[1]));
         HIP_TRY(hipEventElapsedTime(&ms_k[1], c->ev[1], c->ev[2]));
         HIP_TRY(hipEventElapsedTime(&ms_k[2], c->ev[2], c->ev[3]));
         HIP_TRY(hipEventElapsedTime(&ms_k[3], c->ev[3], c->ev[4]));
-        for (int k = 0; k < 4; k++) R->ms[k] = ms_k[k];
-        float all = 0;
-        HIP_TRY(hipEventElapsedTime(&all, c->ev[0], c->ev[4]));
-        R->ms[4] = all;
-        // copy out
-        const int64_t ne = (int64_t)counts[1];
+        HIP_TRY(hipEventElapsedTime(&ms_k[4], c->ev[0], c->ev[4]));
+        for (int k = 0; k < 5; k++) R->ms[k] = ms_k[k];
+        const int64_t ne = (int64_t)ne_dev;
         R->pair.resize(ne);
         R->rank.resize(ne);
         R->flag.resize(ne);
@@ -647,7 +827,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         R->c0.resize(3 * ne);
         R->c1.resize(3 * ne);
         R->atoms.resize(ne * stride);
-        std::vector<int> icm(n_items);
         if (ne > 0) {
             HIP_TRY(hipMemcpyAsync(R->pair.data(), ev.pair, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipMemcpyAsync(R->rank.data(), ev.rank, sizeof(long long) * ne, hipMemcpyDeviceToHost, s));
@@ -658,9 +837,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             HIP_TRY(hipMemcpyAsync(R->c1.data(), ev.c1, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipMemcpyAsync(R->atoms.data(), ev.atoms, sizeof(int) * ne * stride, hipMemcpyDeviceToHost, s));
         }
-        HIP_TRY(hipMemcpyAsync(icm.data(), mt.item_cm, sizeof(int) * n_items, hipMemcpyDeviceToHost, s));
+        if (n_done > 0) {
+            HIP_TRY(hipMemsetAsync(c->p_cm.ptr, 0, sizeof(long long) * n_pairs, s));
+            hipLaunchKernelGGL(k_pair_matches, dim3((unsigned)((n_done + 255) / 256)), dim3(256), 0, s,
+                               c->d_pair.as<int>(), (long long)n_done, (const long long *)c->p_first.as<long long>(),
+                               c->p_cm.as<long long>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(R->pair_cm.data(), c->p_cm.ptr, sizeof(long long) * n_pairs,
+                                   hipMemcpyDeviceToHost, s));
+        }
         HIP_TRY(hipStreamSynchronize(s));
-        for (int i = 0; i < n_items; i++) R->pair_cm[i / qp.items_per_pair] += icm[i];
         *out = R.release();
         return JAM_OK;
     }

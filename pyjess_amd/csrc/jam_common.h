@@ -69,6 +69,35 @@ struct DevSets {
     int64_t *total;           // device scalar: sum of cnt
 };
 
+#define JAM_MAX_LEVELS 32
+
+struct DevPairs {
+    int n_pairs;
+    long long n_items;
+    unsigned char *ord;       // JAM_MAX_LEVELS per pair: level -> template atom
+    int *nchunk;              // level-0 partial matches per pair
+    int *n1;                  // first level-0 list index (1 for one-atom templates)
+    long long *item_base;     // exclusive scan of nchunk
+    int *item_pair;           // pair of every level-0 partial
+};
+
+// Partial matches of one level: a pair and one position per placed level
+// (level 0: index in the candidate list, level l >= 1: kd position).
+struct DevFront {
+    int level;                // L: the level being added
+    int stride;               // ints per position row
+    long long n_in;
+    const int *in_pair;
+    const int *in_pos;
+    int *cnt_next, *cnt_done; // children per partial (continuing / complete)
+    long long *off_next, *off_done;
+    int *out_pair;
+    int *out_pos;
+    long long done_base;      // complete matches found at earlier levels
+    int *done_pair;
+    int *done_pos;
+};
+
 struct DevMatches {
     int64_t cap;
     int stride;
@@ -77,6 +106,7 @@ struct DevMatches {
     int *atoms;               // stride per match: molecule-local atom per template atom
     unsigned long long *count;  // device scalar
     int *item_cm;             // complete matches per work item
+    long long *item_pre;      // exclusive scan of item_cm
 };
 
 struct DevEvents {
@@ -96,7 +126,7 @@ struct QueryParams {
     double max_dynamic_distance;
     int ignore_chain;
     int reorder;
-    int items_per_pair;       // G: level-0 ranges per (template, molecule) pair
+    int chunk;                // level-0 candidates per work item
 };
 
 __host__ __device__ static inline uint32_t jam_upper4(uint32_t v)

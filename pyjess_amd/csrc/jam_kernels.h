@@ -10,14 +10,16 @@
 //                  reversed because its DFS visits right children first; writes
 //                  the set as SoA in that order plus one bounding box per 64
 //                  entries.
-//   k_join         wave per (template, molecule, level-0 range): the Scanner
-//                  DFS (patches/Scanner.c.patch:232-334) with an explicit
-//                  per-level stack in LDS; each level scans 64-entry blocks of
-//                  the kd-ordered set whose box meets the join box, all 64
-//                  lanes testing one candidate each against every annulus and
-//                  check(), survivors taken lowest lane first (= kd order).
+//   k_pairs        thread per (template, molecule) pair: level order and the
+//                  number of level-0 partial matches.
+//   k_expand       wave per partial match of level L-1: the candidates of
+//                  level L (patches/Scanner.c.patch:232-334) from the 64-entry
+//                  blocks of the kd-ordered set whose box meets the join box,
+//                  all 64 lanes testing one candidate each against every
+//                  annulus and check(); a count pass, a scan, a fill pass keep
+//                  the depth-first order of the reference.
 //   k_super        thread per complete match: FP64 superposition, RMSD
-//                  threshold, event compaction.
+//                  threshold, event compaction with the match's rank.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -149,8 +151,12 @@ __global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevS
 //     final queue fills [l+k+1, r-1] from its second element on, its front
 //     goes to r, the pivot to l+k.
 
+// Two workgroup shapes: one wave for sets of up to KD_SMALL entries (LDS
+// 15 KB, so many sets per CU), 256 threads for larger ones (LDS up to
+// KD_CAP entries, a global scratch area beyond).
+#define KD_SMALL 256
+#define KD_CAP 1024
 #define KD_THREADS 256
-#define KD_CAP 1024          // largest set arranged in LDS
 // per set entry: 3 coordinates and 9 ints of work arrays (60 bytes), for sets
 // too large for LDS
 #define KD_SCRATCH_BYTES_PER_ENTRY 64
@@ -165,20 +171,21 @@ struct KdWork {
 };
 
 // in-place exclusive block scan of v[0, n)
+template <int NT>
 __device__ static void kd_scan(int *v, int n, int *partial)
 {
     const int tid = threadIdx.x;
-    const int per = (n + KD_THREADS - 1) / KD_THREADS;
+    const int per = (n + NT - 1) / NT;
     const int b0 = tid * per, b1 = min(n, b0 + per);
     int cnt = 0;
     for (int p = b0; p < b1; p++) cnt += v[p];
     partial[tid] = cnt;
     __syncthreads();
     if (tid < JAM_WAVE) {
-        int x[KD_THREADS / JAM_WAVE], sum = 0;
+        int x[NT / JAM_WAVE], sum = 0;
 #pragma unroll
-        for (int q = 0; q < KD_THREADS / JAM_WAVE; q++) {
-            x[q] = partial[(KD_THREADS / JAM_WAVE) * tid + q];
+        for (int q = 0; q < NT / JAM_WAVE; q++) {
+            x[q] = partial[(NT / JAM_WAVE) * tid + q];
             sum += x[q];
         }
         int incl = sum;
@@ -188,8 +195,8 @@ __device__ static void kd_scan(int *v, int n, int *partial)
         }
         int run = incl - sum;
 #pragma unroll
-        for (int q = 0; q < KD_THREADS / JAM_WAVE; q++) {
-            partial[(KD_THREADS / JAM_WAVE) * tid + q] = run;
+        for (int q = 0; q < NT / JAM_WAVE; q++) {
+            partial[(NT / JAM_WAVE) * tid + q] = run;
             run += x[q];
         }
     }
@@ -210,11 +217,12 @@ __device__ static inline int kd_cmp3(const double *c, int x, int y)
 }
 
 // Arranges elements 0..n-1 into the reference's leaf order, written to out.
+template <int NT>
 __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
 {
     const int tid = threadIdx.x;
     int cur = 0;
-    for (int p = tid; p < n; p += KD_THREADS) {
+    for (int p = tid; p < n; p += NT) {
         w.arr[0][p] = p;
         w.rlo[p] = 0;
         w.rhi[p] = n;
@@ -224,7 +232,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
         const double *c = w.c[d % 3];
         if (tid == 0) flag[0] = 0;
         __syncthreads();
-        for (int p = tid; p < n; p += KD_THREADS) {
+        for (int p = tid; p < n; p += NT) {
             if (w.rlo[p] != p) continue;
             const int hi = w.rhi[p];
             if (hi - p >= 2) {
@@ -241,7 +249,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             int *a = w.arr[cur];
             int *an = w.arr[cur ^ 1];
             // pivot: median of three, moved to the right end of the window
-            for (int p = tid; p < n; p += KD_THREADS) {
+            for (int p = tid; p < n; p += NT) {
                 if (w.rlo[p] != p || w.wr[p] < 0) continue;
                 const int l = w.wl[p], r = w.wr[p], mid = (l + r + 1) / 2;
                 int piv;
@@ -254,7 +262,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             }
             __syncthreads();
             // "<= pivot" flags of the windows
-            for (int i = tid; i < n; i += KD_THREADS) {
+            for (int i = tid; i < n; i += NT) {
                 const int R = w.rlo[i];
                 const int r = w.wr[R];
                 int b = 0;
@@ -262,9 +270,9 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
                 w.pre1[i] = b;
             }
             __syncthreads();
-            kd_scan(w.pre1, n, partial);
+            kd_scan<NT>(w.pre1, n, partial);
             // re-push flags: a "<=" element after the first ">" one
-            for (int i = tid; i < n; i += KD_THREADS) {
+            for (int i = tid; i < n; i += NT) {
                 const int R = w.rlo[i];
                 const int l = w.wl[R], r = w.wr[R];
                 int rot = 0;
@@ -275,9 +283,9 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
                 w.pre2[i] = rot;
             }
             __syncthreads();
-            kd_scan(w.pre2, n, partial);
+            kd_scan<NT>(w.pre2, n, partial);
             // pushes: ">" elements are new pushes, re-pushes point to push q
-            for (int i = tid; i < n; i += KD_THREADS) {
+            for (int i = tid; i < n; i += NT) {
                 const int R = w.rlo[i];
                 const int l = w.wl[R], r = w.wr[R];
                 if (r < 0 || i < l || i >= r) continue;
@@ -293,7 +301,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             for (;;) {
                 if (tid == 0) flag[1] = 0;
                 __syncthreads();
-                for (int i = tid; i < n; i += KD_THREADS) {
+                for (int i = tid; i < n; i += NT) {
                     const int R = w.rlo[i];
                     const int l = w.wl[R], r = w.wr[R];
                     if (r < 0 || i < l || i >= r) continue;
@@ -314,7 +322,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
                 if (!again) break;
             }
             // scatter into the other buffer
-            for (int i = tid; i < n; i += KD_THREADS) {
+            for (int i = tid; i < n; i += NT) {
                 const int R = w.rlo[i];
                 const int l = w.wl[R], r = w.wr[R];
                 if (r < 0 || i < l || i > r) {
@@ -337,7 +345,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             // next window of each QuickSelect
             if (tid == 0) flag[1] = 0;
             __syncthreads();
-            for (int p = tid; p < n; p += KD_THREADS) {
+            for (int p = tid; p < n; p += NT) {
                 if (w.rlo[p] != p || w.wr[p] < 0) continue;
                 const int l = w.wl[p], r = w.wr[p];
                 const int pos = l + (w.pre1[r] - w.pre1[l]);
@@ -357,7 +365,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             if (!more) break;
         }
         // split every range at its median position
-        for (int p = tid; p < n; p += KD_THREADS) {
+        for (int p = tid; p < n; p += NT) {
             const int lo = w.rlo[p], hi = w.rhi[p];
             if (hi - lo < 2) continue;
             const int m = lo + (hi - lo) / 2;
@@ -367,192 +375,265 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
         __syncthreads();
     }
     const int *fin = w.arr[cur];
-    for (int p = tid; p < n; p += KD_THREADS) out[p] = fin[p];
+    for (int p = tid; p < n; p += NT) out[p] = fin[p];
     __syncthreads();
 }
 
-// One workgroup per candidate set. Sets of up to KD_CAP entries work in LDS,
-// larger ones in a global scratch area of the same layout (scr).
-__global__ void __launch_bounds__(KD_THREADS) k_kdorder(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr)
+// Candidate sets of size in (lo, hi], one workgroup of NT threads at a time,
+// in a grid-stride loop over the sets. Sets of up to CAP entries work in
+// LDS, larger ones in a global scratch area of the same layout (scr).
+template <int NT, int CAP>
+__global__ void __launch_bounds__(NT) k_kdorder(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
+                                                int lo, int hi)
 {
-    __shared__ double lc[3][KD_CAP];
-    __shared__ int lint[9][KD_CAP];
-    __shared__ int partial[KD_THREADS];
+    __shared__ double lc[3][CAP];
+    __shared__ int lint[9][CAP];
+    __shared__ int partial[NT];
     __shared__ int flag[2];
-    const int s = blockIdx.x;
     const int tid = threadIdx.x;
-    if (s >= st.n_sets) return;
     if (*st.total > st.cap) return;
-    const int n = st.cnt[s];
-    if (n == 0) return;
-    const int64_t off = st.off[s];
-    const int m = s / tp.n_types;
-    const int abase = ms.atom_off[m];
-    double *cb;
-    int *ib;
-    size_t stride;
-    if (n <= KD_CAP) {
-        cb = &lc[0][0];
-        ib = &lint[0][0];
-        stride = KD_CAP;
-    } else {
-        cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
-        ib = (int *)(cb + 3 * (size_t)n);
-        stride = (size_t)n;
-    }
-    KdWork w;
-    for (int a = 0; a < 3; a++) w.c[a] = cb + a * stride;
-    w.arr[0] = ib;
-    w.arr[1] = ib + stride;
-    w.pre1 = ib + 2 * stride;
-    w.pre2 = ib + 3 * stride;
-    w.src = ib + 4 * stride;
-    w.rlo = ib + 5 * stride;
-    w.rhi = ib + 6 * stride;
-    w.wl = ib + 7 * stride;
-    w.wr = ib + 8 * stride;
-    for (int i = tid; i < n; i += KD_THREADS) {
-        const int g = abase + st.list[off + i];
-        w.c[0][i] = ms.xyz[3 * (size_t)g + 0];
-        w.c[1][i] = ms.xyz[3 * (size_t)g + 1];
-        w.c[2][i] = ms.xyz[3 * (size_t)g + 2];
-    }
-    __syncthreads();
-    int *order = st.kidx + off;
-    kd_exact(w, n, order, partial, flag);
-    // DFS visits right subtrees first: position p of the visit order is
-    // element n-1-p of the arrangement.
-    const int64_t bbase = off / JAM_WAVE + s;
-    const int lane = lane_id();
-    const int wave = tid / JAM_WAVE;
-    for (int b0 = wave * JAM_WAVE; b0 < n; b0 += KD_THREADS) {
-        const int p = b0 + lane;
-        double x = 0.0, y = 0.0, z = 0.0;
-        const bool ok = p < n;
-        if (ok) {
-            const int li = order[n - 1 - p];
-            const int atom = st.list[off + li];
-            const int g = abase + atom;
-            x = w.c[0][li];
-            y = w.c[1][li];
-            z = w.c[2][li];
-            st.kx[off + p] = x;
-            st.ky[off + p] = y;
-            st.kz[off + p] = z;
-            st.katom[off + p] = atom;
-            st.kres[off + p] = ms.resseq[g];
-            st.kch[off + p] = ms.chain[g];
+    for (int s = blockIdx.x; s < st.n_sets; s += gridDim.x) {
+        const int n = st.cnt[s];
+        if (n <= lo || n > hi) continue;
+        const int64_t off = st.off[s];
+        const int m = s / tp.n_types;
+        const int abase = ms.atom_off[m];
+        double *cb;
+        int *ib;
+        size_t stride;
+        if (n <= CAP) {
+            cb = &lc[0][0];
+            ib = &lint[0][0];
+            stride = CAP;
+        } else {
+            cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
+            ib = (int *)(cb + 3 * (size_t)n);
+            stride = (size_t)n;
         }
-        double lo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
-        double hi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
-        for (int sh = 32; sh >= 1; sh >>= 1) {
+        KdWork w;
+        for (int a = 0; a < 3; a++) w.c[a] = cb + a * stride;
+        w.arr[0] = ib;
+        w.arr[1] = ib + stride;
+        w.pre1 = ib + 2 * stride;
+        w.pre2 = ib + 3 * stride;
+        w.src = ib + 4 * stride;
+        w.rlo = ib + 5 * stride;
+        w.rhi = ib + 6 * stride;
+        w.wl = ib + 7 * stride;
+        w.wr = ib + 8 * stride;
+        for (int i = tid; i < n; i += NT) {
+            const int g = abase + st.list[off + i];
+            w.c[0][i] = ms.xyz[3 * (size_t)g + 0];
+            w.c[1][i] = ms.xyz[3 * (size_t)g + 1];
+            w.c[2][i] = ms.xyz[3 * (size_t)g + 2];
+        }
+        __syncthreads();
+        int *order = st.kidx + off;
+        kd_exact<NT>(w, n, order, partial, flag);
+        // DFS visits right subtrees first: position p of the visit order is
+        // element n-1-p of the arrangement.
+        const int64_t bbase = off / JAM_WAVE + s;
+        const int lane = lane_id();
+        const int wave = tid / JAM_WAVE;
+        for (int b0 = wave * JAM_WAVE; b0 < n; b0 += NT) {
+            const int p = b0 + lane;
+            double x = 0.0, y = 0.0, z = 0.0;
+            const bool ok = p < n;
+            if (ok) {
+                const int li = order[n - 1 - p];
+                const int atom = st.list[off + li];
+                const int g = abase + atom;
+                x = w.c[0][li];
+                y = w.c[1][li];
+                z = w.c[2][li];
+                st.kx[off + p] = x;
+                st.ky[off + p] = y;
+                st.kz[off + p] = z;
+                st.katom[off + p] = atom;
+                st.kres[off + p] = ms.resseq[g];
+                st.kch[off + p] = ms.chain[g];
+            }
+            double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
+            double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
+            for (int sh = 32; sh >= 1; sh >>= 1) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                lo[d] = fmin(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
-                hi[d] = fmax(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
+                for (int d = 0; d < 3; d++) {
+                    blo[d] = fmin(blo[d], __shfl_xor(blo[d], sh, JAM_WAVE));
+                    bhi[d] = fmax(bhi[d], __shfl_xor(bhi[d], sh, JAM_WAVE));
+                }
+            }
+            if (lane == 0) {
+                double *bb = st.bbox + 6 * (bbase + b0 / JAM_WAVE);
+                bb[0] = blo[0];
+                bb[1] = blo[1];
+                bb[2] = blo[2];
+                bb[3] = bhi[0];
+                bb[4] = bhi[1];
+                bb[5] = bhi[2];
             }
         }
-        if (lane == 0) {
-            double *bb = st.bbox + 6 * (bbase + b0 / JAM_WAVE);
-            bb[0] = lo[0];
-            bb[1] = lo[1];
-            bb[2] = lo[2];
-            bb[3] = hi[0];
-            bb[4] = hi[1];
-            bb[5] = hi[2];
-        }
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// join (the scanner)
+// pairs: level order of every (template, molecule) pair
 // ---------------------------------------------------------------------------
 
+// Thread per pair: candidate-set sizes, the level order (stable by size when
+// reordering, _CandidateSet_size_compare_r, patches/Scanner.c.patch:98-108),
+// and the number of level-0 partial matches. A pair with an empty set has
+// none (Scanner_reuse returns NULL, patches/Scanner.c.patch:162-166).
+__global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs pr, QueryParams qp)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pr.n_pairs) return;
+    const int T = tp.n_templates, U = tp.n_types;
+    const int m = p / T, t = p % T;
+    const int tbase = tp.atom_off[t];
+    const int n = tp.atom_off[t + 1] - tbase;
+    unsigned char *ord = pr.ord + (size_t)p * JAM_MAX_LEVELS;
+    int chunks = 0;
+    if (n > 0 && n <= JAM_MAX_LEVELS) {
+        int cnt[JAM_MAX_LEVELS];
+        bool empty = false;
+        for (int k = 0; k < n; k++) {
+            cnt[k] = st.cnt[m * U + tp.atom_type[tbase + k]];
+            empty |= cnt[k] == 0;
+            ord[k] = (unsigned char)k;
+        }
+        if (!empty) {
+            if (qp.reorder) {
+                for (int i = 1; i < n; i++) {
+                    const unsigned char v = ord[i];
+                    int j = i - 1;
+                    while (j >= 0 && (cnt[ord[j]] > cnt[v] || (cnt[ord[j]] == cnt[v] && ord[j] > v))) {
+                        ord[j + 1] = ord[j];
+                        j--;
+                    }
+                    ord[j + 1] = v;
+                }
+            }
+            // a one-atom template steps past entry 0 before its first match
+            const int c0 = cnt[ord[0]] - (n == 1 ? 1 : 0);
+            chunks = c0 > 0 ? c0 : 0;
+        }
+    }
+    pr.nchunk[p] = chunks;
+    pr.n1[p] = (n == 1) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_items(DevPairs pr)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pr.n_pairs) return;
+    const int c = pr.nchunk[p];
+    const long long b = pr.item_base[p];
+    for (int i = 0; i < c; i++) pr.item_pair[b + i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// join (the scanner), one tree level at a time
+// ---------------------------------------------------------------------------
+//
+// The reference enumerates complete matches depth first
+// (Scanner_next, patches/Scanner.c.patch:232-334): level 0 walks its
+// candidate list, level L >= 1 the candidates of its set inside every annulus
+// around the atoms placed at levels < L, in kd DFS order, filtered by check().
+// Here every partial match of level L-1 is extended to level L by one wave,
+// all partials in parallel. Children are written after a scan of their
+// counts, so each level's list stays in depth-first order: parents in order,
+// the children of a parent in kd order. The complete matches of a pair are
+// thus contiguous and in the reference's order.
+
+// Level-0 partials: the candidates of every pair's level-0 list, in order.
+__global__ void __launch_bounds__(256) k_level0(DevPairs pr, DevFront fr)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pr.n_items) return;
+    const int p = pr.item_pair[i];
+    const int n1 = pr.n1[p];
+    fr.out_pair[i] = p;
+    fr.out_pos[i * fr.stride] = (int)(i - pr.item_base[p]) + n1;
+}
+
 template <int MAXN>
-struct JoinLds {
-    int ord[MAXN];            // level -> template atom
-    int lvl[MAXN];            // template atom -> level
-    int cnt[MAXN];            // per level: set size
-    long long off[MAXN];      // per level: set offset
-    long long bbase[MAXN];    // per level: first block box
-    int tch[MAXN], tres[MAXN];  // per level: template chain / resSeq
-    double px[MAXN], py[MAXN], pz[MAXN];  // per level: placed atom
-    int pch[MAXN], pres[MAXN], patom[MAXN];
-    double box[MAXN][6];
-    double r2lo[MAXN * (MAXN - 1) / 2], r2hi[MAXN * (MAXN - 1) / 2];
-    unsigned long long blk[MAXN], surv[MAXN];
-    int grp[MAXN], cur[MAXN];
+struct ExpandLds {
+    double px[MAXN], py[MAXN], pz[MAXN];
+    int pch[MAXN], pres[MAXN];
+    double r2lo[MAXN], r2hi[MAXN];
+    int tch[MAXN], tres[MAXN];
 };
 
-__device__ static inline int tri(int L) { return L * (L - 1) / 2; }
-
-template <int MAXN>
-__device__ static inline unsigned long long test_blocks(const DevSets &st, JoinLds<MAXN> &S, int L, int g)
+// One wave extends partial i of level L-1 by the candidates of level L.
+// COUNT: number of children. FILL: write the children (parent row + new kd
+// position) to the next level or, when they complete the template, to the
+// list of complete matches.
+template <int MAXN, bool FILL>
+__device__ static void expand(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
+                              const DevFront &fr, const QueryParams &qp, ExpandLds<MAXN> &S, long long i)
 {
     const int lane = lane_id();
-    const int nblk = (S.cnt[L] + JAM_WAVE - 1) / JAM_WAVE;
-    const int b = g * JAM_WAVE + lane;
-    bool ok = false;
-    if (b < nblk) {
-        const double *bb = st.bbox + 6 * (S.bbase[L] + b);
-        ok = true;
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-            if (bb[d] > S.box[L][3 + d] || S.box[L][d] > bb[3 + d]) ok = false;
-    }
-    return ballot(ok);
-}
-
-template <int MAXN>
-__device__ static inline unsigned long long scan_block(const DevSets &st, JoinLds<MAXN> &S, int L, int blk,
-                                                       int ignore)
-{
-    const int lane = lane_id();
-    const int p = blk * JAM_WAVE + lane;
-    bool ok = p < S.cnt[L];
-    if (ok) {
-        const long long e = S.off[L] + p;
-        const double x = st.kx[e], y = st.ky[e], z = st.kz[e];
-        const int ch = st.kch[e], res = st.kres[e];
-        const int t0 = tri(L);
-        const int tchL = S.tch[L], tresL = S.tres[L];
-        for (int j = 0; j < L; j++) {
-            // _Annulus_po: squared distance accumulated x, y, z from 0.0
-            double tx = S.px[j] - x, ty = S.py[j] - y, tz = S.pz[j] - z;
-            double s = 0.0;
-            s += tx * tx;
-            s += ty * ty;
-            s += tz * tz;
-            if (s < S.r2lo[t0 + j] || s > S.r2hi[t0 + j]) ok = false;
-            // TessTemplate_check
-            const bool same_chain = S.pch[j] == ch;
-            const bool tsame_chain = S.tch[j] == tchL;
-            if (ignore == 0) {
-                if (same_chain != tsame_chain) ok = false;
-                if (!same_chain) continue;
+    const int L = fr.level;
+    const int p = fr.in_pair[i];
+    const int T = tp.n_templates, U = tp.n_types;
+    const int m = p / T, t = p % T;
+    const int tbase = tp.atom_off[t];
+    const int n = tp.atom_off[t + 1] - tbase;
+    if (L >= n) {
+        // a one-atom template is complete at level 0: the partial itself
+        if (FILL) {
+            if (lane == 0) {
+                const long long r = fr.done_base + fr.off_done[i];
+                fr.done_pair[r] = p;
+                fr.done_pos[r * fr.stride] = fr.in_pos[i * fr.stride];
             }
-            const bool same_res = S.pres[j] == res;
-            const bool tsame_res = S.tres[j] == tresL;
-            if (same_res != tsame_res) ok = false;
-            if (ignore != 2 && same_res && !same_chain) ok = false;
+        } else if (lane == 0) {
+            fr.cnt_next[i] = 0;
+            fr.cnt_done[i] = 1;
         }
+        return;
     }
-    return ballot(ok);
-}
-
-// Opens level L: annulus windows against every earlier level
-// (patches/Scanner.c.patch:274-294, Annulus_reuse patches/Annulus.c.patch:107-140),
-// the join box (Join_computeBox, patches/Join.c.patch:154-196), block mask.
-template <int MAXN>
-__device__ static inline void open_level(const DevTpls &tp, const DevSets &st, JoinLds<MAXN> &S, int L,
-                                         int t, int tbase, int n, const QueryParams &qp)
-{
-    const int lane = lane_id();
+    const unsigned char *ord = pr.ord + (size_t)p * JAM_MAX_LEVELS;
+    const int kL = ord[L];
+    const int setL = m * U + tp.atom_type[tbase + kL];
+    const int cntL = st.cnt[setL];
+    const long long offL = st.off[setL];
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
     if (lane < L) {
-        const int i = S.ord[lane], k = S.ord[L];
-        const double d = tp.dist[tp.dist_off[t] + i * n + k];
-        double tau = qp.distance_cutoff + tp.weight[tbase + i] + tp.weight[tbase + k];
+        // the atom placed at level `lane`
+        const int k = ord[lane];
+        const int sset = m * U + tp.atom_type[tbase + k];
+        const int pos = fr.in_pos[i * fr.stride + lane];
+        double x, y, z;
+        int ch, res;
+        if (lane == 0) {
+            const int atom = st.list[st.off[sset] + pos];
+            const size_t g = (size_t)(ms.atom_off[m] + atom);
+            x = ms.xyz[3 * g + 0];
+            y = ms.xyz[3 * g + 1];
+            z = ms.xyz[3 * g + 2];
+            ch = ms.chain[g];
+            res = ms.resseq[g];
+        } else {
+            const long long e = st.off[sset] + pos;
+            x = st.kx[e];
+            y = st.ky[e];
+            z = st.kz[e];
+            ch = st.kch[e];
+            res = st.kres[e];
+        }
+        S.px[lane] = x;
+        S.py[lane] = y;
+        S.pz[lane] = z;
+        S.pch[lane] = ch;
+        S.pres[lane] = res;
+        S.tch[lane] = tp.chain[tbase + k];
+        S.tres[lane] = tp.resseq[tbase + k];
+        // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
+        // Annulus_reuse patches/Annulus.c.patch:107-140
+        const double d = tp.dist[tp.dist_off[t] + k * n + kL];
+        double tau = qp.distance_cutoff + tp.weight[tbase + k] + tp.weight[tbase + kL];
         if (tau > qp.max_dynamic_distance) tau = qp.max_dynamic_distance;
         double a = d - tau, b = d + tau;
         if (a < 0.5) a = 0.5;
@@ -563,15 +644,16 @@ __device__ static inline void open_level(const DevTpls &tp, const DevSets &st, J
         }
         if (a < 0.0) a = 0.0;
         if (b < 0.0) b = 0.0;
-        S.r2lo[tri(L) + lane] = a * a;
-        S.r2hi[tri(L) + lane] = b * b;
-        lo[0] = S.px[lane] - b;
-        lo[1] = S.py[lane] - b;
-        lo[2] = S.pz[lane] - b;
-        hi[0] = S.px[lane] + b;
-        hi[1] = S.py[lane] + b;
-        hi[2] = S.pz[lane] + b;
+        S.r2lo[lane] = a * a;
+        S.r2hi[lane] = b * b;
+        lo[0] = x - b;
+        lo[1] = y - b;
+        lo[2] = z - b;
+        hi[0] = x + b;
+        hi[1] = y + b;
+        hi[2] = z + b;
     }
+    // join box (Join_computeBox, patches/Join.c.patch:154-196)
     for (int sh = 32; sh >= 1; sh >>= 1) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -579,210 +661,121 @@ __device__ static inline void open_level(const DevTpls &tp, const DevSets &st, J
             hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
         }
     }
-    if (lane == 0) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            S.box[L][d] = lo[d];
-            S.box[L][3 + d] = hi[d];
-        }
-    }
+    const int tchL = tp.chain[tbase + kL], tresL = tp.resseq[tbase + kL];
     __syncthreads();
-    const unsigned long long bm = test_blocks<MAXN>(st, S, L, 0);
-    if (lane == 0) {
-        S.grp[L] = 0;
-        S.blk[L] = bm;
-        S.surv[L] = 0ull;
-        S.cur[L] = -1;
-    }
-    __syncthreads();
-}
-
-// Next candidate of level L in kd DFS order, as a position in the set, or -1.
-template <int MAXN>
-__device__ static inline int next_candidate(const DevSets &st, JoinLds<MAXN> &S, int L, int ignore)
-{
-    const int lane = lane_id();
-    unsigned long long surv = S.surv[L];
-    while (surv == 0ull) {
-        unsigned long long blk = S.blk[L];
-        int g = S.grp[L];
-        const int nblk = (S.cnt[L] + JAM_WAVE - 1) / JAM_WAVE;
-        while (blk == 0ull) {
-            g++;
-            if (g * JAM_WAVE >= nblk) {
-                if (lane == 0) {
-                    S.grp[L] = g;
-                    S.blk[L] = 0ull;
-                }
-                __syncthreads();
-                return -1;
+    const int ignore = qp.ignore_chain;
+    const int nblk = (cntL + JAM_WAVE - 1) / JAM_WAVE;
+    const long long bbase = offL / JAM_WAVE + setL;
+    const bool last = (L == n - 1);
+    long long wout = 0;
+    if (FILL) wout = last ? fr.done_base + fr.off_done[i] : fr.off_next[i];
+    int count = 0;
+    for (int g = 0; g * JAM_WAVE < nblk; g++) {
+        // blocks of this group whose box meets the join box
+        unsigned long long bm;
+        if (nblk == 1) {
+            bm = 1ull;
+        } else {
+            const int b = g * JAM_WAVE + lane;
+            bool ok = false;
+            if (b < nblk) {
+                const double *bb = st.bbox + 6 * (bbase + b);
+                ok = !(bb[0] > hi[0] || lo[0] > bb[3] || bb[1] > hi[1] || lo[1] > bb[4] || bb[2] > hi[2] ||
+                       lo[2] > bb[5]);
             }
-            blk = test_blocks<MAXN>(st, S, L, g);
+            bm = ballot(ok);
         }
-        const int b = first_bit(blk);
-        blk &= blk - 1;
-        const int cur = g * JAM_WAVE + b;
-        surv = scan_block<MAXN>(st, S, L, cur, ignore);
-        if (lane == 0) {
-            S.grp[L] = g;
-            S.blk[L] = blk;
-            S.cur[L] = cur;
-        }
-        __syncthreads();
-    }
-    const int i = first_bit(surv);
-    surv &= surv - 1;
-    const int cur = S.cur[L];
-    __syncthreads();
-    if (lane == 0) S.surv[L] = surv;
-    __syncthreads();
-    return cur * JAM_WAVE + i;
-}
-
-template <int MAXN>
-__device__ static inline void place(const DevSets &st, JoinLds<MAXN> &S, int L, int pos)
-{
-    if (lane_id() == 0) {
-        const long long e = S.off[L] + pos;
-        S.px[L] = st.kx[e];
-        S.py[L] = st.ky[e];
-        S.pz[L] = st.kz[e];
-        S.pch[L] = st.kch[e];
-        S.pres[L] = st.kres[e];
-        S.patom[L] = st.katom[e];
-    }
-    __syncthreads();
-}
-
-// One wave per work item = (pair, level-0 range); pair = molecule * T + template.
-template <int MAXN>
-__global__ void __launch_bounds__(64) k_join(DevMols ms, DevTpls tp, DevSets st, DevMatches mt,
-                                             QueryParams qp, int n_items)
-{
-    __shared__ JoinLds<MAXN> S;
-    const int item = blockIdx.x;
-    const int lane = lane_id();
-    if (item >= n_items) return;
-    if (*st.total > st.cap) return;
-    const int G = qp.items_per_pair;
-    const int pair = item / G, g = item % G;
-    const int T = tp.n_templates;
-    const int m = pair / T, t = pair % T;
-    const int tbase = tp.atom_off[t];
-    const int n = tp.atom_off[t + 1] - tbase;
-    const int U = tp.n_types;
-    int seq = 0;
-    if (n <= 0 || n > MAXN) {
-        if (lane == 0) mt.item_cm[item] = 0;
-        return;
-    }
-    // candidate-set sizes and the level order (_CandidateSet_size_compare_r,
-    // patches/Scanner.c.patch:98-108: by size, then template index)
-    int mycnt = 0;
-    long long myoff = 0;
-    if (lane < n) {
-        const int sset = m * U + tp.atom_type[tbase + lane];
-        mycnt = st.cnt[sset];
-        myoff = st.off[sset];
-    }
-    const bool empty = ballot(lane < n && mycnt == 0) != 0ull;
-    if (empty) {
-        if (lane == 0) mt.item_cm[item] = 0;
-        return;
-    }
-    int rank = lane;
-    if (qp.reorder) {
-        rank = 0;
-        for (int j = 0; j < n; j++) {
-            const int cj = __shfl(mycnt, j, JAM_WAVE);
-            if (lane < n && (cj < mycnt || (cj == mycnt && j < lane))) rank++;
-        }
-    }
-    if (lane < n) {
-        S.ord[rank] = lane;
-        S.lvl[lane] = rank;
-        S.cnt[rank] = mycnt;
-        S.off[rank] = myoff;
-        S.bbase[rank] = myoff / JAM_WAVE + (m * U + tp.atom_type[tbase + lane]);
-        S.tch[rank] = tp.chain[tbase + lane];
-        S.tres[rank] = tp.resseq[tbase + lane];
-    }
-    __syncthreads();
-    // level-0 range of this work item
-    const int c0 = S.cnt[0];
-    int a = (int)(((long long)c0 * g) / G), b = (int)(((long long)c0 * (g + 1)) / G);
-    if (n == 1 && a == 0) a = 1;   // the reference's first advance steps past entry 0
-    const long long off0 = S.off[0];
-    const int abase = ms.atom_off[m];
-    const int o0set = m * U + tp.atom_type[tbase + S.ord[0]];
-    (void)o0set;
-    for (int i0 = a; i0 < b; i0++) {
-        if (lane == 0) {
-            const int atom = st.list[off0 + i0];
-            const size_t gi = (size_t)(abase + atom);
-            S.px[0] = ms.xyz[3 * gi + 0];
-            S.py[0] = ms.xyz[3 * gi + 1];
-            S.pz[0] = ms.xyz[3 * gi + 2];
-            S.pch[0] = ms.chain[gi];
-            S.pres[0] = ms.resseq[gi];
-            S.patom[0] = atom;
-        }
-        __syncthreads();
-        int L = 1;
-        bool complete = (n == 1);
-        if (!complete) open_level<MAXN>(tp, st, S, 1, t, tbase, n, qp);
-        while (true) {
-            if (complete) {
-                // emit one complete match
-                unsigned long long slot = 0;
-                if (lane == 0) slot = atomicAdd(mt.count, 1ull);
-                slot = __shfl(slot, 0, JAM_WAVE);
-                if (slot < (unsigned long long)mt.cap) {
-                    if (lane == 0) {
-                        mt.item[slot] = item;
-                        mt.seq[slot] = seq;
+        while (bm) {
+            const int blk = g * JAM_WAVE + first_bit(bm);
+            bm &= bm - 1;
+            const int q = blk * JAM_WAVE + lane;
+            bool ok = q < cntL;
+            if (ok) {
+                const long long e = offL + q;
+                const double x = st.kx[e], y = st.ky[e], z = st.kz[e];
+                const int ch = st.kch[e], res = st.kres[e];
+                for (int j = 0; j < L; j++) {
+                    // _Annulus_po: squared distance summed x, y, z from 0.0
+                    double tx = S.px[j] - x, ty = S.py[j] - y, tz = S.pz[j] - z;
+                    double s2 = 0.0;
+                    s2 += tx * tx;
+                    s2 += ty * ty;
+                    s2 += tz * tz;
+                    if (s2 < S.r2lo[j] || s2 > S.r2hi[j]) ok = false;
+                    // TessTemplate_check
+                    const bool same_chain = S.pch[j] == ch;
+                    const bool tsame_chain = S.tch[j] == tchL;
+                    if (ignore == 0) {
+                        if (same_chain != tsame_chain) ok = false;
+                        if (!same_chain) continue;
                     }
-                    if (lane < n) mt.atoms[slot * (size_t)mt.stride + lane] = S.patom[S.lvl[lane]];
+                    const bool same_res = S.pres[j] == res;
+                    const bool tsame_res = S.tres[j] == tresL;
+                    if (same_res != tsame_res) ok = false;
+                    if (ignore != 2 && same_res && !same_chain) ok = false;
                 }
-                seq++;
-                complete = false;
-                if (n == 1) break;
             }
-            const int pos = next_candidate<MAXN>(st, S, L, qp.ignore_chain);
-            if (pos < 0) {
-                if (--L == 0) break;
-                continue;
+            const unsigned long long mask = ballot(ok);
+            if (FILL && ok) {
+                const long long r = wout + count + rank_below(mask);
+                int *row = last ? fr.done_pos + r * fr.stride : fr.out_pos + r * fr.stride;
+                for (int l = 0; l < L; l++) row[l] = fr.in_pos[i * fr.stride + l];
+                row[L] = q;
+                if (last) fr.done_pair[r] = p;
+                else fr.out_pair[r] = p;
             }
-            place<MAXN>(st, S, L, pos);
-            if (L == n - 1) {
-                complete = true;
-                continue;
-            }
-            L++;
-            open_level<MAXN>(tp, st, S, L, t, tbase, n, qp);
+            count += __popcll(mask);
         }
     }
-    if (lane == 0) mt.item_cm[item] = seq;
+    if (!FILL && lane == 0) {
+        fr.cnt_next[i] = last ? 0 : count;
+        fr.cnt_done[i] = last ? count : 0;
+    }
+}
+
+template <int MAXN, bool FILL>
+__global__ void __launch_bounds__(64) k_expand(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevFront fr,
+                                               QueryParams qp)
+{
+    __shared__ ExpandLds<MAXN> S;
+    const long long i = blockIdx.x;
+    if (i >= fr.n_in) return;
+    expand<MAXN, FILL>(ms, tp, st, pr, fr, qp, S, i);
 }
 
 // ---------------------------------------------------------------------------
 // superposition + events
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevMatches mt, DevEvents ev,
-                                               QueryParams qp, long long n_matches)
+// first complete match of every pair: the matches of a pair are contiguous
+__global__ void __launch_bounds__(256) k_first(const int *done_pair, long long n, long long *first)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || done_pair[i - 1] != done_pair[i]) first[done_pair[i]] = i;
+}
+
+__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevFront fr,
+                                               DevEvents ev, QueryParams qp, long long n_matches,
+                                               const long long *first)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_matches) return;
-    const int G = qp.items_per_pair;
-    const int item = mt.item[i];
-    const int pair = item / G, g = item % G;
-    const int T = tp.n_templates;
+    const int pair = fr.done_pair[i];
+    const int T = tp.n_templates, U = tp.n_types;
     const int m = pair / T, t = pair % T;
     const int tbase = tp.atom_off[t];
     const int n = tp.atom_off[t + 1] - tbase;
     const int abase = ms.atom_off[m];
-    const int *atoms = mt.atoms + i * (size_t)mt.stride;
+    const unsigned char *ord = pr.ord + (size_t)pair * JAM_MAX_LEVELS;
+    const int *pos = fr.done_pos + i * fr.stride;
+    // molecule-local atom of template atom k: placed at the level l with ord[l] == k
+    int atoms[JAM_MAX_LEVELS];
+    for (int l = 0; l < n; l++) {
+        const int k = ord[l];
+        const int sset = m * U + tp.atom_type[tbase + k];
+        atoms[k] = (l == 0) ? st.list[st.off[sset] + pos[0]] : st.katom[st.off[sset] + pos[l]];
+    }
     auto pa = [&](int k, int d) { return ms.xyz[3 * (size_t)(abase + atoms[k]) + d]; };
     auto pb = [&](int k, int d) { return tp.pos[3 * (size_t)(tbase + k) + d]; };
     double rmsd, rot[9], c0[3], c1[3];
@@ -791,12 +784,10 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevMatche
     bool nan = false;
 #pragma unroll
     for (int k = 0; k < 9; k++) nan |= isnan(rot[k]);
-    long long rank = mt.seq[i];
-    for (int h = pair * G; h < item; h++) rank += mt.item_cm[h];
     const unsigned long long e = atomicAdd(ev.count, 1ull);
     if (e >= (unsigned long long)ev.cap) return;
     ev.pair[e] = pair;
-    ev.rank[e] = rank;
+    ev.rank[e] = i - first[pair];
     ev.flag[e] = nan ? 2 : 1;
     ev.rmsd[e] = rmsd;
 #pragma unroll
