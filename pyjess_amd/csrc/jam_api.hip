@@ -96,7 +96,7 @@ struct jam_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     // per-query workspace
-    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr;
+    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr, kd_tied, kd_ntied;
     DBuf m_item, m_seq, m_atoms, m_count, item_cm, item_pre;
     DBuf p_ord, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
     DBuf f_pair[2], f_pos[2], f_cnt_next, f_cnt_done, f_off_next, f_off_done, f_tot, d_pair, d_pos;
@@ -429,6 +429,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
     if (n_sets > c->nsets_cap) {
         HIP_TRY(c->cnt.reserve(sizeof(int) * (size_t)n_sets));
         HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
+        HIP_TRY(c->kd_tied.reserve(sizeof(int) * (size_t)n_sets));
         c->nsets_cap = n_sets;
     }
     const int64_t blk = cap / JAM_WAVE + n_sets + 1;
@@ -451,6 +452,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
     }
     HIP_TRY(c->total.reserve(sizeof(int64_t)));
+    HIP_TRY(c->kd_ntied.reserve(sizeof(unsigned int)));
     return JAM_OK;
 }
 
@@ -564,6 +566,8 @@ int32_t grow_events(jam_context *c, int64_t cap, int stride)
     HIP_TRY(c->e_count.reserve(sizeof(unsigned long long)));
     return JAM_OK;
 }
+
+__global__ void k_zero_u32(unsigned int *v) { *v = 0u; }
 
 __global__ void k_total(const int *cnt, const int64_t *off, int n, int64_t *total)
 {
@@ -712,13 +716,31 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
         {
-            // small sets: one wave each; larger: 256 threads; grid-stride loops
-            const int g_small = std::min(n_sets, 256 * 40);
+            // fast path for every set: <= 64 entries in registers (one wave),
+            // <= 256 in LDS (one wave), larger with 256 threads (LDS up to
+            // 1024 entries, global scratch beyond); sets with a tie across a
+            // split point are queued for the exact QuickSelect path. All
+            // grid-stride loops over the sets.
+            unsigned char *scr = c->kscr.as<unsigned char>();
+            int *tied = c->kd_tied.as<int>();
+            unsigned int *n_tied = c->kd_ntied.as<unsigned int>();
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, n_tied);
+            const int g_tiny = std::min(n_sets, 256 * 32);
+            const int g_small = std::min(n_sets, 256 * 10);
             const int g_big = std::min(n_sets, 256 * 2);
-            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(g_small), dim3(JAM_WAVE), 0, s, dm,
-                               dt, st, c->kscr.as<unsigned char>(), 0, KD_SMALL);
-            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(g_big), dim3(KD_THREADS), 0, s, dm,
-                               dt, st, c->kscr.as<unsigned char>(), KD_SMALL, INT32_MAX);
+            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, JAM_WAVE>), dim3(g_tiny), dim3(JAM_WAVE), 0, s, dm, dt, st,
+                               scr, 0, JAM_WAVE, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, KD_SMALL>), dim3(g_small), dim3(JAM_WAVE), 0, s, dm, dt,
+                               st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdfast<KD_THREADS, KD_CAP>), dim3(g_big), dim3(KD_THREADS), 0, s, dm, dt,
+                               st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
+            const int g_exact = std::min(n_sets, 256);
+            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, JAM_WAVE>), dim3(g_exact), dim3(JAM_WAVE), 0, s, dm, dt,
+                               st, scr, 0, JAM_WAVE, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(g_exact), dim3(JAM_WAVE), 0, s, dm, dt,
+                               st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(g_exact), dim3(KD_THREADS), 0, s, dm,
+                               dt, st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[2], s));

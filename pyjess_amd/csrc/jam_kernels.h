@@ -157,22 +157,24 @@ __global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevS
 #define KD_SMALL 256
 #define KD_CAP 1024
 #define KD_THREADS 256
-// per set entry: 3 coordinates and 9 ints of work arrays (60 bytes), for sets
-// too large for LDS
-#define KD_SCRATCH_BYTES_PER_ENTRY 64
+// global scratch of sets too large for LDS, per set entry (padded to a power
+// of two for the fast path's sorts: 3 doubles, 9 ints and 1 byte per padded
+// entry, at most 2 * 61 bytes per entry)
+#define KD_SCRATCH_BYTES_PER_ENTRY 128
 
 struct KdWork {
-    double *c[3];            // coordinates by element
-    int *arr[2];             // arrangement, double buffered
+    double *c;               // coordinates: axis a of element e at c[a * stride + e]
+    int *arr;                // arrangement, double buffered at arr and arr + stride
     int *pre1, *pre2;        // exclusive prefixes: "<=" flags, re-push flags
     int *src;                // push -> element (>= 0) or -(push index) - 1
     int *rlo, *rhi;          // range of each position at the current level
     int *wl, *wr;            // QuickSelect window of the range starting here; wr < 0: done
+    int stride;
 };
 
 // in-place exclusive block scan of v[0, n)
 template <int NT>
-__device__ static void kd_scan(int *v, int n, int *partial)
+__device__ __attribute__((always_inline)) static inline void kd_scan(int *v, int n, int *partial)
 {
     const int tid = threadIdx.x;
     const int per = (n + NT - 1) / NT;
@@ -210,7 +212,7 @@ __device__ static void kd_scan(int *v, int n, int *partial)
     __syncthreads();
 }
 
-__device__ static inline int kd_cmp3(const double *c, int x, int y)
+__device__ __attribute__((always_inline)) static inline int kd_cmp3(const double *c, int x, int y)
 {
     const double u = c[x], v = c[y];
     return u < v ? -1 : (u > v ? 1 : 0);
@@ -218,18 +220,18 @@ __device__ static inline int kd_cmp3(const double *c, int x, int y)
 
 // Arranges elements 0..n-1 into the reference's leaf order, written to out.
 template <int NT>
-__device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
+__device__ __attribute__((always_inline)) static inline void kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
 {
     const int tid = threadIdx.x;
     int cur = 0;
     for (int p = tid; p < n; p += NT) {
-        w.arr[0][p] = p;
+        w.arr[p] = p;
         w.rlo[p] = 0;
         w.rhi[p] = n;
     }
     __syncthreads();
     for (int d = 0;; d++) {
-        const double *c = w.c[d % 3];
+        const double *c = w.c + (d % 3) * w.stride;
         if (tid == 0) flag[0] = 0;
         __syncthreads();
         for (int p = tid; p < n; p += NT) {
@@ -246,8 +248,8 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
         __syncthreads();
         if (!flag[0]) break;
         for (;;) {
-            int *a = w.arr[cur];
-            int *an = w.arr[cur ^ 1];
+            int *a = w.arr + cur * w.stride;
+            int *an = w.arr + (cur ^ 1) * w.stride;
             // pivot: median of three, moved to the right end of the window
             for (int p = tid; p < n; p += NT) {
                 if (w.rlo[p] != p || w.wr[p] < 0) continue;
@@ -374,20 +376,336 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
         }
         __syncthreads();
     }
-    const int *fin = w.arr[cur];
+    const int *fin = w.arr + cur * w.stride;
     for (int p = tid; p < n; p += NT) out[p] = fin[p];
     __syncthreads();
 }
 
-// Candidate sets of size in (lo, hi], one workgroup of NT threads at a time,
-// in a grid-stride loop over the sets. Sets of up to CAP entries work in
-// LDS, larger ones in a global scratch area of the same layout (scr).
-template <int NT, int CAP>
-__global__ void __launch_bounds__(NT) k_kdorder(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
-                                                int lo, int hi)
+// Writes the kd arrays of set s from its arrangement `order` (element ids =
+// list positions) and the block boxes: position p of the DFS visit order is
+// element n-1-p of the arrangement (right subtrees first).
+template <int NT>
+__device__ static void kd_emit(const DevMols &ms, const DevSets &st, int s, int n, int64_t off, int abase,
+                               const int *order)
 {
-    __shared__ double lc[3][CAP];
-    __shared__ int lint[9][CAP];
+    const int tid = threadIdx.x;
+    const int64_t bbase = off / JAM_WAVE + s;
+    const int lane = lane_id();
+    const int wave = tid / JAM_WAVE;
+    for (int b0 = wave * JAM_WAVE; b0 < n; b0 += NT) {
+        const int p = b0 + lane;
+        double x = 0.0, y = 0.0, z = 0.0;
+        const bool ok = p < n;
+        if (ok) {
+            const int li = order[n - 1 - p];
+            const int atom = st.list[off + li];
+            const size_t g = (size_t)(abase + atom);
+            x = ms.xyz[3 * g + 0];
+            y = ms.xyz[3 * g + 1];
+            z = ms.xyz[3 * g + 2];
+            st.kx[off + p] = x;
+            st.ky[off + p] = y;
+            st.kz[off + p] = z;
+            st.katom[off + p] = atom;
+            st.kres[off + p] = ms.resseq[g];
+            st.kch[off + p] = ms.chain[g];
+        }
+        double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
+        double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                blo[d] = fmin(blo[d], __shfl_xor(blo[d], sh, JAM_WAVE));
+                bhi[d] = fmax(bhi[d], __shfl_xor(bhi[d], sh, JAM_WAVE));
+            }
+        }
+        if (lane == 0) {
+            double *bb = st.bbox + 6 * (bbase + b0 / JAM_WAVE);
+            bb[0] = blo[0];
+            bb[1] = blo[1];
+            bb[2] = blo[2];
+            bb[3] = bhi[0];
+            bb[4] = bhi[1];
+            bb[5] = bhi[2];
+        }
+    }
+}
+
+// ---- fast path -------------------------------------------------------------
+//
+// When no range of the tree has equal coordinates on both sides of its split
+// point, the left half of every range is its n/2 smallest elements whatever
+// the arrangement, and the leaf order follows from memberships alone: sort
+// every axis once, then at level d cut each range of axis d%3 in half and
+// stably partition the two other axes by side. A tie across a split point
+// sends the set to the exact path (kd_exact).
+
+__device__ static inline int shfl_int(int v, int src) { return __shfl(v, src, JAM_WAVE); }
+
+// value of lane `dst` receives v from this lane (ds_permute: push)
+__device__ static inline int push_int(int v, int dst)
+{
+    return __builtin_amdgcn_ds_permute(dst << 2, v);
+}
+
+__device__ static inline bool key_less(double a, int ia, double b, int ib)
+{
+    return a < b || (a == b && ia < ib);
+}
+
+// Sets of at most 64 entries: one element per lane, everything in registers.
+// Returns false (nothing written) when a tie straddles a split point.
+__device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, int n, int64_t off, int abase)
+{
+    const int lane = lane_id();
+    double c[3] = {INFINITY, INFINITY, INFINITY};
+    if (lane < n) {
+        const size_t g = (size_t)(abase + st.list[off + lane]);
+        c[0] = ms.xyz[3 * g + 0];
+        c[1] = ms.xyz[3 * g + 1];
+        c[2] = ms.xyz[3 * g + 2];
+    }
+    // per axis: element at each position, sorted by (coordinate, element)
+    int perm[3];
+    for (int a = 0; a < 3; a++) {
+        double key = c[a];
+        int id = lane;
+        for (int k = 2; k <= JAM_WAVE; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const double ok = __shfl_xor(key, j, JAM_WAVE);
+                const int oi = __shfl_xor(id, j, JAM_WAVE);
+                const bool up = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                const bool other_less = key_less(ok, oi, key, id);
+                // the lower lane of an ascending pair keeps the smaller key
+                const bool take = (lower == up) ? other_less : !other_less && !(ok == key && oi == id);
+                if (take) {
+                    key = ok;
+                    id = oi;
+                }
+            }
+        }
+        perm[a] = id;
+    }
+    int sb = 0, se = n;
+    for (int d = 0;; d++) {
+        const int a = d % 3;
+        const bool act = lane < n && se - sb >= 2;
+        const int m = sb + (se - sb) / 2;
+        const bool split = act && lane == m;
+        if (ballot(act) == 0ull) break;
+        // tie across the split point: compare with the previous position
+        const int pa = (a == 0) ? perm[0] : (a == 1) ? perm[1] : perm[2];
+        const double ca = (a == 0) ? c[0] : (a == 1) ? c[1] : c[2];
+        const double here = __shfl(ca, pa & 63, JAM_WAVE);
+        const double prev = __shfl_up(here, 1, JAM_WAVE);
+        if (ballot(split && prev == here) != 0ull) return false;
+        // side of every element, delivered to the lane of the element id
+        const int left_pos = act && lane < m ? 1 : 0;
+        const int side_by_elem = push_int(left_pos, pa & 63);
+        for (int b = 0; b < 3; b++) {
+            if (b == a) continue;
+            const int pb = (b == 0) ? perm[0] : (b == 1) ? perm[1] : perm[2];
+            const int sd = shfl_int(side_by_elem, pb & 63);
+            const unsigned long long lm = ballot(act && sd);
+            int np = lane;
+            if (act) {
+                const unsigned long long below = (1ull << lane) - (1ull << sb);
+                const int lc = __popcll(lm & below);
+                np = sd ? sb + lc : m + (lane - sb - lc);
+            }
+            const int moved = push_int(pb, np);
+            if (b == 0) perm[0] = lane < n ? moved : perm[0];
+            else if (b == 1) perm[1] = lane < n ? moved : perm[1];
+            else perm[2] = lane < n ? moved : perm[2];
+        }
+        if (act) {
+            if (lane < m) se = m;
+            else sb = m;
+        }
+    }
+    // DFS order: position p holds element perm[n-1-p]
+    const int src = n - 1 - lane;
+    const int e = shfl_int(perm[0], src < 0 ? 0 : src);
+    const bool ok = lane < n;
+    double x = __shfl(c[0], e & 63, JAM_WAVE);
+    double y = __shfl(c[1], e & 63, JAM_WAVE);
+    double z = __shfl(c[2], e & 63, JAM_WAVE);
+    if (ok) {
+        const int atom = st.list[off + e];
+        const size_t g = (size_t)(abase + atom);
+        st.kx[off + lane] = x;
+        st.ky[off + lane] = y;
+        st.kz[off + lane] = z;
+        st.katom[off + lane] = atom;
+        st.kres[off + lane] = ms.resseq[g];
+        st.kch[off + lane] = ms.chain[g];
+    }
+    double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
+    double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            blo[q] = fmin(blo[q], __shfl_xor(blo[q], sh, JAM_WAVE));
+            bhi[q] = fmax(bhi[q], __shfl_xor(bhi[q], sh, JAM_WAVE));
+        }
+    }
+    if (lane == 0) {
+        double *bb = st.bbox + 6 * (off / JAM_WAVE + s);
+        bb[0] = blo[0];
+        bb[1] = blo[1];
+        bb[2] = blo[2];
+        bb[3] = bhi[0];
+        bb[4] = bhi[1];
+        bb[5] = bhi[2];
+    }
+    return true;
+}
+
+// Fast path of larger sets in LDS (or global scratch): same algorithm with
+// bitonic sorts and block prefix sums. Returns false on a split tie.
+struct KdFast {
+    double *c;               // coordinates, axis-major, stride entries per axis
+    int *perm;               // 3 axes x 2 buffers, stride entries each
+    int *pre;                // prefix of left sides
+    int *sb, *se;            // range of each position
+    unsigned char *side;     // by element
+    int stride;
+};
+
+template <int NT>
+__device__ __attribute__((always_inline)) static inline void kdf_sort(const double *c, int *perm, int n, int n2)
+{
+    const int tid = threadIdx.x;
+    for (int i = tid; i < n2; i += NT) perm[i] = i;
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < n2; i += NT) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int x = perm[i], y = perm[ixj];
+                    const double kx = x < n ? c[x] : INFINITY, ky = y < n ? c[y] : INFINITY;
+                    const bool y_less = key_less(ky, y, kx, x);
+                    const bool up = (i & k) == 0;
+                    if (y_less == up) {
+                        perm[i] = y;
+                        perm[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int NT>
+__device__ __attribute__((always_inline)) static inline bool kd_fast(const KdFast &w, int n, int *out, int *partial, int *flag)
+{
+    const int tid = threadIdx.x;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    const int S = w.stride;
+    for (int a = 0; a < 3; a++) kdf_sort<NT>(w.c + a * S, w.perm + 2 * a * S, n, n2);
+    for (int p = tid; p < n; p += NT) {
+        w.sb[p] = 0;
+        w.se[p] = n;
+    }
+    int cur[3] = {0, 0, 0};
+    __syncthreads();
+    for (int d = 0;; d++) {
+        const int a = d % 3;
+        if (tid == 0) {
+            flag[0] = 0;
+            flag[1] = 0;
+        }
+        __syncthreads();
+        const int *pa = w.perm + (2 * a + cur[a]) * S;
+        const double *ca = w.c + a * S;
+        for (int p = tid; p < n; p += NT) {
+            const int sb = w.sb[p], se = w.se[p];
+            if (se - sb < 2) continue;
+            const int m = sb + (se - sb) / 2;
+            w.side[pa[p]] = p < m ? 1 : 0;
+            if (p == m) {
+                flag[0] = 1;
+                if (ca[pa[m - 1]] == ca[pa[m]]) flag[1] = 1;
+            }
+        }
+        __syncthreads();
+        const int splits = flag[0], tie = flag[1];
+        __syncthreads();
+        if (tie) return false;
+        if (!splits) break;
+        for (int b = 0; b < 3; b++) {
+            if (b == a) continue;
+            const int *src = w.perm + (2 * b + cur[b]) * S;
+            int *dst = w.perm + (2 * b + (cur[b] ^ 1)) * S;
+            for (int p = tid; p < n; p += NT) w.pre[p] = w.side[src[p]];
+            __syncthreads();
+            kd_scan<NT>(w.pre, n, partial);
+            for (int p = tid; p < n; p += NT) {
+                const int sb = w.sb[p], se = w.se[p];
+                int np = p;
+                if (se - sb >= 2) {
+                    const int m = sb + (se - sb) / 2;
+                    const int lc = w.pre[p] - w.pre[sb];
+                    np = w.side[src[p]] ? sb + lc : m + (p - sb - lc);
+                }
+                dst[np] = src[p];
+            }
+            __syncthreads();
+            cur[b] ^= 1;
+        }
+        for (int p = tid; p < n; p += NT) {
+            const int sb = w.sb[p], se = w.se[p];
+            if (se - sb < 2) continue;
+            const int m = sb + (se - sb) / 2;
+            if (p < m) w.se[p] = m;
+            else w.sb[p] = m;
+        }
+        __syncthreads();
+    }
+    const int *fin = w.perm + cur[0] * S;
+    for (int p = tid; p < n; p += NT) out[p] = fin[p];
+    __syncthreads();
+    return true;
+}
+
+// Loads the coordinates of set s into w, runs the fast path and writes the kd
+// arrays. Always inlined, once with w in LDS and once with w in global
+// scratch, so that the compiler knows the address space of every access: a
+// generic pointer into LDS lets it merge two adjacent ints into one 8-byte
+// flat access that is only 4-byte aligned, which LDS does not accept.
+template <int NT>
+__device__ __attribute__((always_inline)) static inline bool kd_fast_set(const DevMols &ms, const DevSets &st, int s,
+                                                                         int n, int64_t off, int abase, KdFast w,
+                                                                         int *partial, int *flag)
+{
+    const int tid = threadIdx.x;
+    for (int i = tid; i < n; i += NT) {
+        const size_t g = (size_t)(abase + st.list[off + i]);
+        w.c[i] = ms.xyz[3 * g + 0];
+        w.c[w.stride + i] = ms.xyz[3 * g + 1];
+        w.c[2 * w.stride + i] = ms.xyz[3 * g + 2];
+    }
+    __syncthreads();
+    int *order = st.kidx + off;
+    const bool done = kd_fast<NT>(w, n, order, partial, flag);
+    if (done) kd_emit<NT>(ms, st, s, n, off, abase, order);
+    return done;
+}
+
+// Fast path of every set; sets with a split tie are appended to `tied` for
+// the exact kernel. Tiers: <= 64 entries in registers (one wave), larger in
+// LDS (NT threads, up to CAP entries), beyond KD_CAP in global scratch.
+template <int NT, int CAP>
+__global__ void __launch_bounds__(NT) k_kdfast(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr, int lo,
+                                               int hi, int *tied, unsigned int *n_tied)
+{
+    __shared__ double lc[CAP > JAM_WAVE ? 3 * CAP : 1];
+    __shared__ int lint[CAP > JAM_WAVE ? 9 * CAP : 1];
+    __shared__ unsigned char lside[CAP > JAM_WAVE ? CAP : 1];
     __shared__ int partial[NT];
     __shared__ int flag[2];
     const int tid = threadIdx.x;
@@ -398,79 +716,100 @@ __global__ void __launch_bounds__(NT) k_kdorder(DevMols ms, DevTpls tp, DevSets 
         const int64_t off = st.off[s];
         const int m = s / tp.n_types;
         const int abase = ms.atom_off[m];
-        double *cb;
-        int *ib;
-        size_t stride;
-        if (n <= CAP) {
-            cb = &lc[0][0];
-            ib = &lint[0][0];
-            stride = CAP;
-        } else {
-            cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
-            ib = (int *)(cb + 3 * (size_t)n);
-            stride = (size_t)n;
+        bool done = true;
+        if (CAP <= JAM_WAVE) {
+            done = kd_wave64(ms, st, s, n, off, abase);
+        } else if (n <= CAP) {
+            KdFast w;
+            w.c = lc;
+            w.perm = lint;
+            w.pre = lint + 6 * CAP;
+            w.sb = lint + 7 * CAP;
+            w.se = lint + 8 * CAP;
+            w.side = lside;
+            w.stride = CAP;
+            done = kd_fast_set<NT>(ms, st, s, n, off, abase, w, partial, flag);
+        } else if (CAP == KD_CAP) {
+            int n2 = 1;
+            while (n2 < n) n2 <<= 1;
+            double *cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
+            int *ib = (int *)(cb + 3 * (size_t)n2);
+            KdFast w;
+            w.c = cb;
+            w.perm = ib;
+            w.pre = ib + 6 * n2;
+            w.sb = ib + 7 * n2;
+            w.se = ib + 8 * n2;
+            w.side = (unsigned char *)(ib + 9 * n2);
+            w.stride = n2;
+            done = kd_fast_set<NT>(ms, st, s, n, off, abase, w, partial, flag);
         }
-        KdWork w;
-        for (int a = 0; a < 3; a++) w.c[a] = cb + a * stride;
-        w.arr[0] = ib;
-        w.arr[1] = ib + stride;
-        w.pre1 = ib + 2 * stride;
-        w.pre2 = ib + 3 * stride;
-        w.src = ib + 4 * stride;
-        w.rlo = ib + 5 * stride;
-        w.rhi = ib + 6 * stride;
-        w.wl = ib + 7 * stride;
-        w.wr = ib + 8 * stride;
-        for (int i = tid; i < n; i += NT) {
-            const int g = abase + st.list[off + i];
-            w.c[0][i] = ms.xyz[3 * (size_t)g + 0];
-            w.c[1][i] = ms.xyz[3 * (size_t)g + 1];
-            w.c[2][i] = ms.xyz[3 * (size_t)g + 2];
-        }
+        if (!done && tid == 0) tied[atomicAdd(n_tied, 1u)] = s;
         __syncthreads();
-        int *order = st.kidx + off;
-        kd_exact<NT>(w, n, order, partial, flag);
-        // DFS visits right subtrees first: position p of the visit order is
-        // element n-1-p of the arrangement.
-        const int64_t bbase = off / JAM_WAVE + s;
-        const int lane = lane_id();
-        const int wave = tid / JAM_WAVE;
-        for (int b0 = wave * JAM_WAVE; b0 < n; b0 += NT) {
-            const int p = b0 + lane;
-            double x = 0.0, y = 0.0, z = 0.0;
-            const bool ok = p < n;
-            if (ok) {
-                const int li = order[n - 1 - p];
-                const int atom = st.list[off + li];
-                const int g = abase + atom;
-                x = w.c[0][li];
-                y = w.c[1][li];
-                z = w.c[2][li];
-                st.kx[off + p] = x;
-                st.ky[off + p] = y;
-                st.kz[off + p] = z;
-                st.katom[off + p] = atom;
-                st.kres[off + p] = ms.resseq[g];
-                st.kch[off + p] = ms.chain[g];
-            }
-            double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
-            double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
-            for (int sh = 32; sh >= 1; sh >>= 1) {
-#pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    blo[d] = fmin(blo[d], __shfl_xor(blo[d], sh, JAM_WAVE));
-                    bhi[d] = fmax(bhi[d], __shfl_xor(bhi[d], sh, JAM_WAVE));
-                }
-            }
-            if (lane == 0) {
-                double *bb = st.bbox + 6 * (bbase + b0 / JAM_WAVE);
-                bb[0] = blo[0];
-                bb[1] = blo[1];
-                bb[2] = blo[2];
-                bb[3] = bhi[0];
-                bb[4] = bhi[1];
-                bb[5] = bhi[2];
-            }
+    }
+}
+
+// Exact path of one set (coordinates loaded into w, QuickSelect simulation,
+// kd arrays written); inlined separately for LDS and global work arrays.
+template <int NT>
+__device__ __attribute__((always_inline)) static inline void kd_exact_set(const DevMols &ms, const DevSets &st,
+                                                                          int s, int n, int64_t off, int abase,
+                                                                          KdWork w, int *partial, int *flag)
+{
+    const int tid = threadIdx.x;
+    const size_t stride = (size_t)w.stride;
+    for (int i = tid; i < n; i += NT) {
+        const int g = abase + st.list[off + i];
+        w.c[i] = ms.xyz[3 * (size_t)g + 0];
+        w.c[stride + i] = ms.xyz[3 * (size_t)g + 1];
+        w.c[2 * stride + i] = ms.xyz[3 * (size_t)g + 2];
+    }
+    __syncthreads();
+    int *order = st.kidx + off;
+    kd_exact<NT>(w, n, order, partial, flag);
+    kd_emit<NT>(ms, st, s, n, off, abase, order);
+}
+
+__device__ static inline KdWork kd_work_at(double *cb, int *ib, size_t stride)
+{
+    KdWork w;
+    w.c = cb;
+    w.arr = ib;
+    w.pre1 = ib + 2 * stride;
+    w.pre2 = ib + 3 * stride;
+    w.src = ib + 4 * stride;
+    w.rlo = ib + 5 * stride;
+    w.rhi = ib + 6 * stride;
+    w.wl = ib + 7 * stride;
+    w.wr = ib + 8 * stride;
+    w.stride = (int)stride;
+    return w;
+}
+
+// Exact path for the sets of `tied` with n in (lo, hi], grid-stride.
+template <int NT, int CAP>
+__global__ void __launch_bounds__(NT) k_kdorder(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
+                                                int lo, int hi, const int *tied, const unsigned int *n_tied)
+{
+    __shared__ double lc[3 * CAP];
+    __shared__ int lint[9 * CAP];
+    __shared__ int partial[NT];
+    __shared__ int flag[2];
+    if (*st.total > st.cap) return;
+    const int count = (int)*n_tied;
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        const int s = tied[q];
+        const int n = st.cnt[s];
+        if (n <= lo || n > hi) continue;
+        const int64_t off = st.off[s];
+        const int m = s / tp.n_types;
+        const int abase = ms.atom_off[m];
+        if (n <= CAP) {
+            kd_exact_set<NT>(ms, st, s, n, off, abase, kd_work_at(lc, lint, CAP), partial, flag);
+        } else if (CAP == KD_CAP) {
+            double *cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);
+            int *ib = (int *)(cb + 3 * (size_t)n);
+            kd_exact_set<NT>(ms, st, s, n, off, abase, kd_work_at(cb, ib, (size_t)n), partial, flag);
         }
         __syncthreads();
     }
