@@ -716,31 +716,33 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
         {
-            // fast path for every set: <= 64 entries in registers (one wave),
-            // <= 256 in LDS (one wave), larger with 256 threads (LDS up to
-            // 1024 entries, global scratch beyond); sets with a tie across a
-            // split point are queued for the exact QuickSelect path. All
-            // grid-stride loops over the sets.
+            // fast path for every set: <= 64 entries in one wave's registers,
+            // up to 1024 with one position per thread (128 ... 1024 threads),
+            // larger in global scratch; sets with a tie across a split point
+            // are queued for the exact QuickSelect path. All grid-stride
+            // loops over the sets.
             unsigned char *scr = c->kscr.as<unsigned char>();
             int *tied = c->kd_tied.as<int>();
             unsigned int *n_tied = c->kd_ntied.as<unsigned int>();
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, n_tied);
-            const int g_tiny = std::min(n_sets, 256 * 32);
-            const int g_small = std::min(n_sets, 256 * 10);
-            const int g_big = std::min(n_sets, 256 * 2);
-            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, JAM_WAVE>), dim3(g_tiny), dim3(JAM_WAVE), 0, s, dm, dt, st,
-                               scr, 0, JAM_WAVE, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, KD_SMALL>), dim3(g_small), dim3(JAM_WAVE), 0, s, dm, dt,
-                               st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdfast<KD_THREADS, KD_CAP>), dim3(g_big), dim3(KD_THREADS), 0, s, dm, dt,
-                               st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
-            const int g_exact = std::min(n_sets, 256);
-            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, JAM_WAVE>), dim3(g_exact), dim3(JAM_WAVE), 0, s, dm, dt,
-                               st, scr, 0, JAM_WAVE, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(g_exact), dim3(JAM_WAVE), 0, s, dm, dt,
-                               st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(g_exact), dim3(KD_THREADS), 0, s, dm,
-                               dt, st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, JAM_WAVE>), dim3(std::min(n_sets, 256 * 32)),
+                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, 0, JAM_WAVE, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdblock<128>), dim3(std::min(n_sets, 256 * 16)), dim3(128), 0, s, dm, dt,
+                               st, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdblock<256>), dim3(std::min(n_sets, 256 * 8)), dim3(256), 0, s, dm, dt,
+                               st, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdblock<512>), dim3(std::min(n_sets, 256 * 4)), dim3(512), 0, s, dm, dt,
+                               st, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdblock<1024>), dim3(std::min(n_sets, 256 * 2)), dim3(1024), 0, s, dm, dt,
+                               st, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdfast<KD_THREADS, KD_CAP>), dim3(std::min(n_sets, 256)), dim3(KD_THREADS),
+                               0, s, dm, dt, st, scr, KD_CAP, INT32_MAX, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, JAM_WAVE>), dim3(std::min(n_sets, 256 * 16)),
+                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, 0, JAM_WAVE, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(std::min(n_sets, 256 * 8)),
+                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
+            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(std::min(n_sets, 256 * 2)),
+                               dim3(KD_THREADS), 0, s, dm, dt, st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[2], s));

@@ -672,6 +672,165 @@ __device__ __attribute__((always_inline)) static inline bool kd_fast(const KdFas
     return true;
 }
 
+// Fast path of sets of (NT/2, NT] entries (NT = 128 ... 1024), one position
+// per thread: the same algorithm as kd_wave64 with the arrangement of every
+// axis held in registers (position t: key and element), bitonic stages below
+// 64 lanes apart by shuffles and above through LDS, side flags and the
+// stable partition's prefix counts by ballot and one LDS word per wave.
+// Returns false (nothing written) on a split tie.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets st, int *tied,
+                                                unsigned int *n_tied)
+{
+    constexpr int NW = NT / JAM_WAVE;
+    __shared__ double xkey[NT];
+    __shared__ int xid[NT];
+    __shared__ int xpre[NT];
+    __shared__ int side[NT];
+    __shared__ int wsum[NW];
+    const int t = threadIdx.x;
+    const int lane = lane_id();
+    const int wv = t / JAM_WAVE;
+    if (*st.total > st.cap) return;
+    for (int s = blockIdx.x; s < st.n_sets; s += gridDim.x) {
+        const int n = st.cnt[s];
+        if (n <= NT / 2 || n > NT) continue;
+        const int64_t off = st.off[s];
+        const int abase = ms.atom_off[s / tp.n_types];
+        double c[3] = {INFINITY, INFINITY, INFINITY};
+        if (t < n) {
+            const size_t g = (size_t)(abase + st.list[off + t]);
+            c[0] = ms.xyz[3 * g + 0];
+            c[1] = ms.xyz[3 * g + 1];
+            c[2] = ms.xyz[3 * g + 2];
+        }
+        // per axis: (key, element) at position t, sorted by (key, element)
+        double key[3];
+        int id[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double k0 = c[a];
+            int i0 = t;
+            for (int k = 2; k <= NT; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    double ok;
+                    int oi;
+                    if (j >= JAM_WAVE) {
+                        xkey[t] = k0;
+                        xid[t] = i0;
+                        __syncthreads();
+                        ok = xkey[t ^ j];
+                        oi = xid[t ^ j];
+                        __syncthreads();
+                    } else {
+                        ok = __shfl_xor(k0, j, JAM_WAVE);
+                        oi = __shfl_xor(i0, j, JAM_WAVE);
+                    }
+                    const bool up = (t & k) == 0;
+                    const bool lower = (t & j) == 0;
+                    const bool other_less = key_less(ok, oi, k0, i0);
+                    const bool take = (lower == up) ? other_less : !other_less;
+                    if (take) {
+                        k0 = ok;
+                        i0 = oi;
+                    }
+                }
+            }
+            key[a] = k0;
+            id[a] = i0;
+        }
+        int sb = 0, se = n;
+        bool tie = false;
+        for (int d = 0;; d++) {
+            const int a = d % 3;
+            const bool act = t < n && se - sb >= 2;
+            const int m = sb + (se - sb) / 2;
+            if (!__syncthreads_or(act)) break;
+            const double ka = (a == 0) ? key[0] : (a == 1) ? key[1] : key[2];
+            const int ia = (a == 0) ? id[0] : (a == 1) ? id[1] : id[2];
+            xkey[t] = ka;
+            side[ia] = (act && t < m) ? 1 : 0;
+            __syncthreads();
+            const bool my_tie = act && t == m && xkey[t - 1] == ka;
+            if (__syncthreads_or(my_tie)) {
+                tie = true;
+                break;
+            }
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b == a) continue;
+                const int sd = side[id[b]];
+                const bool f = act && sd;
+                const unsigned long long bal = ballot(f);
+                if (lane == 0) wsum[wv] = __popcll(bal);
+                __syncthreads();
+                int pre = rank_below(bal);
+                for (int q = 0; q < wv; q++) pre += wsum[q];
+                xpre[t] = pre;
+                __syncthreads();
+                int np = t;
+                if (act) {
+                    const int lc = pre - xpre[sb];
+                    np = sd ? sb + lc : m + (t - sb - lc);
+                }
+                xkey[np] = key[b];
+                xid[np] = id[b];
+                __syncthreads();
+                key[b] = xkey[t];
+                id[b] = xid[t];
+                __syncthreads();
+            }
+            if (act) {
+                if (t < m) se = m;
+                else sb = m;
+            }
+        }
+        if (tie) {
+            if (t == 0) tied[atomicAdd(n_tied, 1u)] = s;
+            __syncthreads();
+            continue;
+        }
+        // DFS order: position p holds element arrangement[n-1-p]
+        xid[t] = id[0];
+        __syncthreads();
+        const bool ok = t < n;
+        const int e = ok ? xid[n - 1 - t] : 0;
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (ok) {
+            const int atom = st.list[off + e];
+            const size_t g = (size_t)(abase + atom);
+            x = ms.xyz[3 * g + 0];
+            y = ms.xyz[3 * g + 1];
+            z = ms.xyz[3 * g + 2];
+            st.kx[off + t] = x;
+            st.ky[off + t] = y;
+            st.kz[off + t] = z;
+            st.katom[off + t] = atom;
+            st.kres[off + t] = ms.resseq[g];
+            st.kch[off + t] = ms.chain[g];
+        }
+        double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
+        double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
+        for (int sh = 32; sh >= 1; sh >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                blo[q] = fmin(blo[q], __shfl_xor(blo[q], sh, JAM_WAVE));
+                bhi[q] = fmax(bhi[q], __shfl_xor(bhi[q], sh, JAM_WAVE));
+            }
+        }
+        if (lane == 0 && wv * JAM_WAVE < n) {
+            double *bb = st.bbox + 6 * (off / JAM_WAVE + s + wv);
+            bb[0] = blo[0];
+            bb[1] = blo[1];
+            bb[2] = blo[2];
+            bb[3] = bhi[0];
+            bb[4] = bhi[1];
+            bb[5] = bhi[2];
+        }
+        __syncthreads();
+    }
+}
+
 // Loads the coordinates of set s into w, runs the fast path and writes the kd
 // arrays. Always inlined, once with w in LDS and once with w in global
 // scratch, so that the compiler knows the address space of every access: a
