@@ -1,7 +1,7 @@
 // jam_api.hip -- host side of the C ABI declared in include/jess_amd.h.
 //
 // A query runs as one asynchronous chain on the context's stream:
-//   k_candidates<count> -> exclusive scan -> k_candidates<fill> -> k_kdorder
+//   k_candidates<count> -> exclusive scan -> k_candidates<fill> -> k_kd*
 //   -> k_join -> k_super
 // followed by one copy of the counters. Buffers are sized from the previous
 // query; a kernel that would overflow one skips its writes (the counters keep
@@ -96,7 +96,7 @@ struct jam_context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     // per-query workspace
-    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr, kd_tied, kd_ntied;
+    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr, kd_bucket, kd_tied, kd_count;
     DBuf m_item, m_seq, m_atoms, m_count, item_cm, item_pre;
     DBuf p_ord, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
     DBuf f_pair[2], f_pos[2], f_cnt_next, f_cnt_done, f_off_next, f_off_done, f_tot, d_pair, d_pos;
@@ -429,7 +429,8 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
     if (n_sets > c->nsets_cap) {
         HIP_TRY(c->cnt.reserve(sizeof(int) * (size_t)n_sets));
         HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
-        HIP_TRY(c->kd_tied.reserve(sizeof(int) * (size_t)n_sets));
+        HIP_TRY(c->kd_bucket.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
+        HIP_TRY(c->kd_tied.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
         c->nsets_cap = n_sets;
     }
     const int64_t blk = cap / JAM_WAVE + n_sets + 1;
@@ -452,7 +453,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
     }
     HIP_TRY(c->total.reserve(sizeof(int64_t)));
-    HIP_TRY(c->kd_ntied.reserve(sizeof(unsigned int)));
+    HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * 2 * KD_TIERS));
     return JAM_OK;
 }
 
@@ -567,7 +568,10 @@ int32_t grow_events(jam_context *c, int64_t cap, int stride)
     return JAM_OK;
 }
 
-__global__ void k_zero_u32(unsigned int *v) { *v = 0u; }
+__global__ void k_zero_u32(unsigned int *v, int n)
+{
+    if ((int)threadIdx.x < n) v[threadIdx.x] = 0u;
+}
 
 __global__ void k_total(const int *cnt, const int64_t *off, int n, int64_t *total)
 {
@@ -716,33 +720,45 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
         {
-            // fast path for every set: <= 64 entries in one wave's registers,
-            // up to 1024 with one position per thread (128 ... 1024 threads),
-            // larger in global scratch; sets with a tie across a split point
-            // are queued for the exact QuickSelect path. All grid-stride
-            // loops over the sets.
+            // sets bucketed by size; fast path for every set: <= 64 entries
+            // in one wave's registers, up to KD_BIG with one position per
+            // thread, larger in global scratch; sets with a tie across a split
+            // point are queued per tier for the exact QuickSelect path. Grids
+            // are upper bounds (the counts stay on the device); every kernel
+            // loops over its own list.
             unsigned char *scr = c->kscr.as<unsigned char>();
+            int *bucket = c->kd_bucket.as<int>();
             int *tied = c->kd_tied.as<int>();
-            unsigned int *n_tied = c->kd_ntied.as<unsigned int>();
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, n_tied);
-            hipLaunchKernelGGL((jam::k_kdfast<JAM_WAVE, JAM_WAVE>), dim3(std::min(n_sets, 256 * 32)),
-                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, 0, JAM_WAVE, tied, n_tied);
+            unsigned int *nb = c->kd_count.as<unsigned int>();
+            unsigned int *nt = nb + KD_TIERS;
+            const size_t ns = (size_t)n_sets;
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, nb, 2 * KD_TIERS);
+            hipLaunchKernelGGL(jam::k_kdbucket, dim3(std::min((n_sets + 255) / 256, 1024)), dim3(256), 0, s, st,
+                               bucket, nb);
+            hipLaunchKernelGGL(jam::k_kdwave, dim3(std::min(n_sets, 256 * 32)), dim3(JAM_WAVE), 0, s, dm, dt, st,
+                               bucket, nb, tied, nt);
             hipLaunchKernelGGL((jam::k_kdblock<128>), dim3(std::min(n_sets, 256 * 16)), dim3(128), 0, s, dm, dt,
-                               st, tied, n_tied);
+                               st, bucket + ns, nb + 1, tied + ns, nt + 1);
             hipLaunchKernelGGL((jam::k_kdblock<256>), dim3(std::min(n_sets, 256 * 8)), dim3(256), 0, s, dm, dt,
-                               st, tied, n_tied);
+                               st, bucket + 2 * ns, nb + 2, tied + 2 * ns, nt + 2);
             hipLaunchKernelGGL((jam::k_kdblock<512>), dim3(std::min(n_sets, 256 * 4)), dim3(512), 0, s, dm, dt,
-                               st, tied, n_tied);
+                               st, bucket + 3 * ns, nb + 3, tied + 3 * ns, nt + 3);
             hipLaunchKernelGGL((jam::k_kdblock<1024>), dim3(std::min(n_sets, 256 * 2)), dim3(1024), 0, s, dm, dt,
-                               st, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdfast<KD_THREADS, KD_CAP>), dim3(std::min(n_sets, 256)), dim3(KD_THREADS),
-                               0, s, dm, dt, st, scr, KD_CAP, INT32_MAX, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, JAM_WAVE>), dim3(std::min(n_sets, 256 * 16)),
-                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, 0, JAM_WAVE, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdorder<JAM_WAVE, KD_SMALL>), dim3(std::min(n_sets, 256 * 8)),
-                               dim3(JAM_WAVE), 0, s, dm, dt, st, scr, JAM_WAVE, KD_SMALL, tied, n_tied);
-            hipLaunchKernelGGL((jam::k_kdorder<KD_THREADS, KD_CAP>), dim3(std::min(n_sets, 256 * 2)),
-                               dim3(KD_THREADS), 0, s, dm, dt, st, scr, KD_SMALL, INT32_MAX, tied, n_tied);
+                               st, bucket + 4 * ns, nb + 4, tied + 4 * ns, nt + 4);
+            hipLaunchKernelGGL(jam::k_kdbig, dim3(std::min(n_sets, 256)), dim3(KD_THREADS), 0, s, dm, dt, st, scr,
+                               bucket + 5 * ns, nb + 5, tied + 5 * ns, nt + 5);
+            hipLaunchKernelGGL((jam::k_kdexact<64>), dim3(std::min(n_sets, 256 * 16)), dim3(64), 0, s, dm, dt, st,
+                               tied, nt);
+            hipLaunchKernelGGL((jam::k_kdexact<128>), dim3(std::min(n_sets, 256 * 8)), dim3(128), 0, s, dm, dt, st,
+                               tied + ns, nt + 1);
+            hipLaunchKernelGGL((jam::k_kdexact<256>), dim3(std::min(n_sets, 256 * 4)), dim3(256), 0, s, dm, dt, st,
+                               tied + 2 * ns, nt + 2);
+            hipLaunchKernelGGL((jam::k_kdexact<512>), dim3(std::min(n_sets, 256 * 2)), dim3(512), 0, s, dm, dt, st,
+                               tied + 3 * ns, nt + 3);
+            hipLaunchKernelGGL((jam::k_kdexact<1024>), dim3(std::min(n_sets, 256)), dim3(1024), 0, s, dm, dt, st,
+                               tied + 4 * ns, nt + 4);
+            hipLaunchKernelGGL(jam::k_kdbig_exact, dim3(std::min(n_sets, 256)), dim3(KD_THREADS), 0, s, dm, dt, st,
+                               scr, tied + 5 * ns, nt + 5);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[2], s));
