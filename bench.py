@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
 """Throughput of Jess template matching on one MI355X, next to the CPU oracle.
 
-One step = one pass of the hot path over the workload: every template against
-every molecule (candidate sets, Join enumeration, superposition, RMSD
-threshold) through the drop-in API (``list(Jess(templates).query(mol, ...))``,
-or ``Jess.query_batch`` for many molecules), inputs already resident in HBM.
-The metric is template x molecule queries per second.
+One step = one pass of the hot path over a workload: every template against
+every molecule (candidate sets, kd leaf order, Join enumeration,
+superposition, RMSD threshold, hit policy) through the drop-in API
+(``list(Jess(templates).query(mol, ...))`` for one molecule,
+``Jess.query_batch`` for many), molecules and templates already resident in
+HBM. The metric is template x molecule queries per second.
+
+The headline ``value`` is the plain drop-in on one 8,000-atom protein x 500
+templates; the ``batched`` object is the batched form on 4,096 molecules x 200
+templates. Each carries a CPU baseline (the C restatement of the reference
+algorithm on the host's cores, hit-set parity checked) and the roofline of the
+dominant kernel, the join's k_expand launches, timed with HIP events on the
+library's stream in as many steps again after the timed ones.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -60,29 +67,14 @@ def cpu_share() -> int:
     return max(1, n)
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain-8k", help="plain-2k | plain-8k | batch-4k")
-    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch workload's molecules")
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU baseline")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        log(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-            "(queries are independent: the sharding over GPUs is described in DESIGN.md)")
-        sys.exit(2)
-
+def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
     from pyjess_amd import Jess, synthetic, _native
 
     t0 = time.time()
-    mols, tpls, kw = synthetic.workload(args.workload, seed=args.seed, scale=args.scale)
+    mols, tpls, kw = synthetic.workload(name, seed=seed, scale=scale)
     n_atoms = sum(len(m) for m in mols)
     n_pairs = len(mols) * len(tpls)
-    log(f"workload {args.workload}: {len(mols)} molecules ({n_atoms} atoms) x {len(tpls)} templates, "
+    log(f"workload {name}: {len(mols)} molecules ({n_atoms} atoms) x {len(tpls)} templates, "
         f"generated in {time.time() - t0:.1f}s")
     qargs = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
     qkw = dict(max_candidates=kw["max_candidates"])
@@ -93,74 +85,121 @@ def main():
             return [list(jess.query(mols[0], *qargs, **qkw))]
         return jess.query_batch(mols, *qargs, **qkw)
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         step()
-        log(f"warmup step {i + 1}/{args.warmup}")
+        log(f"{name}: warmup step {i + 1}/{warmup}")
     _native.synchronize()
-    kernel_ms = []
+    stage_ms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         hits = step()
-        kernel_ms.append(jess._last_result.timings_ms.copy())
+        stage_ms.append(jess._last_result.timings_ms.copy())
     _native.synchronize()
     elapsed = time.perf_counter() - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n_pairs / (elapsed / args.steps)
-    kernel_ms = np.array(kernel_ms)
-    mean_k = kernel_ms.mean(axis=0)
-    log(f"{args.steps} steps in {elapsed:.3f}s: {ms_per_step:.3f} ms/step, {value:,.0f} queries/s; "
-        f"kernels ms: candidates {mean_k[0]:.3f} kd {mean_k[1]:.3f} join {mean_k[2]:.3f} "
-        f"super {mean_k[3]:.3f} device {mean_k[4]:.3f}")
+    ms_per_step = 1000.0 * elapsed / steps
+    value = n_pairs / (elapsed / steps)
+    st = np.array(stage_ms).mean(axis=0)
+    # the dominant kernel's launches, timed with HIP events on the library's
+    # stream in as many steps again (the events cost stream time, so the
+    # steps above run without them)
+    ctx = _native.context()
+    ctx.set_profiling(True)
+    join_ms, join_launches = [], []
+    try:
+        for _ in range(steps):
+            step()
+            join_ms.append(jess._last_result.join_kernel_ms)
+            join_launches.append(jess._last_result.join_launches)
+    finally:
+        ctx.set_profiling(False)
+    join_ms = float(np.mean(join_ms))
+    launches = float(np.mean(join_launches))
+    log(f"{name}: {steps} steps in {elapsed:.3f}s: {ms_per_step:.3f} ms/step, {value:,.0f} queries/s; "
+        f"device ms: candidates {st[0]:.3f} kd {st[1]:.3f} join {st[2]:.3f} (k_expand {join_ms:.3f} "
+        f"in {launches:.0f} launches) super {st[3]:.3f} total {st[4]:.3f}")
 
     # the native call alone (device pipeline + event copy, no Python hit objects)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         res = jess._device_query(mols, *qargs, 0, True)
-    native_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    native_ms = 1000.0 * (time.perf_counter() - t0) / steps
     n_hits = sum(len(h) for h in hits)
-    events = res
-    log(f"native query: {native_ms:.3f} ms/step; {n_hits} hits, "
+    log(f"{name}: native query {native_ms:.3f} ms/step; {n_hits} hits, "
         f"{int(res.pair_matches.sum())} complete matches")
 
-    # algorithmic work per step (reference algorithm), for the roofline
+    # algorithmic bytes of the whole search (reference algorithm)
     sizes = [len(t) for t in tpls]
     bytes_alg = 40 * n_atoms + sum(32 * n + 8 * n * n for n in sizes) * len(mols)
     hit_sizes = np.array([sizes[int(p) % len(tpls)] for p in res.pair[res.flag == 1]], dtype=np.int64)
     bytes_alg += int((4 * hit_sizes + 128).sum())
 
-    cpu = None
-    flops_alg = None
-    parity = None
-    if not args.no_cpu:
+    out = {
+        "value": value,
+        "unit": "queries/s",
+        "ms_per_step": ms_per_step,
+        "steps": steps,
+        "warmup": warmup,
+        "config": {
+            "workload": name,
+            "molecules": len(mols),
+            "atoms": n_atoms,
+            "templates": len(tpls),
+            "pairs": n_pairs,
+            "seed": seed,
+            "rmsd_threshold": kw["rmsd_threshold"],
+            "distance_cutoff": kw["distance_cutoff"],
+            "max_dynamic_distance": kw["max_dynamic_distance"],
+            "max_candidates": kw["max_candidates"],
+        },
+        "native_ms_per_step": native_ms,
+        "device_ms": {"candidates": st[0], "kd_order": st[1], "join": st[2], "superposition": st[3],
+                      "total": st[4], "k_expand": join_ms, "k_expand_launches": launches},
+        "hits": n_hits,
+        "complete_matches": int(res.pair_matches.sum()),
+        "bytes_alg": bytes_alg,
+    }
+    hbm = {"bound": "hbm", "kernel": "whole device pipeline",
+           "achieved": bytes_alg / (st[4] / 1000.0) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    hbm["frac"] = hbm["achieved"] / hbm["peak"]
+    out["roofline_search_hbm"] = hbm
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    out["hit_set_parity"] = None
+
+    if cpu:
         from oracle import oracle
 
         threads = cpu_share()
         mv = [oracle.MolView.of(m) for m in mols]
         tv = oracle.TplView(tpls)
-        # instrumented run: annulus tests on candidates inside every join box
+        # instrumented run of the reference algorithm: annulus tests on the
+        # candidates inside every join box (sum over levels L of L x in-box)
+        wmols = mols if work_sample is None else mols[:work_sample]
         work = {"inbox_tests": 0, "complete": 0, "complete_atoms": 0}
-        for m in mols:
+        for m in wmols:
             w = oracle.work(m, tv, kw["distance_cutoff"], kw["max_dynamic_distance"])
             for k in work:
                 work[k] += w[k]
-        flops_alg = 8 * work["inbox_tests"] + 27 * work["complete_atoms"] + C_EIG * work["complete"]
+        f = len(mols) / len(wmols)
+        flops_join = 8.0 * work["inbox_tests"] * f
         reps = 0
         t0 = time.perf_counter()
         while True:
             cm, hc, dg = oracle.batch(mv, tv, *qargs, n_threads=threads)
             reps += 1
-            if time.perf_counter() - t0 >= args.cpu_seconds:
+            if time.perf_counter() - t0 >= cpu_seconds:
                 break
         cpu_elapsed = time.perf_counter() - t0
         cpu_rate = reps * n_pairs / cpu_elapsed
-        cpu = {"value": cpu_rate, "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"the whole {args.workload} workload ({n_pairs} pairs) run {reps} times "
-                         f"in {cpu_elapsed:.1f}s on {threads} threads (C restatement, gcc -O3)"}
-        log(f"cpu baseline: {cpu_rate:,.0f} queries/s on {threads} threads ({reps} reps)")
-        # parity of the hit set (before best_match / max_candidates policies)
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"the whole {name} workload ({n_pairs} pairs) run {reps} times in {cpu_elapsed:.1f}s "
+                      f"on {threads} threads (C restatement of the reference algorithm, gcc -O3)"}
+        out["gpu_vs_cpu"] = value / cpu_rate
+        log(f"{name}: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads ({reps} reps)")
+        # parity of the hit set (before the best_match / max_candidates policies)
         T = len(tpls)
-        gpu_dg = np.zeros(n_pairs, dtype=object)
-        gpu_dg[:] = 0
+        gpu_dg = [0] * n_pairs
         gpu_hc = np.zeros(n_pairs, dtype=np.int64)
         for e in np.nonzero(res.flag == 1)[0]:
             p = int(res.pair[e])
@@ -170,62 +209,86 @@ def main():
         parity = bool(np.array_equal(gpu_hc, hc) and
                       all(int(a) == int(b) for a, b in zip(gpu_dg, dg)) and
                       np.array_equal(res.pair_matches, cm))
-        log(f"hit-set parity with the CPU oracle: {parity}")
+        out["hit_set_parity"] = parity
+        log(f"{name}: hit-set parity with the CPU oracle: {parity}")
+        # dominant kernel: the join's k_expand launches; FP64 VALU work (no
+        # MFMA: nothing here is a dense contraction)
+        per_launch_flops = flops_join / max(launches, 1.0)
+        avg_launch_ms = join_ms / max(launches, 1.0)
+        achieved = per_launch_flops / (avg_launch_ms / 1000.0) / 1e12
+        out["roofline"] = {
+            "bound": "mfma", "kernel": "k_expand", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "per_launch_flops": per_launch_flops, "avg_launch_ms": avg_launch_ms,
+            "launches_per_step": launches,
+            "note": "FP64 vector ALU work of the exact annulus tests (8 flops x level x in-box "
+                    "candidate, counted by an instrumented oracle run"
+                    + ("" if work_sample is None else f" on the first {len(wmols)} molecules, scaled")
+                    + ") against the 78.6 TFLOP/s FP64 peak"}
+        prof = os.path.join(ROOT, "profiles", f"k_expand.{name}.pmc.md")
+        if os.path.exists(prof):
+            # HBM bytes per k_expand launch from the committed counter passes
+            # (FETCH_SIZE and WRITE_SIZE, calibrated against a copy kernel)
+            for line in open(prof):
+                if line.startswith("traffic_bytes_per_launch:"):
+                    out["roofline"]["traffic"] = float(line.split(":")[1])
+                    out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+        flops_all = flops_join + (27.0 * work["complete_atoms"] + C_EIG * work["complete"]) * f
+        out["flops_alg"] = flops_all
+    return out
 
-    join_s = mean_k[2] / 1000.0
-    hbm = {"bound": "hbm", "achieved": bytes_alg / join_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-    hbm["frac"] = hbm["achieved"] / hbm["peak"]
-    hbm["traffic"] = None
-    roofline = hbm
-    alt = None
-    if flops_alg is not None:
-        fp = {"bound": "mfma", "achieved": flops_alg / join_s / 1e12, "peak": FP64_PEAK_TFLOPS,
-              "unit": "TFLOP/s", "traffic": None, "note": "fp64 VALU work; the 78.6 TFLOP/s fp64 peak"}
-        fp["frac"] = fp["achieved"] / fp["peak"]
-        if fp["frac"] > hbm["frac"]:
-            roofline, alt = fp, hbm
-        else:
-            alt = fp
-    roofline["kernel"] = "k_join"
-    roofline["per_launch"] = {"bytes_alg": bytes_alg, "flops_alg": flops_alg,
-                              "kernel_ms": mean_k[2]}
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="plain-8k", help="plain-2k | plain-8k | batch-4k")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch workload's molecules")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of each CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="fraction of the 4,096 molecules of the batched measurement (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+            "(queries are independent: the sharding over GPUs is described in DESIGN.md)")
+        sys.exit(2)
+
+    main_run = measure(args.workload, args.scale, args.seed, args.steps, args.warmup, args.cpu_seconds,
+                       not args.no_cpu, None if args.workload != "batch-4k" else 32)
+    batched = None
+    if args.batch_scale > 0 and args.workload != "batch-4k":
+        batched = measure("batch-4k", args.batch_scale, args.seed, args.batch_steps, 1, args.cpu_seconds,
+                          not args.no_cpu, 32)
+        batched["vs_plain"] = batched["value"] / main_run["value"]
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_run["value"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": args.workload,
-            "molecules": len(mols),
-            "atoms": n_atoms,
-            "templates": len(tpls),
-            "pairs": n_pairs,
-            "seed": args.seed,
-            "rmsd_threshold": kw["rmsd_threshold"],
-            "distance_cutoff": kw["distance_cutoff"],
-            "max_dynamic_distance": kw["max_dynamic_distance"],
-            "max_candidates": kw["max_candidates"],
-            "parallelism": "1 GPU",
-        },
-        "roofline": roofline,
-        "roofline_alt": alt,
-        "cpu_baseline": cpu,
-        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "native_ms_per_step": native_ms,
-        "kernel_ms": {"candidates": mean_k[0], "kd_order": mean_k[1], "join": mean_k[2],
-                      "superposition": mean_k[3], "device_total": mean_k[4]},
-        "hits": n_hits,
-        "complete_matches": int(res.pair_matches.sum()),
-        "hit_set_parity": parity,
+        "config": dict(main_run["config"], parallelism="1 GPU"),
+        "roofline": main_run["roofline"],
+        "cpu_baseline": main_run["cpu_baseline"],
+        "gpu_vs_cpu": main_run.get("gpu_vs_cpu"),
+        "hit_set_parity": main_run["hit_set_parity"],
+        "roofline_search_hbm": main_run["roofline_search_hbm"],
+        "native_ms_per_step": main_run["native_ms_per_step"],
+        "device_ms": main_run["device_ms"],
+        "hits": main_run["hits"],
+        "complete_matches": main_run["complete_matches"],
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -27,7 +27,7 @@ _SYMBOLS = (
     "jam_context_create", "jam_context_free", "jam_templates_create", "jam_templates_free",
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
-    "jam_result_timings", "jam_result_free", "jam_superpose_batch", "jam_last_error",
+    "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_free", "jam_superpose_batch", "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -129,6 +129,10 @@ def lib():
         L.jam_result_pair_matches.restype = i32
         L.jam_result_timings.argtypes = [vp, vp]
         L.jam_result_timings.restype = i32
+        L.jam_result_join_kernels.argtypes = [vp, vp, vp]
+        L.jam_result_join_kernels.restype = i32
+        L.jam_context_set_profiling.argtypes = [vp, i32]
+        L.jam_context_set_profiling.restype = i32
         L.jam_result_free.argtypes = [vp]
         L.jam_result_free.restype = None
         L.jam_superpose_batch.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, vp]
@@ -173,6 +177,10 @@ class Context:
         self.handle = h
         self.device = device
         self.lock = threading.Lock()
+
+    def set_profiling(self, on: bool) -> None:
+        """Records HIP events around every join kernel launch (Result.join_kernel_ms)."""
+        _check(lib().jam_context_set_profiling(self.handle, 1 if on else 0))
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -279,7 +287,7 @@ class Result:
     """Events of one device query, as numpy arrays."""
 
     __slots__ = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms", "pair_matches",
-                 "timings_ms", "n_pairs", "stride")
+                 "timings_ms", "join_kernel_ms", "join_launches", "n_pairs", "stride")
 
     def __init__(self, handle):
         L = lib()
@@ -305,6 +313,11 @@ class Result:
         self.pair_matches = self.pair_matches[:self.n_pairs]
         self.timings_ms = np.zeros(5, dtype=np.float64)
         _check(L.jam_result_timings(handle, _ptr(self.timings_ms)))
+        ms = ctypes.c_double(0.0)
+        nl = ctypes.c_int32(0)
+        _check(L.jam_result_join_kernels(handle, ctypes.byref(ms), ctypes.byref(nl)))
+        self.join_kernel_ms = ms.value       # summed k_expand durations (HIP events)
+        self.join_launches = nl.value
 
 
 def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_cutoff: float,

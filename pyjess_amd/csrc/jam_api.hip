@@ -95,6 +95,10 @@ struct jam_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
+    // start/stop events around every k_expand launch (count and fill pass of
+    // each join level), for the join kernels' own duration
+    hipEvent_t xev[4 * JAM_MAX_LEVELS] = {};
+    bool profile = false;       // record xev (each record costs stream time)
     // per-query workspace
     DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr, kd_bucket, kd_tied, kd_count;
     DBuf m_item, m_seq, m_atoms, m_count, item_cm, item_pre;
@@ -171,6 +175,8 @@ struct jam_result {
     std::vector<int64_t> rank, pair_cm;
     std::vector<double> rmsd, rot, c0, c1;
     double ms[5] = {0, 0, 0, 0, 0};
+    double expand_ms = 0.0;     // summed duration of the k_expand launches
+    int expand_launches = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -215,6 +221,7 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     c->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
+    for (auto &ev : c->xev) HIP_TRY(hipEventCreate(&ev));
     *out = c.release();
     return JAM_OK;
 }
@@ -233,6 +240,8 @@ extern "C" void jam_context_free(jam_context *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto &ev : ctx->xev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -765,6 +774,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
 
         // ---- the join, one level at a time
         int64_t n_done = 0;
+        int n_levels = 0;
         if (n0 > 0) {
             if ((rc = grow_front(c, 0, n0, stride)) != JAM_OK) return rc;
             hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
@@ -788,8 +798,10 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 fr.off_next = c->f_off_next.as<long long>();
                 fr.off_done = c->f_off_done.as<long long>();
                 fr.done_base = n_done;
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 0], s));
                 launch_expand(stride, false, n_in, s, dm, dt, st, pr, fr, qp);
                 HIP_TRY(hipGetLastError());
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 1], s));
                 size_t sb = 0;
                 HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, fr.cnt_next, fr.off_next, n_in, s));
                 HIP_TRY(c->scan_tmp2.reserve(sb));
@@ -810,8 +822,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 fr.out_pos = c->f_pos[cur ^ 1].as<int>();
                 fr.done_pair = c->d_pair.as<int>();
                 fr.done_pos = c->d_pos.as<int>();
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 2], s));
                 launch_expand(stride, true, n_in, s, dm, dt, st, pr, fr, qp);
                 HIP_TRY(hipGetLastError());
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 3], s));
+                n_levels = L;
                 n_done += nt[1];
                 n_in = nt[0];
                 cur ^= 1;
@@ -858,6 +873,13 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipEventElapsedTime(&ms_k[3], c->ev[3], c->ev[4]));
         HIP_TRY(hipEventElapsedTime(&ms_k[4], c->ev[0], c->ev[4]));
         for (int k = 0; k < 5; k++) R->ms[k] = ms_k[k];
+        R->expand_ms = c->profile ? 0.0 : -1.0;
+        for (int q = 0; c->profile && q < 2 * n_levels; q++) {
+            float x = 0.f;
+            HIP_TRY(hipEventElapsedTime(&x, c->xev[2 * q], c->xev[2 * q + 1]));
+            R->expand_ms += x;
+        }
+        R->expand_launches = c->profile ? 2 * n_levels : 0;
         const int64_t ne = (int64_t)ne_dev;
         R->pair.resize(ne);
         R->rank.resize(ne);
@@ -930,6 +952,21 @@ extern "C" int32_t jam_result_timings(const jam_result *r, double *ms5)
 {
     if (!r || !ms5) return fail(JAM_EINVAL, "NULL argument");
     std::memcpy(ms5, r->ms, sizeof(r->ms));
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_context_set_profiling(jam_context *c, int32_t on)
+{
+    if (!c) return fail(JAM_EINVAL, "NULL context");
+    c->profile = on != 0;
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_result_join_kernels(const jam_result *r, double *ms, int32_t *launches)
+{
+    if (!r || !ms || !launches) return fail(JAM_EINVAL, "NULL argument");
+    *ms = r->expand_ms;
+    *launches = r->expand_launches;
     return JAM_OK;
 }
 
