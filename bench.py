@@ -156,6 +156,7 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
                       "total": st[4], "k_expand": join_ms, "k_expand_launches": launches},
         "hits": n_hits,
         "complete_matches": int(res.pair_matches.sum()),
+        "join_level_partials": [int(x) for x in res.level_partials[1:] if x > 0],
         "bytes_alg": bytes_alg,
     }
     hbm = {"bound": "hbm", "kernel": "whole device pipeline",
@@ -288,6 +289,7 @@ def main():
         "device_ms": main_run["device_ms"],
         "hits": main_run["hits"],
         "complete_matches": main_run["complete_matches"],
+        "join_level_partials": main_run["join_level_partials"],
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

@@ -144,6 +144,10 @@ int32_t jam_result_timings(const jam_result *r, double *ms5);
  * -- and the number of those launches; -1 ms and 0 launches unless the
  * context records them (jam_context_set_profiling). */
 int32_t jam_result_join_kernels(const jam_result *r, double *ms, int32_t *launches);
+/* Partial matches expanded at each join level L = 1 .. 31 of the last query,
+ * written to counts32[L] (32 entries; counts32[0] is unused): the number of open Scanner_next
+ * states per depth (patches/Scanner.c.patch:232-334) summed over pairs. */
+int32_t jam_result_level_partials(const jam_result *r, int64_t *counts32);
 /* Switches the per-launch events of the join kernels on (1) or off (0, the
  * default): each event record costs a few microseconds of stream time. No
  * counterpart in the reference (measurement only). */

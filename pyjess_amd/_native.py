@@ -27,7 +27,7 @@ _SYMBOLS = (
     "jam_context_create", "jam_context_free", "jam_templates_create", "jam_templates_free",
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
-    "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_free", "jam_superpose_batch", "jam_last_error",
+    "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_result_free", "jam_superpose_batch", "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -131,6 +131,8 @@ def lib():
         L.jam_result_timings.restype = i32
         L.jam_result_join_kernels.argtypes = [vp, vp, vp]
         L.jam_result_join_kernels.restype = i32
+        L.jam_result_level_partials.argtypes = [vp, vp]
+        L.jam_result_level_partials.restype = i32
         L.jam_context_set_profiling.argtypes = [vp, i32]
         L.jam_context_set_profiling.restype = i32
         L.jam_result_free.argtypes = [vp]
@@ -287,7 +289,7 @@ class Result:
     """Events of one device query, as numpy arrays."""
 
     __slots__ = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms", "pair_matches",
-                 "timings_ms", "join_kernel_ms", "join_launches", "n_pairs", "stride")
+                 "timings_ms", "join_kernel_ms", "join_launches", "level_partials", "n_pairs", "stride")
 
     def __init__(self, handle):
         L = lib()
@@ -318,6 +320,8 @@ class Result:
         _check(L.jam_result_join_kernels(handle, ctypes.byref(ms), ctypes.byref(nl)))
         self.join_kernel_ms = ms.value       # summed k_expand durations (HIP events)
         self.join_launches = nl.value
+        self.level_partials = np.zeros(32, dtype=np.int64)   # partials expanded per join level
+        _check(L.jam_result_level_partials(handle, _ptr(self.level_partials)))
 
 
 def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_cutoff: float,

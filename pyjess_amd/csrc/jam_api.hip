@@ -176,6 +176,7 @@ struct jam_result {
     std::vector<double> rmsd, rot, c0, c1;
     double ms[5] = {0, 0, 0, 0, 0};
     double expand_ms = 0.0;     // summed duration of the k_expand launches
+    int64_t level_partials[JAM_MAX_LEVELS] = {};   // partial matches expanded per join level
     int expand_launches = 0;
 };
 
@@ -775,6 +776,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         // ---- the join, one level at a time
         int64_t n_done = 0;
         int n_levels = 0;
+        int64_t lp[JAM_MAX_LEVELS] = {};
         if (n0 > 0) {
             if ((rc = grow_front(c, 0, n0, stride)) != JAM_OK) return rc;
             hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
@@ -788,6 +790,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             int64_t n_in = n0;
             for (int L = 1; n_in > 0 && L <= std::max(1, stride - 1); L++) {
                 if ((rc = grow_fcnt(c, n_in)) != JAM_OK) return rc;
+                lp[L] = n_in;
                 fr.level = L;
                 fr.stride = c->front_stride;
                 fr.n_in = n_in;
@@ -880,6 +883,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             R->expand_ms += x;
         }
         R->expand_launches = c->profile ? 2 * n_levels : 0;
+        for (int q = 0; q < JAM_MAX_LEVELS; q++) R->level_partials[q] = lp[q];
         const int64_t ne = (int64_t)ne_dev;
         R->pair.resize(ne);
         R->rank.resize(ne);
@@ -952,6 +956,13 @@ extern "C" int32_t jam_result_timings(const jam_result *r, double *ms5)
 {
     if (!r || !ms5) return fail(JAM_EINVAL, "NULL argument");
     std::memcpy(ms5, r->ms, sizeof(r->ms));
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_result_level_partials(const jam_result *r, int64_t *counts)
+{
+    if (!r || !counts) return fail(JAM_EINVAL, "NULL argument");
+    std::memcpy(counts, r->level_partials, sizeof(r->level_partials));
     return JAM_OK;
 }
 

@@ -301,7 +301,7 @@ __device__ static void expand(const DevMols &ms, const DevTpls &tp, const DevSet
         }
     }
     const int tchL = tp.chain[tbase + kL], tresL = tp.resseq[tbase + kL];
-    __syncthreads();
+    wave_sync();
     const int ignore = qp.ignore_chain;
     const int nblk = (cntL + JAM_WAVE - 1) / JAM_WAVE;
     const long long bbase = offL / JAM_WAVE + setL;

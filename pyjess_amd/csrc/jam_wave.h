@@ -20,4 +20,14 @@ __device__ static inline int rank_below(unsigned long long m)
 
 __device__ static inline int first_bit(unsigned long long m) { return __ffsll((long long)m) - 1; }
 
+// Orders the LDS accesses of one wave around this point (other lanes' writes
+// before it are visible to reads after it); waves of 64 lanes run in lockstep,
+// so no s_barrier is needed.
+__device__ static inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace jam
