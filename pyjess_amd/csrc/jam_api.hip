@@ -95,23 +95,29 @@ struct jam_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
-    // start/stop events around every k_expand launch (count and fill pass of
-    // each join level), for the join kernels' own duration
-    hipEvent_t xev[4 * JAM_MAX_LEVELS] = {};
+    // start/stop events around every k_join launch (one per join level), for
+    // the join kernels' own duration
+    hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
     bool profile = false;       // record xev (each record costs stream time)
     // per-query workspace
-    DBuf cnt, off, list, kidx, kx, ky, kz, katom, kres, kch, bbox, total, kscr, kd_bucket, kd_tied, kd_count;
-    DBuf m_item, m_seq, m_atoms, m_count, item_cm, item_pre;
-    DBuf p_ord, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
-    DBuf f_pair[2], f_pos[2], f_cnt_next, f_cnt_done, f_off_next, f_off_done, f_tot, d_pair, d_pos;
-    int64_t front_cap[2] = {0, 0}, fcnt_cap = 0, done_cap = 0;
-    int front_stride = 0;
+    DBuf cnt, off, list, kidx, kent, katom, bbox, total, maxcnt, kscr, kd_bucket, kd_tied, kd_count;
+    DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
+    // join: two frontiers (pair + stride entries per row), complete matches,
+    // counters (rows per level, complete matches, overflow flag)
+    DBuf j_pair[2], j_ent[2], d_pair, d_ent, j_count;
+    int64_t front_cap = 0, done_cap = 0;
+    int front_stride = 0, done_stride = 0;
+    int join_grid[3] = {0, 0, 0};   // resident workgroups of k_join<8/16/32>
+    int kd_grid[12] = {};           // resident workgroups of the kd kernels
+    // sort of the complete matches
+    DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
+    int64_t sort_cap = 0;
     DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms, e_count;
     DBuf scan_tmp, scan_tmp2;
-    int64_t set_cap = 0, blk_cap = 0, match_cap = 0, event_cap = 0;
+    int64_t set_cap = 0, blk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
     int64_t items_cap = 0;
-    int match_stride = 0, event_stride = 0;
+    int event_stride = 0;
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
 };
@@ -448,12 +454,8 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         cap = std::max<int64_t>(cap, 1024);
         HIP_TRY(c->list.reserve(sizeof(int) * cap));
         HIP_TRY(c->kidx.reserve(sizeof(int) * cap));
-        HIP_TRY(c->kx.reserve(sizeof(double) * cap));
-        HIP_TRY(c->ky.reserve(sizeof(double) * cap));
-        HIP_TRY(c->kz.reserve(sizeof(double) * cap));
+        HIP_TRY(c->kent.reserve(sizeof(SetEnt) * cap));
         HIP_TRY(c->katom.reserve(sizeof(int) * cap));
-        HIP_TRY(c->kres.reserve(sizeof(int) * cap));
-        HIP_TRY(c->kch.reserve(sizeof(int) * cap));
         HIP_TRY(c->kscr.reserve((size_t)KD_SCRATCH_BYTES_PER_ENTRY * cap));
         c->set_cap = cap;
     }
@@ -463,22 +465,8 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
     }
     HIP_TRY(c->total.reserve(sizeof(int64_t)));
+    HIP_TRY(c->maxcnt.reserve(sizeof(int)));
     HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * 2 * KD_TIERS));
-    return JAM_OK;
-}
-
-int32_t grow_matches(jam_context *c, int64_t cap, int stride)
-{
-    if (cap > c->match_cap || stride > c->match_stride) {
-        cap = std::max<int64_t>(cap, std::max<int64_t>(c->match_cap, 4096));
-        stride = std::max(stride, c->match_stride);
-        HIP_TRY(c->m_item.reserve(sizeof(int) * cap));
-        HIP_TRY(c->m_seq.reserve(sizeof(int) * cap));
-        HIP_TRY(c->m_atoms.reserve(sizeof(int) * cap * stride));
-        c->match_cap = cap;
-        c->match_stride = stride;
-    }
-    HIP_TRY(c->m_count.reserve(sizeof(unsigned long long)));
     return JAM_OK;
 }
 
@@ -486,6 +474,7 @@ int32_t grow_pairs(jam_context *c, int n_pairs)
 {
     if (n_pairs <= c->npairs_cap) return JAM_OK;
     HIP_TRY(c->p_ord.reserve((size_t)JAM_MAX_LEVELS * n_pairs));
+    HIP_TRY(c->p_lvl.reserve(sizeof(PairLevel) * (size_t)JAM_MAX_LEVELS * n_pairs));
     HIP_TRY(c->p_nchunk.reserve(sizeof(int) * (size_t)n_pairs));
     HIP_TRY(c->p_base.reserve(sizeof(long long) * (size_t)n_pairs));
     HIP_TRY(c->p_cm.reserve(sizeof(long long) * (size_t)n_pairs));
@@ -505,56 +494,51 @@ int32_t grow_items(jam_context *c, int64_t n_items)
     return JAM_OK;
 }
 
-// partial-match rows of buffer b (pair + stride positions each)
-int32_t grow_front(jam_context *c, int b, int64_t rows, int stride)
+// join frontiers (both buffers), rows of `stride` entries
+int32_t grow_front(jam_context *c, int64_t rows, int stride)
 {
-    if (rows <= c->front_cap[b] && stride <= c->front_stride) return JAM_OK;
-    rows = std::max<int64_t>(std::max<int64_t>(rows, c->front_cap[b]), 4096);
+    if (rows <= c->front_cap && stride <= c->front_stride) return JAM_OK;
+    rows = std::max<int64_t>(std::max<int64_t>(rows, c->front_cap), 1 << 16);
     stride = std::max(stride, c->front_stride);
-    HIP_TRY(c->f_pair[b].reserve(sizeof(int) * (size_t)rows));
-    HIP_TRY(c->f_pos[b].reserve(sizeof(int) * (size_t)rows * stride));
-    c->front_cap[b] = rows;
-    if (stride > c->front_stride) {
-        // the other buffer must follow the wider rows too
-        c->front_stride = stride;
-        c->front_cap[b ^ 1] = 0;
+    for (int b = 0; b < 2; b++) {
+        HIP_TRY(c->j_pair[b].reserve(sizeof(int) * (size_t)rows));
+        HIP_TRY(c->j_ent[b].reserve(sizeof(int) * (size_t)rows * stride));
     }
+    c->front_cap = rows;
+    c->front_stride = stride;
     return JAM_OK;
 }
 
-int32_t grow_fcnt(jam_context *c, int64_t rows)
+int32_t grow_done(jam_context *c, int64_t rows, int stride)
 {
-    if (rows <= c->fcnt_cap) return JAM_OK;
-    rows = std::max<int64_t>(rows, 4096);
-    HIP_TRY(c->f_cnt_next.reserve(sizeof(int) * (size_t)rows));
-    HIP_TRY(c->f_cnt_done.reserve(sizeof(int) * (size_t)rows));
-    HIP_TRY(c->f_off_next.reserve(sizeof(long long) * (size_t)rows));
-    HIP_TRY(c->f_off_done.reserve(sizeof(long long) * (size_t)rows));
-    HIP_TRY(c->f_tot.reserve(sizeof(long long) * 2));
-    c->fcnt_cap = rows;
-    return JAM_OK;
-}
-
-int32_t grow_done(jam_context *c, int64_t rows, int stride, int64_t keep)
-{
-    // keep: rows already written that must survive the growth
-    if (rows <= c->done_cap && (size_t)stride * sizeof(int) * std::max<int64_t>(rows, 1) <= c->d_pos.bytes)
-        return JAM_OK;
-    rows = std::max<int64_t>(std::max<int64_t>(rows, 2 * c->done_cap), 4096);
-    DBuf np, nq;
-    HIP_TRY(np.reserve(sizeof(int) * (size_t)rows));
-    HIP_TRY(nq.reserve(sizeof(int) * (size_t)rows * stride));
-    if (keep > 0) {
-        HIP_TRY(hipMemcpyAsync(np.ptr, c->d_pair.ptr, sizeof(int) * keep, hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(nq.ptr, c->d_pos.ptr, sizeof(int) * keep * stride, hipMemcpyDeviceToDevice,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
-    std::swap(c->d_pair.ptr, np.ptr);
-    std::swap(c->d_pair.bytes, np.bytes);
-    std::swap(c->d_pos.ptr, nq.ptr);
-    std::swap(c->d_pos.bytes, nq.bytes);
+    if (rows <= c->done_cap && stride <= c->done_stride) return JAM_OK;
+    rows = std::max<int64_t>(std::max<int64_t>(rows, c->done_cap), 1 << 14);
+    stride = std::max(stride, c->done_stride);
+    HIP_TRY(c->d_pair.reserve(sizeof(int) * (size_t)rows));
+    HIP_TRY(c->d_ent.reserve(sizeof(int) * (size_t)rows * stride));
     c->done_cap = rows;
+    c->done_stride = stride;
+    return JAM_OK;
+}
+
+// keys, permutations and the radix sort's temporary storage for n matches
+int32_t grow_sort(jam_context *c, int64_t n)
+{
+    if (n <= c->sort_cap) return JAM_OK;
+    n = std::max<int64_t>(n, 1 << 14);
+    for (int b = 0; b < 2; b++) {
+        HIP_TRY(c->s_key[b].reserve(sizeof(unsigned long long) * (size_t)n));
+        HIP_TRY(c->s_perm[b].reserve(sizeof(int) * (size_t)n));
+        HIP_TRY(c->s_pkey[b].reserve(sizeof(unsigned int) * (size_t)n));
+    }
+    size_t t1 = 0, t2 = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (const unsigned long long *)nullptr,
+                                               (unsigned long long *)nullptr, (const int *)nullptr, (int *)nullptr,
+                                               (int)n, 0, 64, c->stream));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, (const unsigned int *)nullptr, (unsigned int *)nullptr,
+                                               (const int *)nullptr, (int *)nullptr, (int)n, 0, 32, c->stream));
+    HIP_TRY(c->s_tmp.reserve(std::max(t1, t2)));
+    c->sort_cap = n;
     return JAM_OK;
 }
 
@@ -593,15 +577,6 @@ __global__ void k_total_items(const int *cnt, const long long *off, long long n,
     if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
 }
 
-// complete matches per pair: the matches of a pair are contiguous
-__global__ void k_pair_matches(const int *done_pair, long long n, const long long *first, long long *out)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (i == n - 1 || done_pair[i + 1] != done_pair[i]) out[done_pair[i]] = i - first[done_pair[i]] + 1;
-}
-
-
 __global__ void k_items_any(DevPairs pr)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -611,20 +586,54 @@ __global__ void k_items_any(DevPairs pr)
     for (int i = 0; i < c; i++) pr.item_pair[b + i] = p;
 }
 
-void launch_expand(int stride, bool fill, int64_t n_in, hipStream_t s, const DevMols &dm, const DevTpls &dt,
-                   const DevSets &st, const DevPairs &pr, const DevFront &fr, const QueryParams &qp)
+__global__ void k_zero_u64(unsigned long long *v, long long n)
 {
-    const dim3 g((unsigned)n_in), b(64);
-    if (stride <= 8) {
-        if (fill) hipLaunchKernelGGL((jam::k_expand<8, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
-        else hipLaunchKernelGGL((jam::k_expand<8, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
-    } else if (stride <= 16) {
-        if (fill) hipLaunchKernelGGL((jam::k_expand<16, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
-        else hipLaunchKernelGGL((jam::k_expand<16, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
-    } else {
-        if (fill) hipLaunchKernelGGL((jam::k_expand<32, true>), g, b, 0, s, dm, dt, st, pr, fr, qp);
-        else hipLaunchKernelGGL((jam::k_expand<32, false>), g, b, 0, s, dm, dt, st, pr, fr, qp);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        v[i] = 0ull;
+}
+
+// workgroups of `kernel` (with `threads` threads) that stay resident on the device
+template <class K>
+int resident_grid(K kernel, int threads, int device)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 1024;
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+template <int MAXN>
+int join_grid(int device)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jam::k_join<MAXN>, JOIN_WAVES * JAM_WAVE, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return std::max(1, per_cu) * std::max(1, cus);
+}
+
+// resident workgroups of the k_join variant for rows of `stride` entries
+int join_workgroups(jam_context *c, int stride)
+{
+    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : 2;
+    if (c->join_grid[v] == 0) {
+        c->join_grid[v] = v == 0 ? join_grid<8>(c->device) : v == 1 ? join_grid<16>(c->device) : join_grid<32>(c->device);
+        if (c->join_grid[v] == 0) c->join_grid[v] = 1024;
     }
+    return c->join_grid[v];
+}
+
+// one join level; the grid is the number of workgroups that stay resident
+void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, const DevTpls &dt, const DevSets &st,
+                 const DevPairs &pr, const DevJoin &jn, const QueryParams &qp)
+{
+    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : 2;
+    const dim3 g((unsigned)join_workgroups(c, stride)), b(JOIN_WAVES * JAM_WAVE);
+    if (v == 0) hipLaunchKernelGGL((jam::k_join<8>), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    else if (v == 1) hipLaunchKernelGGL((jam::k_join<16>), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    else hipLaunchKernelGGL((jam::k_join<32>), g, b, 0, s, dm, dt, st, pr, jn, qp);
 }
 
 }  // namespace
@@ -667,15 +676,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     if ((rc = grow_pairs(c, n_pairs)) != JAM_OK) return rc;
     DevMols dm = ms->dev();
     DevTpls dt = tp->dev();
-    size_t scan_sets = 0, scan_pairs = 0;
+    size_t scan_sets = 0, scan_pairs = 0, scan_max = 0;
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_sets, (const int *)nullptr, (int64_t *)nullptr,
                                              n_sets, s));
+    HIP_TRY(hipcub::DeviceReduce::Max(nullptr, scan_max, (const int *)nullptr, (int *)nullptr, n_sets, s));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_pairs, (const int *)nullptr, (long long *)nullptr,
                                              n_pairs, s));
-    HIP_TRY(c->scan_tmp.reserve(std::max(scan_sets, scan_pairs)));
+    HIP_TRY(c->scan_tmp.reserve(std::max(std::max(scan_sets, scan_pairs), scan_max)));
     const int pair_blocks = (n_pairs + 255) / 256;
 
-    for (int attempt = 0; attempt < 4; attempt++) {
+    for (int attempt = 0; attempt < 8; attempt++) {
         DevSets st;
         st.n_sets = n_sets;
         st.cap = c->set_cap;
@@ -684,18 +694,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         st.off = c->off.as<int64_t>();
         st.list = c->list.as<int>();
         st.kidx = c->kidx.as<int>();
-        st.kx = c->kx.as<double>();
-        st.ky = c->ky.as<double>();
-        st.kz = c->kz.as<double>();
+        st.kent = c->kent.as<SetEnt>();
         st.katom = c->katom.as<int>();
-        st.kres = c->kres.as<int>();
-        st.kch = c->kch.as<int>();
         st.bbox = c->bbox.as<double>();
         st.total = c->total.as<int64_t>();
+        st.maxcnt = c->maxcnt.as<int>();
         DevPairs pr;
         pr.n_pairs = n_pairs;
         pr.n_items = 0;
         pr.ord = c->p_ord.as<unsigned char>();
+        pr.lvl = c->p_lvl.as<PairLevel>();
         pr.nchunk = c->p_nchunk.as<int>();
         pr.n1 = c->p_n1.as<int>();
         pr.item_base = c->p_base.as<long long>();
@@ -707,6 +715,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_sets, st.cnt, st.off, n_sets, s));
+        HIP_TRY(hipcub::DeviceReduce::Max(c->scan_tmp.ptr, scan_max, st.cnt, st.maxcnt, n_sets, s));
         hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
         hipLaunchKernelGGL(jam::k_pairs, dim3(pair_blocks), dim3(256), 0, s, dt, st, pr, qp);
         HIP_TRY(hipGetLastError());
@@ -715,9 +724,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, pr.nchunk, pr.item_base, (long long)n_pairs,
                            c->n_items.as<long long>());
         int64_t totals[2] = {0, 0};
+        int maxcnt = 0;
         HIP_TRY(hipMemcpyAsync(&totals[0], st.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipMemcpyAsync(&totals[1], c->n_items.ptr, sizeof(long long), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&maxcnt, st.maxcnt, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        if (totals[0] >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 candidate-set entries in one query");
         if (totals[0] > c->set_cap) {
             if ((rc = grow_sets(c, totals[0], n_sets)) != JAM_OK) return rc;
             continue;
@@ -745,101 +757,147 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, nb, 2 * KD_TIERS);
             hipLaunchKernelGGL(jam::k_kdbucket, dim3(std::min((n_sets + 255) / 256, 1024)), dim3(256), 0, s, st,
                                bucket, nb);
-            hipLaunchKernelGGL(jam::k_kdwave, dim3(std::min(n_sets, 256 * 32)), dim3(JAM_WAVE), 0, s, dm, dt, st,
-                               bucket, nb, tied, nt);
-            hipLaunchKernelGGL((jam::k_kdblock<128>), dim3(std::min(n_sets, 256 * 16)), dim3(128), 0, s, dm, dt,
-                               st, bucket + ns, nb + 1, tied + ns, nt + 1);
-            hipLaunchKernelGGL((jam::k_kdblock<256>), dim3(std::min(n_sets, 256 * 8)), dim3(256), 0, s, dm, dt,
-                               st, bucket + 2 * ns, nb + 2, tied + 2 * ns, nt + 2);
-            hipLaunchKernelGGL((jam::k_kdblock<512>), dim3(std::min(n_sets, 256 * 4)), dim3(512), 0, s, dm, dt,
-                               st, bucket + 3 * ns, nb + 3, tied + 3 * ns, nt + 3);
-            hipLaunchKernelGGL((jam::k_kdblock<1024>), dim3(std::min(n_sets, 256 * 2)), dim3(1024), 0, s, dm, dt,
-                               st, bucket + 4 * ns, nb + 4, tied + 4 * ns, nt + 4);
-            hipLaunchKernelGGL(jam::k_kdbig, dim3(std::min(n_sets, 256)), dim3(KD_THREADS), 0, s, dm, dt, st, scr,
-                               bucket + 5 * ns, nb + 5, tied + 5 * ns, nt + 5);
-            hipLaunchKernelGGL((jam::k_kdexact<64>), dim3(std::min(n_sets, 256 * 16)), dim3(64), 0, s, dm, dt, st,
-                               tied, nt);
-            hipLaunchKernelGGL((jam::k_kdexact<128>), dim3(std::min(n_sets, 256 * 8)), dim3(128), 0, s, dm, dt, st,
+            if (c->kd_grid[0] == 0) {
+                const int d = c->device;
+                c->kd_grid[0] = resident_grid(jam::k_kdwave, JAM_WAVE, d);
+                c->kd_grid[1] = resident_grid(jam::k_kdblock<128>, 128, d);
+                c->kd_grid[2] = resident_grid(jam::k_kdblock<256>, 256, d);
+                c->kd_grid[3] = resident_grid(jam::k_kdblock<512>, 512, d);
+                c->kd_grid[4] = resident_grid(jam::k_kdblock<1024>, 1024, d);
+                c->kd_grid[5] = resident_grid(jam::k_kdbig, KD_THREADS, d);
+                c->kd_grid[6] = resident_grid(jam::k_kdexact<64>, 64, d);
+                c->kd_grid[7] = resident_grid(jam::k_kdexact<128>, 128, d);
+                c->kd_grid[8] = resident_grid(jam::k_kdexact<256>, 256, d);
+                c->kd_grid[9] = resident_grid(jam::k_kdexact<512>, 512, d);
+                c->kd_grid[10] = resident_grid(jam::k_kdexact<1024>, 1024, d);
+                c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_THREADS, d);
+            }
+            auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
+            hipLaunchKernelGGL(jam::k_kdwave, grid(0), dim3(JAM_WAVE), 0, s, dm, dt, st, bucket, nb, tied, nt);
+            hipLaunchKernelGGL((jam::k_kdblock<128>), grid(1), dim3(128), 0, s, dm, dt, st, bucket + ns, nb + 1,
                                tied + ns, nt + 1);
-            hipLaunchKernelGGL((jam::k_kdexact<256>), dim3(std::min(n_sets, 256 * 4)), dim3(256), 0, s, dm, dt, st,
+            hipLaunchKernelGGL((jam::k_kdblock<256>), grid(2), dim3(256), 0, s, dm, dt, st, bucket + 2 * ns, nb + 2,
                                tied + 2 * ns, nt + 2);
-            hipLaunchKernelGGL((jam::k_kdexact<512>), dim3(std::min(n_sets, 256 * 2)), dim3(512), 0, s, dm, dt, st,
+            hipLaunchKernelGGL((jam::k_kdblock<512>), grid(3), dim3(512), 0, s, dm, dt, st, bucket + 3 * ns, nb + 3,
                                tied + 3 * ns, nt + 3);
-            hipLaunchKernelGGL((jam::k_kdexact<1024>), dim3(std::min(n_sets, 256)), dim3(1024), 0, s, dm, dt, st,
-                               tied + 4 * ns, nt + 4);
-            hipLaunchKernelGGL(jam::k_kdbig_exact, dim3(std::min(n_sets, 256)), dim3(KD_THREADS), 0, s, dm, dt, st,
-                               scr, tied + 5 * ns, nt + 5);
+            hipLaunchKernelGGL((jam::k_kdblock<1024>), grid(4), dim3(1024), 0, s, dm, dt, st, bucket + 4 * ns,
+                               nb + 4, tied + 4 * ns, nt + 4);
+            hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, s, dm, dt, st, scr, bucket + 5 * ns,
+                               nb + 5, tied + 5 * ns, nt + 5);
+            hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, s, dm, dt, st, tied, nt);
+            hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, s, dm, dt, st, tied + ns, nt + 1);
+            hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, s, dm, dt, st, tied + 2 * ns, nt + 2);
+            hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, s, dm, dt, st, tied + 3 * ns, nt + 3);
+            hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, s, dm, dt, st, tied + 4 * ns,
+                               nt + 4);
+            hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_THREADS), 0, s, dm, dt, st, scr, tied + 5 * ns,
+                               nt + 5);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[2], s));
 
-        // ---- the join, one level at a time
-        int64_t n_done = 0;
-        int n_levels = 0;
-        int64_t lp[JAM_MAX_LEVELS] = {};
+        // ---- the join: one launch per level, back to back; the counters
+        // stay on the device until the end
+        const int levels = std::max(1, stride - 1);
+        if ((rc = grow_front(c, std::max<int64_t>(c->front_cap, n0), stride)) != JAM_OK) return rc;
+        if ((rc = grow_done(c, std::max<int64_t>(c->done_cap, 1 << 14), stride)) != JAM_OK) return rc;
+        HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * (JAM_MAX_LEVELS + 2)));
+        unsigned long long *fcount = c->j_count.as<unsigned long long>();   // rows per level
+        unsigned long long *n_done_dev = fcount + JAM_MAX_LEVELS;
+        unsigned int *overflow = (unsigned int *)(fcount + JAM_MAX_LEVELS + 1);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, fcount, (long long)(JAM_MAX_LEVELS + 2));
+        DevJoin jn{};
+        jn.stride = c->front_stride;
+        jn.n_done = n_done_dev;
+        jn.done_cap = c->done_cap;
+        jn.done_pair = c->d_pair.as<int>();
+        jn.done_ent = c->d_ent.as<int>();
+        jn.overflow = overflow;
+        if (c->done_stride != c->front_stride) {
+            if ((rc = grow_done(c, c->done_cap, c->front_stride)) != JAM_OK) return rc;
+            jn.done_pair = c->d_pair.as<int>();
+            jn.done_ent = c->d_ent.as<int>();
+        }
         if (n0 > 0) {
-            if ((rc = grow_front(c, 0, n0, stride)) != JAM_OK) return rc;
             hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
-            DevFront fr{};
-            fr.stride = c->front_stride;
-            fr.out_pair = c->f_pair[0].as<int>();
-            fr.out_pos = c->f_pos[0].as<int>();
-            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, pr, fr);
+            jn.level = 0;
+            jn.n_out = fcount;
+            jn.out_cap = c->front_cap;
+            jn.out_pair = c->j_pair[0].as<int>();
+            jn.out_ent = c->j_ent[0].as<int>();
+            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, pr, jn);
             HIP_TRY(hipGetLastError());
-            int cur = 0;
-            int64_t n_in = n0;
-            for (int L = 1; n_in > 0 && L <= std::max(1, stride - 1); L++) {
-                if ((rc = grow_fcnt(c, n_in)) != JAM_OK) return rc;
-                lp[L] = n_in;
-                fr.level = L;
-                fr.stride = c->front_stride;
-                fr.n_in = n_in;
-                fr.in_pair = c->f_pair[cur].as<int>();
-                fr.in_pos = c->f_pos[cur].as<int>();
-                fr.cnt_next = c->f_cnt_next.as<int>();
-                fr.cnt_done = c->f_cnt_done.as<int>();
-                fr.off_next = c->f_off_next.as<long long>();
-                fr.off_done = c->f_off_done.as<long long>();
-                fr.done_base = n_done;
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 0], s));
-                launch_expand(stride, false, n_in, s, dm, dt, st, pr, fr, qp);
+            for (int L = 1; L <= levels; L++) {
+                const int bi = (L - 1) & 1, bo = L & 1;
+                jn.level = L;
+                jn.n_in = fcount + (L - 1);
+                jn.in_cap = c->front_cap;
+                jn.in_pair = c->j_pair[bi].as<int>();
+                jn.in_ent = c->j_ent[bi].as<int>();
+                jn.n_out = fcount + L;
+                jn.out_cap = c->front_cap;
+                jn.out_pair = c->j_pair[bo].as<int>();
+                jn.out_ent = c->j_ent[bo].as<int>();
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[2 * (L - 1) + 0], s));
+                launch_join(c, stride, s, dm, dt, st, pr, jn, qp);
                 HIP_TRY(hipGetLastError());
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 1], s));
-                size_t sb = 0;
-                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, fr.cnt_next, fr.off_next, n_in, s));
-                HIP_TRY(c->scan_tmp2.reserve(sb));
-                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp2.ptr, sb, fr.cnt_next, fr.off_next, n_in, s));
-                HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp2.ptr, sb, fr.cnt_done, fr.off_done, n_in, s));
-                long long *tot = c->f_tot.as<long long>();
-                hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, fr.cnt_next, fr.off_next, n_in, tot);
-                hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, fr.cnt_done, fr.off_done, n_in, tot + 1);
-                long long nt[2];
-                HIP_TRY(hipMemcpyAsync(nt, tot, sizeof(long long) * 2, hipMemcpyDeviceToHost, s));
-                HIP_TRY(hipStreamSynchronize(s));
-                if ((rc = grow_front(c, cur ^ 1, nt[0], stride)) != JAM_OK) return rc;
-                if ((rc = grow_done(c, n_done + nt[1], c->front_stride, n_done)) != JAM_OK) return rc;
-                fr.stride = c->front_stride;
-                fr.in_pos = c->f_pos[cur].as<int>();
-                fr.in_pair = c->f_pair[cur].as<int>();
-                fr.out_pair = c->f_pair[cur ^ 1].as<int>();
-                fr.out_pos = c->f_pos[cur ^ 1].as<int>();
-                fr.done_pair = c->d_pair.as<int>();
-                fr.done_pos = c->d_pos.as<int>();
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 2], s));
-                launch_expand(stride, true, n_in, s, dm, dt, st, pr, fr, qp);
-                HIP_TRY(hipGetLastError());
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[4 * (L - 1) + 3], s));
-                n_levels = L;
-                n_done += nt[1];
-                n_in = nt[0];
-                cur ^= 1;
+                if (c->profile) HIP_TRY(hipEventRecord(c->xev[2 * (L - 1) + 1], s));
             }
         }
+        unsigned long long jc[JAM_MAX_LEVELS + 2];
+        HIP_TRY(hipMemcpyAsync(jc, fcount, sizeof(jc), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const unsigned int ovf = (unsigned int)(jc[JAM_MAX_LEVELS + 1] & 0xFFFFFFFFull);
+        const int64_t n_done = (int64_t)jc[JAM_MAX_LEVELS];
+        if (ovf || n_done > c->done_cap) {
+            // rows that did not fit: grow to what was seen (deeper levels saw
+            // truncated inputs, so this may take another round)
+            int64_t need = 0;
+            for (int L = 0; L < JAM_MAX_LEVELS; L++) need = std::max<int64_t>(need, (int64_t)jc[L]);
+            if ((rc = grow_front(c, std::max<int64_t>(2 * c->front_cap, need), stride)) != JAM_OK) return rc;
+            if ((rc = grow_done(c, std::max<int64_t>(c->done_cap, 2 * n_done), c->front_stride)) != JAM_OK)
+                return rc;
+            continue;
+        }
         HIP_TRY(hipEventRecord(c->ev[3], s));
-        DevFront fd{};
-        fd.stride = c->front_stride;
-        fd.done_pair = c->d_pair.as<int>();
-        fd.done_pos = c->d_pos.as<int>();
+
+        // ---- reference order of the complete matches: LSD radix passes over
+        // the level positions (`bits` bits each, `per` levels per 64-bit key),
+        // the last levels first, then the pair
+        if ((rc = grow_sort(c, n_done)) != JAM_OK) return rc;
+        int cur = 0;
+        const unsigned gs = (unsigned)((n_done + 255) / 256);
+        if (n_done > 0) {
+            int bits = 1;
+            while ((1ll << bits) <= (long long)maxcnt) bits++;
+            const int per = std::max(1, 64 / bits);
+            const int nt = stride;
+            hipLaunchKernelGGL(jam::k_iota, dim3(gs), dim3(256), 0, s, c->s_perm[0].as<int>(), (long long)n_done);
+            size_t tb = c->s_tmp.bytes;
+            for (int l0 = ((nt - 1) / per) * per; l0 >= 0; l0 -= per) {
+                const int cnt_l = std::min(per, nt - l0);
+                // levels [l0, l0 + cnt_l) packed into cnt_l * bits bits
+                hipLaunchKernelGGL(jam::k_sortkey, dim3(gs), dim3(256), 0, s, dt, pr, jn,
+                                   c->s_perm[cur].as<const int>(), (long long)n_done, l0, cnt_l, bits,
+                                   c->s_key[0].as<unsigned long long>());
+                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+                    c->s_tmp.ptr, tb, c->s_key[0].as<unsigned long long>(), c->s_key[1].as<unsigned long long>(),
+                    c->s_perm[cur].as<int>(), c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, cnt_l * bits, s));
+                cur ^= 1;
+            }
+            hipLaunchKernelGGL(jam::k_pairkey, dim3(gs), dim3(256), 0, s, jn, c->s_perm[cur].as<const int>(),
+                               (long long)n_done, c->s_pkey[0].as<unsigned int>());
+            int pbits = 1;
+            while ((1ll << pbits) < (long long)n_pairs) pbits++;
+            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->s_tmp.ptr, tb, c->s_pkey[0].as<unsigned int>(),
+                                                       c->s_pkey[1].as<unsigned int>(), c->s_perm[cur].as<int>(),
+                                                       c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, pbits, s));
+            cur ^= 1;
+            HIP_TRY(hipGetLastError());
+        }
+        const int *perm = c->s_perm[cur].as<const int>();
+
+        // ---- superposition of every complete match, events
         DevEvents ev;
         ev.cap = c->event_cap;
         ev.stride = stride;
@@ -852,12 +910,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         ev.c1 = c->e_c1.as<double>();
         ev.atoms = c->e_atoms.as<int>();
         ev.count = c->e_count.as<unsigned long long>();
-        HIP_TRY(hipMemsetAsync(ev.count, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ev.count, 1ll);
         if (n_done > 0) {
-            const unsigned gb = (unsigned)((n_done + 255) / 256);
-            hipLaunchKernelGGL(jam::k_first, dim3(gb), dim3(256), 0, s, fd.done_pair, (long long)n_done,
+            hipLaunchKernelGGL(jam::k_first, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
                                c->p_first.as<long long>());
-            hipLaunchKernelGGL(jam::k_super, dim3(gb), dim3(256), 0, s, dm, dt, st, pr, fd, ev, qp,
+            hipLaunchKernelGGL(jam::k_super, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
                                (long long)n_done, (const long long *)c->p_first.as<long long>());
             HIP_TRY(hipGetLastError());
         }
@@ -876,14 +933,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipEventElapsedTime(&ms_k[3], c->ev[3], c->ev[4]));
         HIP_TRY(hipEventElapsedTime(&ms_k[4], c->ev[0], c->ev[4]));
         for (int k = 0; k < 5; k++) R->ms[k] = ms_k[k];
+        const int n_launched = n0 > 0 ? levels : 0;
         R->expand_ms = c->profile ? 0.0 : -1.0;
-        for (int q = 0; c->profile && q < 2 * n_levels; q++) {
+        for (int q = 0; c->profile && q < n_launched; q++) {
             float x = 0.f;
             HIP_TRY(hipEventElapsedTime(&x, c->xev[2 * q], c->xev[2 * q + 1]));
             R->expand_ms += x;
         }
-        R->expand_launches = c->profile ? 2 * n_levels : 0;
-        for (int q = 0; q < JAM_MAX_LEVELS; q++) R->level_partials[q] = lp[q];
+        R->expand_launches = c->profile ? n_launched : 0;
+        for (int q = 0; q < JAM_MAX_LEVELS; q++) R->level_partials[q] = 0;
+        for (int L = 1; L < JAM_MAX_LEVELS && L <= n_launched; L++) R->level_partials[L] = (int64_t)jc[L - 1];
         const int64_t ne = (int64_t)ne_dev;
         R->pair.resize(ne);
         R->rank.resize(ne);
@@ -904,10 +963,10 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             HIP_TRY(hipMemcpyAsync(R->atoms.data(), ev.atoms, sizeof(int) * ne * stride, hipMemcpyDeviceToHost, s));
         }
         if (n_done > 0) {
-            HIP_TRY(hipMemsetAsync(c->p_cm.ptr, 0, sizeof(long long) * n_pairs, s));
-            hipLaunchKernelGGL(k_pair_matches, dim3((unsigned)((n_done + 255) / 256)), dim3(256), 0, s,
-                               c->d_pair.as<int>(), (long long)n_done, (const long long *)c->p_first.as<long long>(),
-                               c->p_cm.as<long long>());
+            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min(1024, (n_pairs + 255) / 256)), dim3(256), 0, s,
+                               (unsigned long long *)c->p_cm.ptr, (long long)n_pairs);
+            hipLaunchKernelGGL(jam::k_pair_matches, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
+                               (const long long *)c->p_first.as<long long>(), c->p_cm.as<long long>());
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(R->pair_cm.data(), c->p_cm.ptr, sizeof(long long) * n_pairs,
                                    hipMemcpyDeviceToHost, s));

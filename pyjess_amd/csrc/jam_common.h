@@ -12,9 +12,12 @@
 //               atom type (deduplicated across templates): mode, names,
 //               upper-cased residue names in template order.
 //   candidate   one set per (molecule, atom type): the candidate list in
-//   sets        reference order, and a copy in kd-tree DFS leaf order as SoA
-//               (x, y, z, atom, resSeq, chain) with one bounding box per 64
+//   sets        reference order (molecule-local atom index), and the set in
+//               kd-tree DFS leaf order (a 32-byte entry: x, y, z, chain,
+//               resSeq; and the atom index) with one bounding box per 64
 //               consecutive entries.
+//   join        frontier rows (pair, one entry index per placed level),
+//               double buffered, and the complete matches.
 #pragma once
 
 #include <stdint.h>
@@ -55,6 +58,13 @@ struct DevTpls {
     const uint32_t *type_rns;    // upper-cased packed residue names, template order
 };
 
+// One candidate-set entry as the join reads it: coordinates, chainID2 and
+// resSeq of the atom, 32 bytes (two 16-byte loads).
+struct SetEnt {
+    double x, y, z;
+    int ch, res;
+};
+
 struct DevSets {
     int n_sets;               // M * U
     int64_t cap;              // capacity of the entry arrays
@@ -62,51 +72,51 @@ struct DevSets {
     int *cnt;                 // per set
     int64_t *off;             // per set: exclusive scan of cnt
     int *list;                // reference-order candidate list (molecule-local atom)
-    int *kidx;                // scratch: kd permutation
-    double *kx, *ky, *kz;     // kd DFS order
-    int *katom, *kres, *kch;
+    int *kidx;                // scratch: kd permutation (sets larger than the LDS paths)
+    SetEnt *kent;             // kd DFS order
+    int *katom;               // kd DFS order: molecule-local atom
     double *bbox;             // 6 per block: lo xyz, hi xyz
     int64_t *total;           // device scalar: sum of cnt
+    int *maxcnt;              // device scalar: largest cnt
 };
 
 #define JAM_MAX_LEVELS 32
+
+// What the join needs about level l of a pair: its candidate set (id, size,
+// first entry) and the template atom placed at that level.
+struct PairLevel {
+    int set, cnt, off, k;
+};
 
 struct DevPairs {
     int n_pairs;
     long long n_items;
     unsigned char *ord;       // JAM_MAX_LEVELS per pair: level -> template atom
+    PairLevel *lvl;           // JAM_MAX_LEVELS per pair
     int *nchunk;              // level-0 partial matches per pair
     int *n1;                  // first level-0 list index (1 for one-atom templates)
     long long *item_base;     // exclusive scan of nchunk
     int *item_pair;           // pair of every level-0 partial
 };
 
-// Partial matches of one level: a pair and one position per placed level
-// (level 0: index in the candidate list, level l >= 1: kd position).
-struct DevFront {
-    int level;                // L: the level being added
-    int stride;               // ints per position row
-    long long n_in;
+// One level of the join. Rows are partial matches: the pair and, per placed
+// level, the entry index of its atom (level 0: into the list entries,
+// level l >= 1: into the kd entries). Rows are appended in any order; the
+// complete matches are sorted into the reference order at the end.
+struct DevJoin {
+    int level;                          // L: the level being added
+    int stride;                         // ints per row
+    const unsigned long long *n_in;     // rows of the input frontier (may exceed in_cap)
+    long long in_cap;
     const int *in_pair;
-    const int *in_pos;
-    int *cnt_next, *cnt_done; // children per partial (continuing / complete)
-    long long *off_next, *off_done;
-    int *out_pair;
-    int *out_pos;
-    long long done_base;      // complete matches found at earlier levels
-    int *done_pair;
-    int *done_pos;
-};
-
-struct DevMatches {
-    int64_t cap;
-    int stride;
-    int *item;                // work item that found the match
-    int *seq;                 // rank within the work item
-    int *atoms;               // stride per match: molecule-local atom per template atom
-    unsigned long long *count;  // device scalar
-    int *item_cm;             // complete matches per work item
-    long long *item_pre;      // exclusive scan of item_cm
+    const int *in_ent;
+    unsigned long long *n_out;          // slots taken in the output frontier
+    long long out_cap;
+    int *out_pair, *out_ent;
+    unsigned long long *n_done;         // complete matches
+    long long done_cap;
+    int *done_pair, *done_ent;
+    unsigned int *overflow;             // set when a row did not fit
 };
 
 struct DevEvents {

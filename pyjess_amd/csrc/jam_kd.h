@@ -9,8 +9,9 @@
 // Join walks the tree depth first, right child first, so the order in which
 // candidates are met -- and therefore the order of the hits -- is the tree's
 // leaf order read backwards. The kernels here compute that order for every
-// set and store the set as SoA in it (x, y, z, atom, resSeq, chain) with one
-// bounding box per 64 consecutive entries, which the join uses for pruning.
+// set and store the set in it (32-byte entries: x, y, z, chain, resSeq; and
+// the atom index) with one bounding box per 64 consecutive entries, which the
+// join uses for pruning.
 //
 // Two algorithms, both exact:
 //   fast   when no range of the tree has equal coordinates on both sides of
@@ -109,12 +110,14 @@ __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st,
         x = ms.xyz[3 * g + 0];
         y = ms.xyz[3 * g + 1];
         z = ms.xyz[3 * g + 2];
-        st.kx[off + p] = x;
-        st.ky[off + p] = y;
-        st.kz[off + p] = z;
+        SetEnt en;
+        en.x = x;
+        en.y = y;
+        en.z = z;
+        en.ch = ms.chain[g];
+        en.res = ms.resseq[g];
+        st.kent[off + p] = en;
         st.katom[off + p] = atom;
-        st.kres[off + p] = ms.resseq[g];
-        st.kch[off + p] = ms.chain[g];
     }
     double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
     double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};

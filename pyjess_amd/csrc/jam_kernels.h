@@ -9,12 +9,14 @@
 //                  that order plus one bounding box per 64 entries.
 //   k_pairs        thread per (template, molecule) pair: level order and the
 //                  number of level-0 partial matches.
-//   k_expand       wave per partial match of level L-1: the candidates of
-//                  level L (patches/Scanner.c.patch:232-334) from the 64-entry
-//                  blocks of the kd-ordered set whose box meets the join box,
-//                  all 64 lanes testing one candidate each against every
-//                  annulus and check(); a count pass, a scan, a fill pass keep
-//                  the depth-first order of the reference.
+//   k_join         one launch per join level, a wave per partial match of
+//                  level L-1: the candidates of level L
+//                  (patches/Scanner.c.patch:232-334) from the 64-entry blocks
+//                  of the kd-ordered set whose box meets the join box, all 64
+//                  lanes testing one candidate each against every annulus and
+//                  check(); children appended in any order.
+//   k_sortkey      complete matches sorted by (pair, level positions): the
+//                  reference's depth-first order and each match's rank.
 //   k_super        thread per complete match: FP64 superposition, RMSD
 //                  threshold, event compaction with the match's rank.
 #pragma once
@@ -156,6 +158,15 @@ __global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs 
             // a one-atom template steps past entry 0 before its first match
             const int c0 = cnt[ord[0]] - (n == 1 ? 1 : 0);
             chunks = c0 > 0 ? c0 : 0;
+            PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
+            for (int l = 0; l < n; l++) {
+                PairLevel h;
+                h.k = ord[l];
+                h.set = m * U + tp.atom_type[tbase + ord[l]];
+                h.cnt = cnt[ord[l]];
+                h.off = (int)st.off[h.set];
+                lv[l] = h;
+            }
         }
     }
     pr.nchunk[p] = chunks;
@@ -180,93 +191,127 @@ __global__ void __launch_bounds__(256) k_items(DevPairs pr)
 // candidate list, level L >= 1 the candidates of its set inside every annulus
 // around the atoms placed at levels < L, in kd DFS order, filtered by check().
 // Here every partial match of level L-1 is extended to level L by one wave,
-// all partials in parallel. Children are written after a scan of their
-// counts, so each level's list stays in depth-first order: parents in order,
-// the children of a parent in kd order. The complete matches of a pair are
-// thus contiguous and in the reference's order.
+// all partials of the level in parallel, one launch per level, and the
+// children are appended in any order. Depth-first order is lexicographic
+// order of (level-0 list position, level-1 kd position, ...) within a pair,
+// so sorting the complete matches by (pair, positions) recovers the
+// reference's enumeration order and each match's rank.
 
-// Level-0 partials: the candidates of every pair's level-0 list, in order.
-__global__ void __launch_bounds__(256) k_level0(DevPairs pr, DevFront fr)
+// Level-0 rows: the candidates of every pair's level-0 list.
+__global__ void __launch_bounds__(256) k_level0(DevPairs pr, DevJoin jn)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *jn.n_out = (unsigned long long)pr.n_items;
     if (i >= pr.n_items) return;
     const int p = pr.item_pair[i];
-    const int n1 = pr.n1[p];
-    fr.out_pair[i] = p;
-    fr.out_pos[i * fr.stride] = (int)(i - pr.item_base[p]) + n1;
+    const int pos0 = (int)(i - pr.item_base[p]) + pr.n1[p];
+    jn.out_pair[i] = p;
+    jn.out_ent[i * jn.stride] = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off + pos0;
 }
 
+#define JOIN_WAVES 4
+#define JOIN_STAGE 256
+#ifndef JOIN_UNROLL
+#define JOIN_UNROLL 1
+#endif
+// sets of at most JOIN_SMALL blocks skip the block-box test: their candidates
+// are loaded at once
+#ifndef JOIN_SMALL
+#define JOIN_SMALL 4
+#endif
+
 template <int MAXN>
-struct ExpandLds {
+struct JoinLds {
     double px[MAXN], py[MAXN], pz[MAXN];
-    int pch[MAXN], pres[MAXN];
     double r2lo[MAXN], r2hi[MAXN];
+    int pch[MAXN], pres[MAXN];
     int tch[MAXN], tres[MAXN];
+    int ent[MAXN];
+    // children waiting to be written: parent row and the new entry
+    int sp[JOIN_STAGE], se[JOIN_STAGE];
 };
 
+// Writes the wave's staged children to the next frontier: one atomic for all
+// of them, then the rows (parent's entries + the new one) by all lanes.
+template <int MAXN>
+__device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
+{
+    if (staged == 0) return;
+    wave_sync();
+    const int lane = lane_id();
+    unsigned long long b0 = 0;
+    if (lane == 0) b0 = atomicAdd(jn.n_out, (unsigned long long)staged);
+    const long long base = (long long)__shfl(b0, 0, JAM_WAVE);
+    const int L = jn.level;
+    const int w = L + 1;
+    const int total = staged * w;
+    for (int idx = lane; idx < total; idx += JAM_WAVE) {
+        const int k = idx / w, l = idx - k * w;
+        const long long r = base + k;
+        if (r >= jn.out_cap) {
+            if (l == 0) atomicOr(jn.overflow, 1u);
+            continue;
+        }
+        const int par = S.sp[k];
+        jn.out_ent[r * jn.stride + l] = l < L ? jn.in_ent[(long long)par * jn.stride + l] : S.se[k];
+        if (l == 0) jn.out_pair[r] = jn.in_pair[par];
+    }
+    wave_sync();
+    staged = 0;
+}
+
 // One wave extends partial i of level L-1 by the candidates of level L.
-// COUNT: number of children. FILL: write the children (parent row + new kd
-// position) to the next level or, when they complete the template, to the
-// list of complete matches.
-template <int MAXN, bool FILL>
-__device__ static void expand(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
-                              const DevFront &fr, const QueryParams &qp, ExpandLds<MAXN> &S, long long i)
+template <int MAXN>
+__device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
+                                const DevJoin &jn, const QueryParams &qp, JoinLds<MAXN> &S, long long i,
+                                int &staged)
 {
     const int lane = lane_id();
-    const int L = fr.level;
-    const int p = fr.in_pair[i];
-    const int T = tp.n_templates, U = tp.n_types;
-    const int m = p / T, t = p % T;
+    const int L = jn.level;
+    const int p = jn.in_pair[i];
+    const int T = tp.n_templates;
+    const int t = p % T;
     const int tbase = tp.atom_off[t];
     const int n = tp.atom_off[t + 1] - tbase;
+    const int *row = jn.in_ent + i * jn.stride;
+    const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
     if (L >= n) {
-        // a one-atom template is complete at level 0: the partial itself
-        if (FILL) {
-            if (lane == 0) {
-                const long long r = fr.done_base + fr.off_done[i];
-                fr.done_pair[r] = p;
-                fr.done_pos[r * fr.stride] = fr.in_pos[i * fr.stride];
+        // a one-atom template is complete at level 0
+        if (lane == 0) {
+            const unsigned long long r = atomicAdd(jn.n_done, 1ull);
+            if ((long long)r < jn.done_cap) {
+                jn.done_pair[r] = p;
+                jn.done_ent[r * jn.stride] = row[0];
+            } else {
+                atomicOr(jn.overflow, 1u);
             }
-        } else if (lane == 0) {
-            fr.cnt_next[i] = 0;
-            fr.cnt_done[i] = 1;
         }
         return;
     }
-    const unsigned char *ord = pr.ord + (size_t)p * JAM_MAX_LEVELS;
-    const int kL = ord[L];
-    const int setL = m * U + tp.atom_type[tbase + kL];
-    const int cntL = st.cnt[setL];
-    const long long offL = st.off[setL];
+    const PairLevel hL = lv[L];
+    const int kL = hL.k, setL = hL.set, cntL = hL.cnt, offL = hL.off;
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
     if (lane < L) {
         // the atom placed at level `lane`
-        const int k = ord[lane];
-        const int sset = m * U + tp.atom_type[tbase + k];
-        const int pos = fr.in_pos[i * fr.stride + lane];
-        double x, y, z;
-        int ch, res;
+        const int e = row[lane];
+        const int k = lv[lane].k;
+        SetEnt en;
         if (lane == 0) {
-            const int atom = st.list[st.off[sset] + pos];
-            const size_t g = (size_t)(ms.atom_off[m] + atom);
-            x = ms.xyz[3 * g + 0];
-            y = ms.xyz[3 * g + 1];
-            z = ms.xyz[3 * g + 2];
-            ch = ms.chain[g];
-            res = ms.resseq[g];
+            const size_t g = (size_t)(ms.atom_off[p / T] + st.list[e]);
+            en.x = ms.xyz[3 * g + 0];
+            en.y = ms.xyz[3 * g + 1];
+            en.z = ms.xyz[3 * g + 2];
+            en.ch = ms.chain[g];
+            en.res = ms.resseq[g];
         } else {
-            const long long e = st.off[sset] + pos;
-            x = st.kx[e];
-            y = st.ky[e];
-            z = st.kz[e];
-            ch = st.kch[e];
-            res = st.kres[e];
+            en = st.kent[e];
         }
-        S.px[lane] = x;
-        S.py[lane] = y;
-        S.pz[lane] = z;
-        S.pch[lane] = ch;
-        S.pres[lane] = res;
+        S.ent[lane] = e;
+        S.px[lane] = en.x;
+        S.py[lane] = en.y;
+        S.pz[lane] = en.z;
+        S.pch[lane] = en.ch;
+        S.pres[lane] = en.res;
         S.tch[lane] = tp.chain[tbase + k];
         S.tres[lane] = tp.resseq[tbase + k];
         // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
@@ -285,35 +330,35 @@ __device__ static void expand(const DevMols &ms, const DevTpls &tp, const DevSet
         if (b < 0.0) b = 0.0;
         S.r2lo[lane] = a * a;
         S.r2hi[lane] = b * b;
-        lo[0] = x - b;
-        lo[1] = y - b;
-        lo[2] = z - b;
-        hi[0] = x + b;
-        hi[1] = y + b;
-        hi[2] = z + b;
+        lo[0] = en.x - b;
+        lo[1] = en.y - b;
+        lo[2] = en.z - b;
+        hi[0] = en.x + b;
+        hi[1] = en.y + b;
+        hi[2] = en.z + b;
     }
-    // join box (Join_computeBox, patches/Join.c.patch:154-196)
-    for (int sh = 32; sh >= 1; sh >>= 1) {
+    const int nblk = (cntL + JAM_WAVE - 1) / JAM_WAVE;
+    if (nblk > JOIN_SMALL) {
+        // join box (Join_computeBox, patches/Join.c.patch:154-196)
+        for (int sh = 32; sh >= 1; sh >>= 1) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
-            hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
+                hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
+            }
         }
     }
     const int tchL = tp.chain[tbase + kL], tresL = tp.resseq[tbase + kL];
     wave_sync();
     const int ignore = qp.ignore_chain;
-    const int nblk = (cntL + JAM_WAVE - 1) / JAM_WAVE;
-    const long long bbase = offL / JAM_WAVE + setL;
+    const long long bbase = (long long)offL / JAM_WAVE + setL;
     const bool last = (L == n - 1);
-    long long wout = 0;
-    if (FILL) wout = last ? fr.done_base + fr.off_done[i] : fr.off_next[i];
-    int count = 0;
     for (int g = 0; g * JAM_WAVE < nblk; g++) {
-        // blocks of this group whose box meets the join box
+        // blocks of this group whose box meets the join box (all of them for
+        // small sets: the test is a prefilter only)
         unsigned long long bm;
-        if (nblk == 1) {
-            bm = 1ull;
+        if (nblk <= JOIN_SMALL) {
+            bm = (nblk >= 64) ? ~0ull : ((1ull << nblk) - 1ull);
         } else {
             const int b = g * JAM_WAVE + lane;
             bool ok = false;
@@ -325,96 +370,187 @@ __device__ static void expand(const DevMols &ms, const DevTpls &tp, const DevSet
             bm = ballot(ok);
         }
         while (bm) {
-            const int blk = g * JAM_WAVE + first_bit(bm);
-            bm &= bm - 1;
-            const int q = blk * JAM_WAVE + lane;
-            bool ok = q < cntL;
-            if (ok) {
-                const long long e = offL + q;
-                const double x = st.kx[e], y = st.ky[e], z = st.kz[e];
-                const int ch = st.kch[e], res = st.kres[e];
-                for (int j = 0; j < L; j++) {
-                    // _Annulus_po: squared distance summed x, y, z from 0.0
-                    double tx = S.px[j] - x, ty = S.py[j] - y, tz = S.pz[j] - z;
-                    double s2 = 0.0;
-                    s2 += tx * tx;
-                    s2 += ty * ty;
-                    s2 += tz * tz;
-                    if (s2 < S.r2lo[j] || s2 > S.r2hi[j]) ok = false;
-                    // TessTemplate_check
-                    const bool same_chain = S.pch[j] == ch;
-                    const bool tsame_chain = S.tch[j] == tchL;
-                    if (ignore == 0) {
-                        if (same_chain != tsame_chain) ok = false;
-                        if (!same_chain) continue;
-                    }
-                    const bool same_res = S.pres[j] == res;
-                    const bool tsame_res = S.tres[j] == tresL;
-                    if (same_res != tsame_res) ok = false;
-                    if (ignore != 2 && same_res && !same_chain) ok = false;
+            // up to JOIN_UNROLL blocks per round, their loads in flight together
+            int blk[JOIN_UNROLL];
+            int nb = 0;
+#pragma unroll
+            for (int u = 0; u < JOIN_UNROLL; u++) {
+                blk[u] = -1;
+                if (bm) {
+                    blk[u] = g * JAM_WAVE + first_bit(bm);
+                    bm &= bm - 1;
+                    nb++;
                 }
             }
-            const unsigned long long mask = ballot(ok);
-            if (FILL && ok) {
-                const long long r = wout + count + rank_below(mask);
-                int *row = last ? fr.done_pos + r * fr.stride : fr.out_pos + r * fr.stride;
-                for (int l = 0; l < L; l++) row[l] = fr.in_pos[i * fr.stride + l];
-                row[L] = q;
-                if (last) fr.done_pair[r] = p;
-                else fr.out_pair[r] = p;
+            SetEnt cand[JOIN_UNROLL];
+#pragma unroll
+            for (int u = 0; u < JOIN_UNROLL; u++) {
+                const int q = blk[u] * JAM_WAVE + lane;
+                if (blk[u] >= 0 && q < cntL) cand[u] = st.kent[offL + q];
             }
-            count += __popcll(mask);
+#pragma unroll
+            for (int u = 0; u < JOIN_UNROLL; u++) {
+                if (u >= nb) break;
+                const int q = blk[u] * JAM_WAVE + lane;
+                bool ok = q < cntL;
+                if (ok) {
+                    const SetEnt c = cand[u];
+                    for (int j = 0; j < L; j++) {
+                        // _Annulus_po: squared distance summed x, y, z from 0.0
+                        double tx = S.px[j] - c.x, ty = S.py[j] - c.y, tz = S.pz[j] - c.z;
+                        double s2 = 0.0;
+                        s2 += tx * tx;
+                        s2 += ty * ty;
+                        s2 += tz * tz;
+                        if (s2 < S.r2lo[j] || s2 > S.r2hi[j]) ok = false;
+                        // TessTemplate_check
+                        const bool same_chain = S.pch[j] == c.ch;
+                        const bool tsame_chain = S.tch[j] == tchL;
+                        if (ignore == 0) {
+                            if (same_chain != tsame_chain) ok = false;
+                            if (!same_chain) continue;
+                        }
+                        const bool same_res = S.pres[j] == c.res;
+                        const bool tsame_res = S.tres[j] == tresL;
+                        if (same_res != tsame_res) ok = false;
+                        if (ignore != 2 && same_res && !same_chain) ok = false;
+                    }
+                }
+                const unsigned long long mask = ballot(ok);
+                if (mask == 0ull) continue;
+                const int cnt = __popcll(mask);
+                const int r = rank_below(mask);
+                if (last) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(jn.n_done, (unsigned long long)cnt);
+                    base = __shfl(base, 0, JAM_WAVE);
+                    const long long slot = (long long)base + r;
+                    if (ok) {
+                        if (slot < jn.done_cap) {
+                            int *out = jn.done_ent + slot * jn.stride;
+                            for (int l = 0; l < L; l++) out[l] = S.ent[l];
+                            out[L] = offL + q;
+                            jn.done_pair[slot] = p;
+                        } else {
+                            atomicOr(jn.overflow, 1u);
+                        }
+                    }
+                } else {
+                    if (staged + cnt > JOIN_STAGE) join_flush(jn, S, staged);
+                    if (ok) {
+                        S.sp[staged + r] = (int)i;
+                        S.se[staged + r] = offL + q;
+                    }
+                    staged += cnt;
+                }
+            }
         }
-    }
-    if (!FILL && lane == 0) {
-        fr.cnt_next[i] = last ? 0 : count;
-        fr.cnt_done[i] = last ? count : 0;
     }
 }
 
-template <int MAXN, bool FILL>
-__global__ void __launch_bounds__(64) k_expand(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevFront fr,
-                                               QueryParams qp)
+// One launch per join level: waves loop over the input rows (the grid is what
+// stays resident on the device); the output frontier's counter is also the
+// next level's input size, so levels run back to back without the host.
+template <int MAXN>
+__global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
+                                                              DevJoin jn, QueryParams qp)
 {
-    __shared__ ExpandLds<MAXN> S;
-    const long long i = blockIdx.x;
-    if (i >= fr.n_in) return;
-    expand<MAXN, FILL>(ms, tp, st, pr, fr, qp, S, i);
+    __shared__ JoinLds<MAXN> S[JOIN_WAVES];
+    const int w = threadIdx.x / JAM_WAVE;
+    long long n_in = (long long)*jn.n_in;
+    if (n_in > jn.in_cap) n_in = jn.in_cap;
+    int staged = 0;
+    const long long nw = (long long)gridDim.x * JOIN_WAVES;
+    for (long long i = (long long)blockIdx.x * JOIN_WAVES + w; i < n_in; i += nw) {
+        wave_sync();
+        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, staged);
+    }
+    join_flush(jn, S[w], staged);
+}
+
+// ---------------------------------------------------------------------------
+// reference order of the complete matches
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_iota(int *v, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (int)i;
+}
+
+// Sort key of sorted position s for levels [l0, l0 + per): the positions of
+// the match's atoms in their sets (list position at level 0, kd position
+// above), `bits` bits each, the earliest level most significant; levels past
+// the template's size count as 0 (all matches of a pair have the same size).
+__global__ void __launch_bounds__(256) k_sortkey(DevTpls tp, DevPairs pr, DevJoin jn, const int *perm,
+                                                 long long n, int l0, int per, int bits, unsigned long long *key)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = perm[s];
+    const int p = jn.done_pair[i];
+    const int t = p % tp.n_templates;
+    const int nt = tp.atom_off[t + 1] - tp.atom_off[t];
+    const int *row = jn.done_ent + (long long)i * jn.stride;
+    unsigned long long k = 0;
+    for (int q = 0; q < per; q++) {
+        const int l = l0 + q;
+        unsigned long long pos = 0;
+        if (l < nt) pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
+        k = (k << bits) | pos;
+    }
+    key[s] = k;
+}
+
+__global__ void __launch_bounds__(256) k_pairkey(DevJoin jn, const int *perm, long long n, unsigned int *key)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) key[s] = (unsigned int)jn.done_pair[perm[s]];
+}
+
+// first sorted position of every pair with complete matches
+__global__ void __launch_bounds__(256) k_first(DevJoin jn, const int *perm, long long n, long long *first)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p = jn.done_pair[perm[s]];
+    if (s == 0 || jn.done_pair[perm[s - 1]] != p) first[p] = s;
+}
+
+// complete matches per pair
+__global__ void __launch_bounds__(256) k_pair_matches(DevJoin jn, const int *perm, long long n,
+                                                      const long long *first, long long *out)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p = jn.done_pair[perm[s]];
+    if (s == n - 1 || jn.done_pair[perm[s + 1]] != p) out[p] = s - first[p] + 1;
 }
 
 // ---------------------------------------------------------------------------
 // superposition + events
 // ---------------------------------------------------------------------------
 
-// first complete match of every pair: the matches of a pair are contiguous
-__global__ void __launch_bounds__(256) k_first(const int *done_pair, long long n, long long *first)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (i == 0 || done_pair[i - 1] != done_pair[i]) first[done_pair[i]] = i;
-}
-
-__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevFront fr,
-                                               DevEvents ev, QueryParams qp, long long n_matches,
+// Thread per complete match, in reference order: superposition, RMSD
+// threshold, event with the match's rank within its pair.
+__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevJoin jn,
+                                               DevEvents ev, QueryParams qp, const int *perm, long long n_matches,
                                                const long long *first)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_matches) return;
-    const int pair = fr.done_pair[i];
-    const int T = tp.n_templates, U = tp.n_types;
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_matches) return;
+    const int i = perm[s];
+    const int pair = jn.done_pair[i];
+    const int T = tp.n_templates;
     const int m = pair / T, t = pair % T;
     const int tbase = tp.atom_off[t];
     const int n = tp.atom_off[t + 1] - tbase;
     const int abase = ms.atom_off[m];
     const unsigned char *ord = pr.ord + (size_t)pair * JAM_MAX_LEVELS;
-    const int *pos = fr.done_pos + i * fr.stride;
+    const int *row = jn.done_ent + (long long)i * jn.stride;
     // molecule-local atom of template atom k: placed at the level l with ord[l] == k
     int atoms[JAM_MAX_LEVELS];
-    for (int l = 0; l < n; l++) {
-        const int k = ord[l];
-        const int sset = m * U + tp.atom_type[tbase + k];
-        atoms[k] = (l == 0) ? st.list[st.off[sset] + pos[0]] : st.katom[st.off[sset] + pos[l]];
-    }
+    for (int l = 0; l < n; l++) atoms[ord[l]] = (l == 0) ? st.list[row[0]] : st.katom[row[l]];
     auto pa = [&](int k, int d) { return ms.xyz[3 * (size_t)(abase + atoms[k]) + d]; };
     auto pb = [&](int k, int d) { return tp.pos[3 * (size_t)(tbase + k) + d]; };
     double rmsd, rot[9], c0[3], c1[3];
@@ -426,7 +562,7 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets s
     const unsigned long long e = atomicAdd(ev.count, 1ull);
     if (e >= (unsigned long long)ev.cap) return;
     ev.pair[e] = pair;
-    ev.rank[e] = i - first[pair];
+    ev.rank[e] = s - first[pair];
     ev.flag[e] = nan ? 2 : 1;
     ev.rmsd[e] = rmsd;
 #pragma unroll
