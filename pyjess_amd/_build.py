@@ -1,7 +1,8 @@
-"""Compiles the native library ``libjess_amd.so`` in-tree with hipcc for gfx950.
+"""Compiles the package's native code in-tree: ``libjess_amd.so`` (hipcc,
+gfx950) and the ``_hits`` extension (Cython, then the host C compiler).
 
-Run ``python -m pyjess_amd._build`` (or ``__graft_entry__.build()``). The
-library is written next to this file so that it travels with the source tree.
+Run ``python pyjess_amd/_build.py`` (or ``__graft_entry__.build()``). Both are
+written next to this file so that they travel with the source tree.
 """
 
 from __future__ import annotations
@@ -10,6 +11,7 @@ import os
 import shutil
 import subprocess
 import sys
+import sysconfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -48,8 +50,32 @@ def flags() -> list:
     ]
 
 
+def build_hits(verbose: bool = False) -> str:
+    """Cythonize ``_hits.pyx`` and compile it into an extension module."""
+    from Cython.Build import cythonize
+
+    pyx = os.path.join(HERE, "_hits.pyx")
+    c_file = os.path.join(HERE, "_hits.c")
+    cythonize([pyx], language_level=3, quiet=not verbose, force=True)
+    out = os.path.join(HERE, "_hits" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cc = os.environ.get("CC") or shutil.which("gcc") or "cc"
+    cmd = [cc, "-O2", "-shared", "-fPIC", "-fno-strict-aliasing", f"-I{sysconfig.get_paths()['include']}",
+           c_file, "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        sys.stderr.write(proc.stderr)
+        raise RuntimeError(f"building the _hits extension failed with exit code {proc.returncode}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source into ``libjess_amd.so``; returns its path."""
+    """Compile every source into ``libjess_amd.so`` and the ``_hits``
+    extension; returns the library's path."""
+    build_hits(verbose)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB_PATH + ".tmp"
     cmd = [hipcc()] + flags() + srcs + ["-o", tmp]

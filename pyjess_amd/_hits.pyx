@@ -1,0 +1,220 @@
+# cython: language_level=3, boundscheck=False, wraparound=False
+"""`Hit`, the hit list builder and the iteration of `Query`, compiled
+(reference ``src/pyjess/_jess.pyx:2090-2621``).
+
+A query returns thousands of hits; each is a small object that refers to a
+row of its query's result arrays (rotation, centroids, matched atoms), read
+when asked for. Building them here keeps the per-hit cost of the drop-in API
+to an allocation.
+"""
+
+cimport cython
+
+import io
+import math
+import warnings
+
+import numpy as np
+
+from ._model import Atom, Mat4, Molecule, Template, _apply
+
+# TessTemplate_logE (patches/TessTemplate.c.patch:164-170):
+# logE = logA + alpha * rmsd + beta * dim + ln(n). logA is visible in the
+# reference; alpha and beta are fixed by the 156 logE values of the
+# 1AMY+1.3.3 fixture (the only pair consistent with all of them at their
+# printed precision).
+LOG_A = -8.5
+ALPHA = 2.5
+BETA = -0.7
+
+
+class Block:
+    """Row-major result arrays shared by the hits of one query."""
+
+    __slots__ = ("rotation", "c0", "c1", "atoms")
+
+    def __init__(self, rotation, c0, c1, atoms):
+        self.rotation = rotation
+        self.c0 = c0
+        self.c1 = c1
+        self.atoms = atoms
+
+
+@cython.freelist(1024)
+cdef class Hit:
+    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``)."""
+
+    cdef readonly double rmsd
+    cdef object _block
+    cdef Py_ssize_t _row
+    cdef object _template
+    cdef object _molecule
+
+    def __init__(self, rmsd, rotation, c0, c1, atoms, template, molecule):
+        self.rmsd = float(rmsd)
+        self._block = Block(np.asarray(rotation, dtype=np.float64).reshape(1, 9),
+                            np.asarray(c0, dtype=np.float64).reshape(1, 3),
+                            np.asarray(c1, dtype=np.float64).reshape(1, 3),
+                            np.asarray(list(atoms), dtype=np.int64).reshape(1, -1))
+        self._row = 0
+        self._template = template
+        self._molecule = molecule
+
+    @property
+    def _rotation(self):
+        return self._block.rotation[self._row].tolist()
+
+    @property
+    def _centre(self):
+        return self._block.c0[self._row].tolist(), self._block.c1[self._row].tolist()
+
+    @property
+    def _index(self):
+        return self._block.atoms[self._row, :len(self._template)].tolist()
+
+    @property
+    def _atoms(self):
+        mol = self._molecule
+        return [mol._atom(k) for k in self._index]
+
+    def __reduce__(self):
+        c0, c1 = self._centre
+        return Hit, (self.rmsd, self._rotation, c0, c1, list(self._index), self._template, self._molecule)
+
+    def __repr__(self):
+        return f"<Hit rmsd={self.rmsd!r} template={self._template.id!r}>"
+
+    @property
+    def transformation(self):
+        """Matrix taking molecule coordinates into the template frame (``_jess.pyx:2343-2377``)."""
+        M = self._rotation
+        c, v = self._centre
+        Mc = np.eye(4)
+        Mc[0:3, 3] = [-c[0], -c[1], -c[2]]
+        Mr = np.eye(4)
+        Mr[0:3, 0:3] = np.array(M).reshape(3, 3)
+        Mv = np.eye(4)
+        Mv[0:3, 3] = v
+        return Mat4._wrap(Mv) @ Mat4._wrap(Mr) @ Mat4._wrap(Mc)
+
+    @property
+    def inverse_transformation(self):
+        """Matrix taking template coordinates into the molecule frame (``_jess.pyx:2379-2413``)."""
+        M = self._rotation
+        c, v = self._centre
+        Mv = np.eye(4)
+        Mv[0:3, 3] = [-v[0], -v[1], -v[2]]
+        Mr = np.eye(4)
+        Mr[0:3, 0:3] = np.array(M).reshape(3, 3).T
+        Mc = np.eye(4)
+        Mc[0:3, 3] = c
+        return Mat4._wrap(Mc) @ Mat4._wrap(Mr) @ Mat4._wrap(Mv)
+
+    @property
+    def determinant(self):
+        p = self._rotation
+        det = 0.0
+        det += p[0] * (p[4] * p[8] - p[5] * p[7])
+        det -= p[1] * (p[3] * p[8] - p[5] * p[6])
+        det += p[2] * (p[3] * p[7] - p[4] * p[6])
+        return det
+
+    @property
+    def log_evalue(self):
+        n = len(self._molecule)
+        return LOG_A + ALPHA * self.rmsd + BETA * float(self._template.dimension) + math.log(float(n))
+
+    @property
+    def evalue(self):
+        return math.exp(self.log_evalue)
+
+    def atoms(self, transform=True):
+        if not transform:
+            return self._atoms
+        trans = self.transformation
+        return [a.transform(trans) for a in self._atoms]
+
+    def molecule(self, transform=False):
+        if not transform:
+            return self._molecule
+        return self._molecule.transform(self.transformation)
+
+    def template(self, transform=False):
+        if not transform:
+            return self._template
+        return self._template.transform(self.inverse_transformation)
+
+    def dumps(self, format="pdb", transform=True):
+        out = io.StringIO()
+        self.dump(out, format=format, transform=transform)
+        return out.getvalue()
+
+    def dump(self, file, format="pdb", transform=True):
+        if format != "pdb":
+            raise ValueError(f"invalid value for `format` argument: {format!r}")
+        if self._template.id is None:
+            raise RuntimeError("cannot dump `Hit` where `self.template.id` is `None`")
+        if self._molecule.id is None:
+            raise RuntimeError("cannot dump `Hit` where `self.molecule.id` is `None`")
+        file.write("REMARK ")
+        file.write(self._molecule.id)
+        file.write(f" {self.rmsd:5.3f} ")
+        file.write(self._template.id)
+        file.write(f" Det={self.determinant:4,.1f} log(E)~ {self.log_evalue:4.2f}\n")
+        mat = [float(v) for v in np.asarray(self.transformation).reshape(16)]
+        for a in self._atoms:
+            if transform:
+                x, y, z = _apply(mat, a.x, a.y, a.z)
+            else:
+                x, y, z = a.x, a.y, a.z
+            file.write(a._line(x, y, z))
+        file.write("ENDMDL\n")
+
+
+def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const int[:] tpl_of,
+               const int[:] flag, const double[:] rmsd, object block, list templates, list molecules):
+    """Appends to out[molecule] the hit of every chosen event (None for a NaN
+    rotation), in the given order."""
+    cdef Py_ssize_t i, e, n = chosen.shape[0]
+    cdef int mi
+    cdef Hit h
+    for i in range(n):
+        e = chosen[i]
+        mi = mol_of[e]
+        if flag[e] == 2:
+            (<list>out[mi]).append(None)
+            continue
+        h = Hit.__new__(Hit)
+        h.rmsd = rmsd[e]
+        h._block = block
+        h._row = e
+        h._template = templates[tpl_of[e]]
+        h._molecule = molecules[mi]
+        (<list>out[mi]).append(h)
+
+
+cdef class HitIterator:
+    """Iteration of `Query` over its hits in pyjess order; NaN-rotation
+    entries (None) become warnings, as in ``Query.__next__``
+    (``_jess.pyx:2239-2251``). Subclasses provide ``_run()``, which returns the
+    hit list the first time the iterator is advanced."""
+
+    cdef public object _hits
+    cdef public Py_ssize_t _pos
+
+    def __iter__(self):
+        return self
+
+    def __next__(self):
+        if self._hits is None:
+            self._hits = self._run()
+        cdef list hits = self._hits
+        cdef Py_ssize_t n = len(hits)
+        while self._pos < n:
+            item = hits[self._pos]
+            self._pos += 1
+            if item is None:
+                warnings.warn("Jess returned a superposition matrix with NaN values", UserWarning, stacklevel=2)
+                continue
+            return item
+        raise StopIteration

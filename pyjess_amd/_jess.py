@@ -25,15 +25,6 @@ from ._model import Atom, Mat4, Molecule, Template, _apply
 
 __all__ = ["Jess", "Query", "Hit"]
 
-# TessTemplate_logE (patches/TessTemplate.c.patch:164-170):
-# logE = logA + alpha * rmsd + beta * dim + ln(n). logA is visible in the
-# reference; alpha and beta are fixed by the 156 logE values of the
-# 1AMY+1.3.3 fixture (the only pair consistent with all of them at their
-# printed precision).
-_LOG_A = -8.5
-_ALPHA = 2.5
-_BETA = -0.7
-
 _IGNORE = {None: 0, "residues": 1, "atoms": 2}
 _IGNORE_NAMES = {0: None, 1: "residues", 2: "atoms"}
 
@@ -42,162 +33,10 @@ def _warn_nan():
     warnings.warn("Jess returned a superposition matrix with NaN values", UserWarning, stacklevel=3)
 
 
-class _Block:
-    """Row-major result arrays shared by the hits of one query."""
-
-    __slots__ = ("rotation", "c0", "c1", "atoms")
-
-    def __init__(self, rotation, c0, c1, atoms):
-        self.rotation = rotation
-        self.c0 = c0
-        self.c1 = c1
-        self.atoms = atoms
+from ._hits import Block as _Block, Hit, HitIterator as _HitIterator, build_hits as _build_hits  # noqa: E402
 
 
-class Hit:
-    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``).
-
-    A hit refers to a row of its query's result arrays; rotation, centroids
-    and matched atoms are read from there when asked for."""
-
-    __slots__ = ("rmsd", "_block", "_row", "_template", "_molecule")
-
-    def __init__(self, rmsd: float, rotation: Sequence[float], c0: Sequence[float], c1: Sequence[float],
-                 atoms, template: Template, molecule: Molecule):
-        self.rmsd = float(rmsd)
-        self._block = _Block(np.asarray(rotation, dtype=np.float64).reshape(1, 9),
-                             np.asarray(c0, dtype=np.float64).reshape(1, 3),
-                             np.asarray(c1, dtype=np.float64).reshape(1, 3),
-                             np.asarray(list(atoms), dtype=np.int64).reshape(1, -1))
-        self._row = 0
-        self._template = template
-        self._molecule = molecule
-
-    @classmethod
-    def _at(cls, rmsd: float, block: _Block, row: int, template: Template, molecule: Molecule) -> "Hit":
-        h = cls.__new__(cls)
-        h.rmsd = rmsd
-        h._block = block
-        h._row = row
-        h._template = template
-        h._molecule = molecule
-        return h
-
-    @property
-    def _rotation(self) -> List[float]:
-        return self._block.rotation[self._row].tolist()
-
-    @property
-    def _centre(self):
-        return self._block.c0[self._row].tolist(), self._block.c1[self._row].tolist()
-
-    @property
-    def _index(self) -> List[int]:
-        return self._block.atoms[self._row, :len(self._template)].tolist()
-
-    @property
-    def _atoms(self) -> List[Atom]:
-        return [self._molecule._atom(k) for k in self._index]
-
-    def __reduce__(self):
-        return _restore_hit, (self.rmsd, self._rotation, self._centre[0], self._centre[1],
-                              list(self._index), self._template, self._molecule)
-
-    def __repr__(self):
-        return f"<Hit rmsd={self.rmsd!r} template={self._template.id!r}>"
-
-    @property
-    def transformation(self) -> Mat4:
-        """Matrix taking molecule coordinates into the template frame (``_jess.pyx:2343-2377``)."""
-        M = self._rotation
-        c, v = self._centre
-        Mc = np.eye(4)
-        Mc[0:3, 3] = [-c[0], -c[1], -c[2]]
-        Mr = np.eye(4)
-        Mr[0:3, 0:3] = np.array(M).reshape(3, 3)
-        Mv = np.eye(4)
-        Mv[0:3, 3] = v
-        return Mat4._wrap(Mv) @ Mat4._wrap(Mr) @ Mat4._wrap(Mc)
-
-    @property
-    def inverse_transformation(self) -> Mat4:
-        """Matrix taking template coordinates into the molecule frame (``_jess.pyx:2379-2413``)."""
-        M = self._rotation
-        c, v = self._centre
-        Mv = np.eye(4)
-        Mv[0:3, 3] = [-v[0], -v[1], -v[2]]
-        Mr = np.eye(4)
-        Mr[0:3, 0:3] = np.array(M).reshape(3, 3).T
-        Mc = np.eye(4)
-        Mc[0:3, 3] = c
-        return Mat4._wrap(Mc) @ Mat4._wrap(Mr) @ Mat4._wrap(Mv)
-
-    @property
-    def determinant(self) -> float:
-        p = self._rotation
-        det = 0.0
-        det += p[0] * (p[4] * p[8] - p[5] * p[7])
-        det -= p[1] * (p[3] * p[8] - p[5] * p[6])
-        det += p[2] * (p[3] * p[7] - p[4] * p[6])
-        return det
-
-    @property
-    def log_evalue(self) -> float:
-        n = len(self._molecule)
-        return _LOG_A + _ALPHA * self.rmsd + _BETA * float(self._template.dimension) + math.log(float(n))
-
-    @property
-    def evalue(self) -> float:
-        return math.exp(self.log_evalue)
-
-    def atoms(self, transform: bool = True) -> List[Atom]:
-        if not transform:
-            return self._atoms
-        trans = self.transformation
-        return [a.transform(trans) for a in self._atoms]
-
-    def molecule(self, transform: bool = False) -> Molecule:
-        if not transform:
-            return self._molecule
-        return self._molecule.transform(self.transformation)
-
-    def template(self, transform: bool = False) -> Template:
-        if not transform:
-            return self._template
-        return self._template.transform(self.inverse_transformation)
-
-    def dumps(self, format: str = "pdb", transform: bool = True) -> str:
-        out = io.StringIO()
-        self.dump(out, format=format, transform=transform)
-        return out.getvalue()
-
-    def dump(self, file, format: str = "pdb", transform: bool = True) -> None:
-        if format != "pdb":
-            raise ValueError(f"invalid value for `format` argument: {format!r}")
-        if self._template.id is None:
-            raise RuntimeError("cannot dump `Hit` where `self.template.id` is `None`")
-        if self._molecule.id is None:
-            raise RuntimeError("cannot dump `Hit` where `self.molecule.id` is `None`")
-        file.write("REMARK ")
-        file.write(self._molecule.id)
-        file.write(f" {self.rmsd:5.3f} ")
-        file.write(self._template.id)
-        file.write(f" Det={self.determinant:4,.1f} log(E)~ {self.log_evalue:4.2f}\n")
-        mat = [float(v) for v in np.asarray(self.transformation).reshape(16)]
-        for a in self._atoms:
-            if transform:
-                x, y, z = _apply(mat, a.x, a.y, a.z)
-            else:
-                x, y, z = a.x, a.y, a.z
-            file.write(a._line(x, y, z))
-        file.write("ENDMDL\n")
-
-
-def _restore_hit(rmsd, rotation, c0, c1, atoms, template, molecule):
-    return Hit(rmsd, rotation, c0, c1, atoms, template, molecule)
-
-
-class Query:
+class Query(_HitIterator):
     """An iterator over the hits of a query (``_jess.pyx:2090-2283``)."""
 
     def __init__(self):
@@ -244,21 +83,6 @@ class Query:
         else:
             raise ValueError(f"invalid value for `max_candidates` argument: {max_candidates!r}")
 
-    def __iter__(self):
-        return self
-
-    def __next__(self) -> Hit:
-        if self._hits is None:
-            self._hits = self._run()
-        while self._pos < len(self._hits):
-            item = self._hits[self._pos]
-            self._pos += 1
-            if item is None:        # a NaN rotation reached in pyjess's iteration order
-                _warn_nan()
-                continue
-            return item
-        raise StopIteration
-
     def _run(self) -> list:
         if len(self.jess) == 0 or len(self.molecule) == 0:
             return []
@@ -289,24 +113,19 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
     tpl_of = res.pair % T
     # molecule, then scan position of the template (reverse order), then rank
     order = np.lexsort((res.rank, (T - 1) - tpl_of, mol_of))
+    # the candidate counter of a template never exceeds its number of complete
+    # matches, so a cap at or above the largest one never cuts anything
+    if mc != -1 and len(res.pair_matches) and mc >= int(res.pair_matches.max()):
+        mc = -1
     if not best_match and mc == -1:
         # no policy can drop an event: every event is reported, in order
         chosen = order
     else:
         chosen = _select(res, order, best_match, mc)
     block = _Block(res.rotation, res.c0, res.c1, res.atoms)
-    rmsd = res.rmsd.tolist()
-    flag = res.flag.tolist()
-    mol_l = mol_of.tolist()
-    tpl_l = tpl_of.tolist()
-    templates = jess._templates
-    make = Hit._at
-    for e in (chosen.tolist() if isinstance(chosen, np.ndarray) else chosen):
-        mi = mol_l[e]
-        if flag[e] == 2:
-            out[mi].append(None)
-            continue
-        out[mi].append(make(rmsd[e], block, e, templates[tpl_l[e]], molecules[mi]))
+    _build_hits(out, np.asarray(chosen, dtype=np.int64), mol_of.astype(np.int32), tpl_of.astype(np.int32),
+                np.ascontiguousarray(res.flag, dtype=np.int32), np.ascontiguousarray(res.rmsd, dtype=np.float64),
+                block, list(jess._templates), list(molecules))
     return out
 
 

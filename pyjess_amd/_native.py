@@ -1,6 +1,6 @@
 """ctypes binding of the native library (``libjess_amd.so``, C ABI in ``include/jess_amd.h``).
 
-The library must be built (``python -m pyjess_amd._build``) and a gfx950 GPU
+The library must be built (``python pyjess_amd/_build.py``) and a gfx950 GPU
 must be visible for queries; there is no CPU fallback, every failure raises.
 """
 
@@ -99,7 +99,7 @@ def lib():
         path = library_path()
         if not os.path.exists(path):
             raise ImportError(
-                f"native library {path} is missing: build it with `python -m pyjess_amd._build`"
+                f"native library {path} is missing: build it with `python pyjess_amd/_build.py`"
             )
         L = ctypes.CDLL(path)
         vp, i32, i64, P = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER

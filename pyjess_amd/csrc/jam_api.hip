@@ -717,7 +717,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_sets, st.cnt, st.off, n_sets, s));
         HIP_TRY(hipcub::DeviceReduce::Max(c->scan_tmp.ptr, scan_max, st.cnt, st.maxcnt, n_sets, s));
         hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
-        hipLaunchKernelGGL(jam::k_pairs, dim3(pair_blocks), dim3(256), 0, s, dt, st, pr, qp);
+        hipLaunchKernelGGL(jam::k_pairs, dim3((unsigned)((n_pairs + 7) / 8)), dim3(256), 0, s, dt, st, pr, qp);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_pairs, pr.nchunk, pr.item_base, n_pairs,
                                                  s));

@@ -125,52 +125,61 @@ __global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevS
 // reordering, _CandidateSet_size_compare_r, patches/Scanner.c.patch:98-108),
 // and the number of level-0 partial matches. A pair with an empty set has
 // none (Scanner_reuse returns NULL, patches/Scanner.c.patch:162-166).
+// Half a wave per pair, lane k for template atom k (templates have at most
+// 32 atoms): the level of atom k is its rank by (set size, k) when
+// reordering, k otherwise.
 __global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs pr, QueryParams qp)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pr.n_pairs) return;
+    const int lane = lane_id();
+    const int team = lane >> 5, k = lane & 31;
+    const int p = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / JAM_WAVE) * 2 + team;
+    const bool valid = p < pr.n_pairs;
     const int T = tp.n_templates, U = tp.n_types;
-    const int m = p / T, t = p % T;
-    const int tbase = tp.atom_off[t];
-    const int n = tp.atom_off[t + 1] - tbase;
-    unsigned char *ord = pr.ord + (size_t)p * JAM_MAX_LEVELS;
-    int chunks = 0;
-    if (n > 0 && n <= JAM_MAX_LEVELS) {
-        int cnt[JAM_MAX_LEVELS];
-        bool empty = false;
-        for (int k = 0; k < n; k++) {
-            cnt[k] = st.cnt[m * U + tp.atom_type[tbase + k]];
-            empty |= cnt[k] == 0;
-            ord[k] = (unsigned char)k;
-        }
-        if (!empty) {
-            if (qp.reorder) {
-                for (int i = 1; i < n; i++) {
-                    const unsigned char v = ord[i];
-                    int j = i - 1;
-                    while (j >= 0 && (cnt[ord[j]] > cnt[v] || (cnt[ord[j]] == cnt[v] && ord[j] > v))) {
-                        ord[j + 1] = ord[j];
-                        j--;
-                    }
-                    ord[j + 1] = v;
-                }
-            }
-            // a one-atom template steps past entry 0 before its first match
-            const int c0 = cnt[ord[0]] - (n == 1 ? 1 : 0);
-            chunks = c0 > 0 ? c0 : 0;
-            PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
-            for (int l = 0; l < n; l++) {
-                PairLevel h;
-                h.k = ord[l];
-                h.set = m * U + tp.atom_type[tbase + ord[l]];
-                h.cnt = cnt[ord[l]];
-                h.off = (int)st.off[h.set];
-                lv[l] = h;
-            }
+    int m = 0, t = 0, tbase = 0, n = 0;
+    if (valid) {
+        m = p / T;
+        t = p % T;
+        tbase = tp.atom_off[t];
+        n = tp.atom_off[t + 1] - tbase;
+    }
+    const bool mine = valid && k < n;
+    int set = 0, cnt = 0;
+    long long off = 0;
+    if (mine) {
+        set = m * U + tp.atom_type[tbase + k];
+        cnt = st.cnt[set];
+        off = st.off[set];
+    }
+    // an empty candidate set means no match for the pair (Scanner_reuse
+    // returns NULL, patches/Scanner.c.patch:162-166)
+    const unsigned long long empty = ballot(mine && cnt == 0);
+    const unsigned long long team_mask = team ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+    const bool any_empty = (empty & team_mask) != 0ull;
+    int rank = k;
+    if (qp.reorder) {
+        rank = 0;
+        for (int j = 0; j < 32; j++) {
+            const int cj = __shfl(cnt, (team << 5) + j, JAM_WAVE);
+            if (j < n && (cj < cnt || (cj == cnt && j < k))) rank++;
         }
     }
-    pr.nchunk[p] = chunks;
-    pr.n1[p] = (n == 1) ? 1 : 0;
+    if (mine && !any_empty) {
+        pr.ord[(size_t)p * JAM_MAX_LEVELS + rank] = (unsigned char)k;
+        PairLevel h;
+        h.k = k;
+        h.set = set;
+        h.cnt = cnt;
+        h.off = (int)off;
+        pr.lvl[(size_t)p * JAM_MAX_LEVELS + rank] = h;
+    }
+    if (valid && k == 0) pr.n1[p] = (n == 1) ? 1 : 0;
+    // level-0 list size: the atom of rank 0; a one-atom template steps past
+    // entry 0 before its first match
+    if (valid && (n == 0 || any_empty) && k == 0) pr.nchunk[p] = 0;
+    if (mine && !any_empty && rank == 0) {
+        const int c0 = cnt - (n == 1 ? 1 : 0);
+        pr.nchunk[p] = c0 > 0 ? c0 : 0;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_items(DevPairs pr)

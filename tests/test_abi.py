@@ -28,7 +28,7 @@ def test_header_declares_the_api():
 def test_library_exports_every_declared_symbol():
     from pyjess_amd import _native
     path = _native.library_path()
-    assert os.path.exists(path), "build the library first: python -m pyjess_amd._build"
+    assert os.path.exists(path), "build the library first: python pyjess_amd/_build.py"
     lib = ctypes.CDLL(path)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing

@@ -55,6 +55,8 @@ def events_of(stream, thr, n_templates):
     res.c0 = np.zeros((len(rows), 3))
     res.c1 = np.zeros((len(rows), 3))
     res.atoms = np.array([r[5] for r in rows], dtype=np.int32).reshape(-1, 2)
+    # complete matches per pair (before the RMSD threshold), as the device reports them
+    res.pair_matches = np.array([pos.get(t, 0) for t in range(n_templates)], dtype=np.int64)
     return res
 
 
