@@ -165,6 +165,30 @@ int32_t jam_superpose_batch(jam_context *ctx, const double *a, const double *b, 
                             int64_t n_sets, double *rmsd, double *rotation9, double *c0_3,
                             double *c1_3);
 
+/* ---- differentiable superposition RMSD --------------------------------- */
+
+/* RMSD of the least-squares superposition of a onto b for a batch of point
+ * sets of any sizes, all arrays in DEVICE memory: set s owns points
+ * [off[s], off[s+1]) of a and b (3 doubles per point, x y z); rmsd receives
+ * n_sets values, rot9 (may be NULL) the 9 row-major entries of each optimal
+ * proper rotation. The quantity of Superposition_rmsd/rotation
+ * (include/jess/super.pxd:1-14), computed for differentiation: rotation from
+ * a one-sided Jacobi SVD of the correlation, RMSD from the residuals, so that
+ * near-degenerate sets keep full precision (a set of rank < 2 gets one of its
+ * optimal rotations instead of NaN). Asynchronous on `stream` (a hipStream_t;
+ * NULL for the default stream). */
+int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+                         double *rmsd, double *rot9, void *stream);
+/* Gradient of sum_s grad_rmsd[s] * rmsd[s] with respect to a and b (device
+ * arrays laid out like a and b; either may be NULL), given the rmsd of
+ * jam_rmsd_forward. No counterpart in the reference, which has no gradients:
+ * d rmsd / d a_i = (A_i - R^T B_i) / (n rmsd) and
+ * d rmsd / d b_i = (B_i - R A_i) / (n rmsd), A and B the centred points, R
+ * the optimal proper rotation; zero where rmsd is 0. Asynchronous on `stream`. */
+int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+                          const double *rmsd, const double *grad_rmsd, double *grad_a, double *grad_b,
+                          void *stream);
+
 /* ---- misc ------------------------------------------------------------- */
 
 /* Waits until all work queued on the context's device has finished

@@ -19,7 +19,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB_NAME = "libjess_amd.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["jam_api.hip"]
-HEADERS = ["jam_common.h", "jam_kd.h", "jam_kernels.h", "jam_super.h", "jam_wave.h"]
+HEADERS = ["jam_common.h", "jam_grad.h", "jam_kd.h", "jam_kernels.h", "jam_super.h", "jam_wave.h"]
 ARCH = "gfx950"
 
 

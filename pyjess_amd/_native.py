@@ -27,7 +27,8 @@ _SYMBOLS = (
     "jam_context_create", "jam_context_free", "jam_templates_create", "jam_templates_free",
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
-    "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_result_free", "jam_superpose_batch", "jam_last_error",
+    "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_rmsd_forward",
+    "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch", "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -133,6 +134,10 @@ def lib():
         L.jam_result_join_kernels.restype = i32
         L.jam_result_level_partials.argtypes = [vp, vp]
         L.jam_result_level_partials.restype = i32
+        L.jam_rmsd_forward.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+        L.jam_rmsd_forward.restype = i32
+        L.jam_rmsd_backward.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.jam_rmsd_backward.restype = i32
         L.jam_context_set_profiling.argtypes = [vp, i32]
         L.jam_context_set_profiling.restype = i32
         L.jam_result_free.argtypes = [vp]

@@ -23,6 +23,7 @@
 
 #include "../../include/jess_amd.h"
 #include "jam_common.h"
+#include "jam_grad.h"
 #include "jam_kernels.h"
 
 namespace {
@@ -185,6 +186,37 @@ struct jam_result {
     int64_t level_partials[JAM_MAX_LEVELS] = {};   // partial matches expanded per join level
     int expand_launches = 0;
 };
+
+// ---------------------------------------------------------------------------
+// superposition RMSD and its gradient on device arrays
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+                                    double *rmsd, double *rot9, void *stream)
+{
+    if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets == 0) return JAM_OK;
+    if (!a || !b || !off || !rmsd) return fail(JAM_EINVAL, "NULL argument");
+    const int tb = 256;
+    hipLaunchKernelGGL(jam::k_rmsd_fwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
+                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, rot9);
+    HIP_TRY(hipGetLastError());
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+                                     const double *rmsd, const double *grad_rmsd, double *grad_a, double *grad_b,
+                                     void *stream)
+{
+    if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets == 0) return JAM_OK;
+    if (!a || !b || !off || !rmsd || !grad_rmsd || (!grad_a && !grad_b)) return fail(JAM_EINVAL, "NULL argument");
+    const int tb = 256;
+    hipLaunchKernelGGL(jam::k_rmsd_bwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
+                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, grad_rmsd, grad_a, grad_b);
+    HIP_TRY(hipGetLastError());
+    return JAM_OK;
+}
 
 // ---------------------------------------------------------------------------
 // misc

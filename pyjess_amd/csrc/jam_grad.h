@@ -1,0 +1,244 @@
+// jam_grad.h -- differentiable superposition RMSD of many point-set pairs.
+//
+// rmsd = sqrt(E / n), E = sum_i |B_i - R A_i|^2, A and B the centred points
+// and R the proper rotation that minimises E (the least-squares
+// superposition of a onto b: the quantity of Superposition_rmsd,
+// include/jess/super.pxd:1-14). With X = sum_i B_i A_i^T = U S V^T,
+// R = U diag(1, 1, det(U) det(V)) V^T.
+//
+// R is computed from a one-sided Jacobi SVD of X (Hestenes: plane rotations
+// of the columns until they are orthogonal; Demmel & Veselic, "Jacobi's
+// method is more accurate than QR", 1992), which keeps the small singular
+// values -- nearly planar or nearly collinear sets -- to high relative
+// accuracy, with u3 = u1 x u2 so that the frame is proper even when
+// sigma3 = 0. E is then summed from the residuals directly: no cancellation,
+// and only second order in the error of R because R minimises E. (The
+// search's hit decisions use the reference's eigenvalue formula instead,
+// jam_super.h; the two agree to rounding on well-conditioned sets.)
+//
+// Gradient: dE/dR vanishes at the optimum (envelope theorem) and centring
+// drops out of the chain rule because centred points sum to zero, so
+//     d rmsd / d a_i = (A_i - R^T B_i) / (n rmsd),
+//     d rmsd / d b_i = (B_i - R A_i) / (n rmsd);
+// taken as zero where rmsd = 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace jam {
+
+// Optimal proper rotation R (row-major 3x3) for the correlation X.
+__device__ static inline void optimal_rotation(const double X[3][3], double R[3][3])
+{
+    double W[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            W[i][j] = X[i][j];
+            V[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 16; sweep++) {
+        bool rotated = false;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const int p = (r == 2) ? 1 : 0;
+            const int q = (r == 0) ? 1 : 2;
+            double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                al += W[j][p] * W[j][p];
+                be += W[j][q] * W[j][q];
+                ga += W[j][p] * W[j][q];
+            }
+            if (ga == 0.0 || fabs(ga) <= 1e-15 * sqrt(al * be)) continue;
+            const double ze = (be - al) / (2.0 * ga);
+            double t = 1.0 / (fabs(ze) + sqrt(1.0 + ze * ze));
+            if (ze < 0.0) t = -t;
+            const double c = 1.0 / sqrt(1.0 + t * t);
+            const double s = c * t;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double wp = W[j][p], wq = W[j][q];
+                W[j][p] = c * wp - s * wq;
+                W[j][q] = s * wp + c * wq;
+                const double vp = V[j][p], vq = V[j][q];
+                V[j][p] = c * vp - s * vq;
+                V[j][q] = s * vp + c * vq;
+            }
+            rotated = true;
+        }
+        if (!rotated) break;
+    }
+    // columns of W are U S: order by singular value
+    double sg[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) sg[i] = sqrt(W[0][i] * W[0][i] + W[1][i] * W[1][i] + W[2][i] * W[2][i]);
+    int i1 = 0, i2 = 1, i3 = 2;
+    if (sg[i2] > sg[i1]) { int t = i1; i1 = i2; i2 = t; }
+    if (sg[i3] > sg[i2]) { int t = i2; i2 = i3; i3 = t; }
+    if (sg[i2] > sg[i1]) { int t = i1; i1 = i2; i2 = t; }
+    double u1[3], u2[3], u3[3], v1[3], v2[3], v3[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        u1[j] = sg[i1] > 0.0 ? W[j][i1] / sg[i1] : (j == 0 ? 1.0 : 0.0);
+        u2[j] = W[j][i2];
+        v1[j] = V[j][i1];
+        v2[j] = V[j][i2];
+        v3[j] = V[j][i3];
+    }
+    if (sg[i2] > 0.0) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) u2[j] /= sg[i2];
+    } else {
+        // rank <= 1: any unit vector orthogonal to u1 (the axis of u1's
+        // smallest component, made orthogonal)
+        int ax = 0;
+        if (fabs(u1[1]) < fabs(u1[ax])) ax = 1;
+        if (fabs(u1[2]) < fabs(u1[ax])) ax = 2;
+        double e[3] = {0.0, 0.0, 0.0};
+        e[ax] = 1.0;
+        const double dp = u1[0] * e[0] + u1[1] * e[1] + u1[2] * e[2];
+        double w[3] = {e[0] - dp * u1[0], e[1] - dp * u1[1], e[2] - dp * u1[2]};
+        const double nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+#pragma unroll
+        for (int j = 0; j < 3; j++) u2[j] = w[j] / nw;
+    }
+    u3[0] = u1[1] * u2[2] - u1[2] * u2[1];
+    u3[1] = u1[2] * u2[0] - u1[0] * u2[2];
+    u3[2] = u1[0] * u2[1] - u1[1] * u2[0];
+    const double detv = v1[0] * (v2[1] * v3[2] - v2[2] * v3[1]) - v1[1] * (v2[0] * v3[2] - v2[2] * v3[0]) +
+                        v1[2] * (v2[0] * v3[1] - v2[1] * v3[0]);
+    const double sgn = detv < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) R[j][k] = u1[j] * v1[k] + u2[j] * v2[k] + sgn * u3[j] * v3[k];
+}
+
+// Centroids and the correlation X of points [0, n) of A and B.
+__device__ static inline void centre_and_correlate(int n, const double *A, const double *B, double ca[3],
+                                                   double cb[3], double X[3][3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ca[d] = 0.0;
+        cb[d] = 0.0;
+    }
+    for (int i = 0; i < n; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ca[d] += A[3 * i + d];
+            cb[d] += B[3 * i + d];
+        }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        ca[d] /= (double)n;
+        cb[d] /= (double)n;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) X[j][k] = 0.0;
+    for (int i = 0; i < n; i++) {
+        double u[3], v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            u[d] = A[3 * i + d] - ca[d];
+            v[d] = B[3 * i + d] - cb[d];
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) X[j][k] += v[j] * u[k];
+    }
+}
+
+// Set s owns points [off[s], off[s+1]) of a and b, 3 doubles per point.
+__global__ void __launch_bounds__(256) k_rmsd_fwd(const double *a, const double *b, const int64_t *off,
+                                                  long long n_sets, double *rmsd, double *rot)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sets) return;
+    const int64_t p0 = off[s];
+    const int n = (int)(off[s + 1] - p0);
+    if (n <= 0) {
+        rmsd[s] = 0.0;
+        if (rot)
+            for (int q = 0; q < 9; q++) rot[9 * s + q] = (q % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    const double *A = a + 3 * p0;
+    const double *B = b + 3 * p0;
+    double ca[3], cb[3], X[3][3], R[3][3];
+    centre_and_correlate(n, A, B, ca, cb, X);
+    optimal_rotation(X, R);
+    double e = 0.0;
+    for (int i = 0; i < n; i++) {
+        double u[3], v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            u[d] = A[3 * i + d] - ca[d];
+            v[d] = B[3 * i + d] - cb[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double ra = R[d][0] * u[0] + R[d][1] * u[1] + R[d][2] * u[2];
+            const double r = v[d] - ra;
+            e += r * r;
+        }
+    }
+    rmsd[s] = sqrt(e / (double)n);
+    if (rot)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) rot[9 * s + 3 * j + k] = R[j][k];
+}
+
+__global__ void __launch_bounds__(256) k_rmsd_bwd(const double *a, const double *b, const int64_t *off,
+                                                  long long n_sets, const double *rmsd, const double *grad_rmsd,
+                                                  double *grad_a, double *grad_b)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_sets) return;
+    const int64_t p0 = off[s];
+    const int n = (int)(off[s + 1] - p0);
+    if (n <= 0) return;
+    const double *A = a + 3 * p0;
+    const double *B = b + 3 * p0;
+    double *GA = grad_a ? grad_a + 3 * p0 : nullptr;
+    double *GB = grad_b ? grad_b + 3 * p0 : nullptr;
+    const double r = rmsd[s];
+    const double g = grad_rmsd[s];
+    if (!(r > 0.0) || g == 0.0) {
+        for (int q = 0; q < 3 * n; q++) {
+            if (GA) GA[q] = 0.0;
+            if (GB) GB[q] = 0.0;
+        }
+        return;
+    }
+    double ca[3], cb[3], X[3][3], R[3][3];
+    centre_and_correlate(n, A, B, ca, cb, X);
+    optimal_rotation(X, R);
+    const double scale = g / ((double)n * r);
+    for (int i = 0; i < n; i++) {
+        double u[3], v[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            u[d] = A[3 * i + d] - ca[d];
+            v[d] = B[3 * i + d] - cb[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            // (R^T B_i)_d and (R A_i)_d
+            const double rtb = R[0][d] * v[0] + R[1][d] * v[1] + R[2][d] * v[2];
+            const double ra = R[d][0] * u[0] + R[d][1] * u[1] + R[d][2] * u[2];
+            if (GA) GA[3 * i + d] = scale * (u[d] - rtb);
+            if (GB) GB[3 * i + d] = scale * (v[d] - ra);
+        }
+    }
+}
+
+}  // namespace jam

@@ -360,3 +360,56 @@ def _batch_molecules(rng: np.random.Generator, sizes: Sequence[int]) -> List[Mol
         start = int(rng.integers(0, pool_len - n + 1))
         mols.append(protein(rng, n, mol_id=f"M{i:04d}", ca=w[start:start + n]))
     return mols
+
+
+def random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    """n rotation matrices, uniform over SO(3) (unit quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def superposition_pairs(rng: np.random.Generator, n_sets: int, k_min: int = 3, k_max: int = 12,
+                        noise: float = 0.5, collinear: float = 0.02, scatter: float = 2e-2, n_proteins: int = 4):
+    """Point-set pairs for the differentiable RMSD: (a, b, offsets).
+
+    Each query set a is ``k`` consecutive atoms (k uniform in [k_min, k_max])
+    of a synthetic protein; its template b is a random rotation of it, moved
+    to another place, plus N(0, noise) per coordinate. A fraction
+    ``collinear`` of the sets are near-degenerate: k = 3 points on a line
+    (spacing 1.5 A) with ``scatter`` A of scatter. Returns a and b as
+    (points, 3) float64 and offsets as int64 (n_sets + 1).
+    """
+    pool = np.concatenate([protein_with_atoms(rng, 8000, mol_id=f"DIF{i}").xyz for i in range(n_proteins)])
+    k = rng.integers(k_min, k_max + 1, size=n_sets)
+    deg = rng.random(n_sets) < collinear
+    k[deg] = 3
+    offsets = np.zeros(n_sets + 1, dtype=np.int64)
+    np.cumsum(k, out=offsets[1:])
+    total = int(offsets[-1])
+    set_of = np.repeat(np.arange(n_sets), k)
+    local = np.arange(total) - offsets[set_of]
+    start = rng.integers(0, len(pool) - k_max, size=n_sets)
+    a = pool[start[set_of] + local].copy()
+    # near-degenerate sets: three points on a random line
+    if deg.any():
+        dsel = np.nonzero(deg[set_of])[0]
+        dirs = _unit(rng, n_sets)
+        origin = pool[start]
+        a[dsel] = (origin[set_of[dsel]] + 1.5 * local[dsel, None] * dirs[set_of[dsel]]
+                   + rng.normal(scale=scatter, size=(len(dsel), 3)))
+    R = random_rotations(rng, n_sets)
+    shift = rng.normal(scale=10.0, size=(n_sets, 3))
+    b = np.einsum("nij,nj->ni", R[set_of], a) + shift[set_of] + rng.normal(scale=noise, size=(total, 3))
+    return np.ascontiguousarray(a), np.ascontiguousarray(b), offsets
