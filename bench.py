@@ -10,7 +10,8 @@ HBM. The metric is template x molecule queries per second.
 
 The headline ``value`` is the plain drop-in on one 8,000-atom protein x 500
 templates; the ``batched`` object is the batched form on 4,096 molecules x 200
-templates. Each carries a CPU baseline (the C restatement of the reference
+templates; ``differentiation`` is the differentiable superposition RMSD
+(forward + backward) over 1M point-set pairs. Each carries a CPU baseline (the C restatement of the reference
 algorithm on the host's cores, hit-set parity checked) and the roofline of the
 dominant kernel, the join's k_expand launches, timed with HIP events on the
 library's stream in as many steps again after the timed ones.
@@ -239,6 +240,138 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
     return out
 
 
+def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
+    """Differentiable superposition RMSD: forward + backward over n_sets point-set
+    pairs (k uniform in 3..12, 2% near-collinear k = 3), FP64 on the GPU."""
+    import torch
+
+    from pyjess_amd import synthetic
+    from pyjess_amd.autograd import superposition_rmsd
+
+    t0 = time.time()
+    a, b, off = synthetic.superposition_pairs(np.random.default_rng(seed), n_sets)
+    n_pts = int(off[-1])
+    log(f"differentiation: {n_sets} point-set pairs ({n_pts} points) generated in {time.time() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    ta = torch.tensor(a, device=dev, requires_grad=True)
+    tb = torch.tensor(b, device=dev, requires_grad=True)
+    toff = torch.tensor(off, device=dev)
+
+    def step():
+        ta.grad = None
+        tb.grad = None
+        r = superposition_rmsd(ta, tb, toff)
+        r.sum().backward()
+        return r
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = n_sets * steps / elapsed
+    # the two kernels alone, HIP events on the stream they run on (torch's)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd_ms, bwd_ms = [], []
+    for _ in range(steps):
+        ta.grad = None
+        tb.grad = None
+        ev[0].record()
+        r = superposition_rmsd(ta, tb, toff)
+        ev[1].record()
+        r.sum().backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+        fwd_ms.append(ev[0].elapsed_time(ev[1]))
+        bwd_ms.append(ev[1].elapsed_time(ev[2]))
+    fwd_ms, bwd_ms = float(np.median(fwd_ms)), float(np.median(bwd_ms))
+    # algorithmic HBM bytes: forward reads a, b and the offsets, writes the
+    # RMSD; backward reads a, b, offsets, RMSD and upstream gradient, writes
+    # both gradients
+    fwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 8 * n_sets
+    bwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 16 * n_sets + 2 * 24 * n_pts
+    bwd_gbs = bwd_bytes / (bwd_ms / 1000.0) / 1e9
+    log(f"differentiation: {value:,.0f} matches/s forward+backward ({1000 * elapsed / steps:.3f} ms/step); "
+        f"forward {fwd_ms:.3f} ms ({fwd_bytes / fwd_ms / 1e6:.0f} GB/s), backward {bwd_ms:.3f} ms "
+        f"({bwd_gbs:.0f} GB/s, includes torch's sum backward)")
+    out = {
+        "value": value,
+        "unit": "matches/s (forward+backward)",
+        "ms_per_step": 1000.0 * elapsed / steps,
+        "config": {"workload": "rmsd-grad-1m", "sets": n_sets, "points": n_pts, "k": "3..12", "seed": seed,
+                   "near_collinear": 0.02},
+        "kernel_ms": {"forward": fwd_ms, "backward": bwd_ms},
+        "roofline": {"bound": "hbm", "kernel": "k_rmsd_fwd", "achieved": fwd_bytes / (fwd_ms / 1000.0) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None, "bytes_per_launch": fwd_bytes},
+        "roofline_backward": {"bound": "hbm", "kernel": "k_rmsd_bwd", "achieved": bwd_gbs, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "traffic": None, "bytes_per_launch": bwd_bytes},
+        "cpu_baseline": None,
+        "gradient_parity": None,
+    }
+    for k in ("roofline", "roofline_backward"):
+        out[k]["frac"] = out[k]["achieved"] / out[k]["peak"]
+    if cpu:
+        from oracle.kabsch_torch import kabsch_rmsd_grad
+
+        # parity on a sample: gradients against autograd through an SVD (CPU)
+        ga = ta.grad.detach().cpu().numpy()
+        gb = tb.grad.detach().cpu().numpy()
+        rr = r.detach().cpu().numpy()
+        worst = 0.0
+        for s in range(0, n_sets, max(1, n_sets // 200)):
+            A, B = a[off[s]:off[s + 1]], b[off[s]:off[s + 1]]
+            r0, xa, xb = kabsch_rmsd_grad(A, B)
+            x = np.concatenate([ga[off[s]:off[s + 1]], gb[off[s]:off[s + 1]]]).ravel()
+            y = np.concatenate([xa, xb]).ravel()
+            worst = max(worst, float(np.linalg.norm(x - y) / np.linalg.norm(y)), abs(rr[s] - r0) / r0)
+        out["gradient_parity"] = {"max_rel_error_vs_cpu_autograd": worst, "sample": 200}
+        # CPU baseline: the same autograd-through-SVD, batched by set size in
+        # float64 torch on the host cores, on a bounded sample
+        threads = cpu_share()
+        torch.set_num_threads(threads)
+        m = min(n_sets, 200_000)
+        sizes = np.diff(off[:m + 1])
+        groups = {}
+        for k in np.unique(sizes):
+            idx = np.nonzero(sizes == k)[0]
+            rows = (off[idx][:, None] + np.arange(k)[None, :]).ravel()
+            groups[int(k)] = (torch.tensor(a[rows].reshape(len(idx), k, 3)), torch.tensor(b[rows].reshape(len(idx), k, 3)))
+
+        def cpu_step():
+            for A, B in groups.values():
+                A = A.clone().requires_grad_(True)
+                B = B.clone().requires_grad_(True)
+                Ac = A - A.mean(dim=1, keepdim=True)
+                Bc = B - B.mean(dim=1, keepdim=True)
+                X = Bc.transpose(1, 2) @ Ac
+                S = torch.linalg.svdvals(X)
+                sg = torch.where(torch.linalg.det(X) < 0, -1.0, 1.0).to(A.dtype)
+                e = (Ac * Ac).sum((1, 2)) + (Bc * Bc).sum((1, 2)) - 2.0 * (S[:, 0] + S[:, 1] + sg * S[:, 2])
+                rr = torch.sqrt(torch.clamp(e, min=0.0) / A.shape[1])
+                rr.sum().backward()
+
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            cpu_step()
+            reps += 1
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        cpu_elapsed = time.perf_counter() - t0
+        cpu_rate = reps * m / cpu_elapsed
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "matches/s (forward+backward)", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the first {m} sets, batched by size, autograd through torch's SVD in "
+                                         f"float64 on {threads} threads, {reps} passes in {cpu_elapsed:.1f}s"}
+        out["gpu_vs_cpu"] = value / cpu_rate
+        log(f"differentiation: cpu baseline {cpu_rate:,.0f} matches/s on {threads} threads; "
+            f"gradient parity (max rel vs CPU autograd) {worst:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +385,8 @@ def main():
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the 4,096 molecules of the batched measurement (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--diff-sets", type=int, default=1_000_000,
+                    help="point-set pairs of the differentiable-RMSD measurement (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         log(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -265,6 +400,11 @@ def main():
         batched = measure("batch-4k", args.batch_scale, args.seed, args.batch_steps, 1, args.cpu_seconds,
                           not args.no_cpu, 32)
         batched["vs_plain"] = batched["value"] / main_run["value"]
+
+    diff = None
+    if args.diff_sets > 0:
+        diff = measure_diff(args.diff_sets, args.seed, max(3, args.batch_steps), 1, args.cpu_seconds,
+                            not args.no_cpu)
 
     out = {
         "metric": METRIC,
@@ -291,6 +431,7 @@ def main():
         "complete_matches": main_run["complete_matches"],
         "join_level_partials": main_run["join_level_partials"],
         "batched": batched,
+        "differentiation": diff,
     }
     print(json.dumps(out), flush=True)
 

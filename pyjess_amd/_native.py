@@ -7,7 +7,9 @@ must be visible for queries; there is no CPU fallback, every failure raises.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+import sys
 import threading
 from typing import List, Optional, Sequence
 
@@ -91,6 +93,17 @@ def library_path() -> str:
     return _build.LIB_PATH
 
 
+def _one_hip_runtime() -> None:
+    """PyTorch for ROCm bundles its own HIP runtime under the same soname as
+    the system's (libamdhip64.so.7); whichever is loaded first serves the
+    whole process, and torch cannot run on the system one. When torch is
+    installed it is therefore imported before the library is opened, so that
+    the library and torch (the differentiable RMSD, user code) share one
+    runtime whatever the import order."""
+    if "torch" not in sys.modules and importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
+
+
 def lib():
     """Loads ``libjess_amd.so``; raises if it has not been built."""
     global _lib
@@ -102,6 +115,7 @@ def lib():
             raise ImportError(
                 f"native library {path} is missing: build it with `python pyjess_amd/_build.py`"
             )
+        _one_hip_runtime()
         L = ctypes.CDLL(path)
         vp, i32, i64, P = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER
         L.jam_context_create.argtypes = [i32, P(vp)]
