@@ -13,7 +13,7 @@ templates; the ``batched`` object is the batched form on 4,096 molecules x 200
 templates; ``differentiation`` is the differentiable superposition RMSD
 (forward + backward) over 1M point-set pairs. Each carries a CPU baseline (the C restatement of the reference
 algorithm on the host's cores, hit-set parity checked) and the roofline of the
-dominant kernel, the join's k_expand launches, timed with HIP events on the
+dominant kernel, the join's k_join launches (one per level), timed with HIP events on the
 library's stream in as many steps again after the timed ones.
 
 Prints one JSON line on stdout; progress goes to stderr.
@@ -116,7 +116,7 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
     join_ms = float(np.mean(join_ms))
     launches = float(np.mean(join_launches))
     log(f"{name}: {steps} steps in {elapsed:.3f}s: {ms_per_step:.3f} ms/step, {value:,.0f} queries/s; "
-        f"device ms: candidates {st[0]:.3f} kd {st[1]:.3f} join {st[2]:.3f} (k_expand {join_ms:.3f} "
+        f"device ms: candidates {st[0]:.3f} kd {st[1]:.3f} join {st[2]:.3f} (k_join {join_ms:.3f} "
         f"in {launches:.0f} launches) super {st[3]:.3f} total {st[4]:.3f}")
 
     # the native call alone (device pipeline + event copy, no Python hit objects)
@@ -154,7 +154,7 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
         },
         "native_ms_per_step": native_ms,
         "device_ms": {"candidates": st[0], "kd_order": st[1], "join": st[2], "superposition": st[3],
-                      "total": st[4], "k_expand": join_ms, "k_expand_launches": launches},
+                      "total": st[4], "k_join": join_ms, "k_join_launches": launches},
         "hits": n_hits,
         "complete_matches": int(res.pair_matches.sum()),
         "join_level_partials": [int(x) for x in res.level_partials[1:] if x > 0],
@@ -213,13 +213,13 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
                       np.array_equal(res.pair_matches, cm))
         out["hit_set_parity"] = parity
         log(f"{name}: hit-set parity with the CPU oracle: {parity}")
-        # dominant kernel: the join's k_expand launches; FP64 VALU work (no
+        # dominant kernel: the join's k_join launches; FP64 VALU work (no
         # MFMA: nothing here is a dense contraction)
         per_launch_flops = flops_join / max(launches, 1.0)
         avg_launch_ms = join_ms / max(launches, 1.0)
         achieved = per_launch_flops / (avg_launch_ms / 1000.0) / 1e12
         out["roofline"] = {
-            "bound": "mfma", "kernel": "k_expand", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "bound": "mfma", "kernel": "k_join", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
             "per_launch_flops": per_launch_flops, "avg_launch_ms": avg_launch_ms,
             "launches_per_step": launches,
@@ -227,9 +227,9 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
                     "candidate, counted by an instrumented oracle run"
                     + ("" if work_sample is None else f" on the first {len(wmols)} molecules, scaled")
                     + ") against the 78.6 TFLOP/s FP64 peak"}
-        prof = os.path.join(ROOT, "profiles", f"k_expand.{name}.pmc.md")
+        prof = os.path.join(ROOT, "profiles", f"k_join.{name}.pmc.md")
         if os.path.exists(prof):
-            # HBM bytes per k_expand launch from the committed counter passes
+            # HBM bytes per k_join launch from the committed counter passes
             # (FETCH_SIZE and WRITE_SIZE, calibrated against a copy kernel)
             for line in open(prof):
                 if line.startswith("traffic_bytes_per_launch:"):
@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain-8k", help="plain-2k | plain-8k | batch-4k")
+    ap.add_argument("--workload", default="plain-8k", help="plain-2k | plain-8k | batch-4k | rmsd-grad-1m")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch workload's molecules")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of each CPU baseline")
@@ -393,6 +393,17 @@ def main():
             "(queries are independent: the sharding over GPUs is described in DESIGN.md)")
         sys.exit(2)
 
+    if args.workload == "rmsd-grad-1m":
+        # the differentiable RMSD alone
+        d = measure_diff(args.diff_sets or 1_000_000, args.seed, args.steps, args.warmup, args.cpu_seconds,
+                         not args.no_cpu)
+        out = {"metric": "superposition RMSD forward+backward matches/sec, 1 GPU vs CPU", "value": d["value"],
+               "unit": d["unit"], "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64", "data": "synthetic", "config": dict(d["config"], parallelism="1 GPU"),
+               "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"], "differentiation": d}
+        print(json.dumps(out), flush=True)
+        return
     main_run = measure(args.workload, args.scale, args.seed, args.steps, args.warmup, args.cpu_seconds,
                        not args.no_cpu, None if args.workload != "batch-4k" else 32)
     batched = None
@@ -402,7 +413,7 @@ def main():
         batched["vs_plain"] = batched["value"] / main_run["value"]
 
     diff = None
-    if args.diff_sets > 0:
+    if args.diff_sets > 0 and args.workload != "batch-4k":
         diff = measure_diff(args.diff_sets, args.seed, max(3, args.batch_steps), 1, args.cpu_seconds,
                             not args.no_cpu)
 

@@ -139,8 +139,8 @@ int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts);
  * join, superposition, total device time]. */
 int32_t jam_result_timings(const jam_result *r, double *ms5);
 /* Summed duration (milliseconds, HIP events on the library's stream) of the
- * join's k_expand launches of the last query -- a count and a fill pass per
- * join level, the inner loop of Scanner_next (patches/Scanner.c.patch:232-334)
+ * join's k_join launches of the last query -- one per join level, the inner
+ * loop of Scanner_next (patches/Scanner.c.patch:232-334)
  * -- and the number of those launches; -1 ms and 0 launches unless the
  * context records them (jam_context_set_profiling). */
 int32_t jam_result_join_kernels(const jam_result *r, double *ms, int32_t *launches);

@@ -337,7 +337,7 @@ class Result:
         ms = ctypes.c_double(0.0)
         nl = ctypes.c_int32(0)
         _check(L.jam_result_join_kernels(handle, ctypes.byref(ms), ctypes.byref(nl)))
-        self.join_kernel_ms = ms.value       # summed k_expand durations (HIP events)
+        self.join_kernel_ms = ms.value       # summed k_join durations (HIP events)
         self.join_launches = nl.value
         self.level_partials = np.zeros(32, dtype=np.int64)   # partials expanded per join level
         _check(L.jam_result_level_partials(handle, _ptr(self.level_partials)))

@@ -182,7 +182,7 @@ struct jam_result {
     std::vector<int64_t> rank, pair_cm;
     std::vector<double> rmsd, rot, c0, c1;
     double ms[5] = {0, 0, 0, 0, 0};
-    double expand_ms = 0.0;     // summed duration of the k_expand launches
+    double expand_ms = 0.0;     // summed duration of the k_join launches
     int64_t level_partials[JAM_MAX_LEVELS] = {};   // partial matches expanded per join level
     int expand_launches = 0;
 };
@@ -197,7 +197,7 @@ extern "C" int32_t jam_rmsd_forward(const double *a, const double *b, const int6
     if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
     if (n_sets == 0) return JAM_OK;
     if (!a || !b || !off || !rmsd) return fail(JAM_EINVAL, "NULL argument");
-    const int tb = 256;
+    const int tb = RG_SETS;
     hipLaunchKernelGGL(jam::k_rmsd_fwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
                        (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, rot9);
     HIP_TRY(hipGetLastError());
@@ -211,7 +211,7 @@ extern "C" int32_t jam_rmsd_backward(const double *a, const double *b, const int
     if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
     if (n_sets == 0) return JAM_OK;
     if (!a || !b || !off || !rmsd || !grad_rmsd || (!grad_a && !grad_b)) return fail(JAM_EINVAL, "NULL argument");
-    const int tb = 256;
+    const int tb = RG_SETS;
     hipLaunchKernelGGL(jam::k_rmsd_bwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
                        (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, grad_rmsd, grad_a, grad_b);
     HIP_TRY(hipGetLastError());

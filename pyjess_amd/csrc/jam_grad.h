@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "jam_wave.h"
+
 namespace jam {
 
 // Optimal proper rotation R (row-major 3x3) for the correlation X.
@@ -155,22 +157,10 @@ __device__ static inline void centre_and_correlate(int n, const double *A, const
     }
 }
 
-// Set s owns points [off[s], off[s+1]) of a and b, 3 doubles per point.
-__global__ void __launch_bounds__(256) k_rmsd_fwd(const double *a, const double *b, const int64_t *off,
-                                                  long long n_sets, double *rmsd, double *rot)
+// One set: RMSD and optimal rotation (rot may be null).
+__device__ __attribute__((always_inline)) static inline void rmsd_one(int n, const double *A, const double *B,
+                                                                      double *rmsd, double *rot)
 {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_sets) return;
-    const int64_t p0 = off[s];
-    const int n = (int)(off[s + 1] - p0);
-    if (n <= 0) {
-        rmsd[s] = 0.0;
-        if (rot)
-            for (int q = 0; q < 9; q++) rot[9 * s + q] = (q % 4 == 0) ? 1.0 : 0.0;
-        return;
-    }
-    const double *A = a + 3 * p0;
-    const double *B = b + 3 * p0;
     double ca[3], cb[3], X[3][3], R[3][3];
     centre_and_correlate(n, A, B, ca, cb, X);
     optimal_rotation(X, R);
@@ -189,29 +179,20 @@ __global__ void __launch_bounds__(256) k_rmsd_fwd(const double *a, const double 
             e += r * r;
         }
     }
-    rmsd[s] = sqrt(e / (double)n);
+    *rmsd = sqrt(e / (double)n);
     if (rot)
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
-            for (int k = 0; k < 3; k++) rot[9 * s + 3 * j + k] = R[j][k];
+            for (int k = 0; k < 3; k++) rot[3 * j + k] = R[j][k];
 }
 
-__global__ void __launch_bounds__(256) k_rmsd_bwd(const double *a, const double *b, const int64_t *off,
-                                                  long long n_sets, const double *rmsd, const double *grad_rmsd,
-                                                  double *grad_a, double *grad_b)
+// One set: the gradient of g * rmsd, written to GA / GB (which may alias A /
+// B: every point is read before its gradient is written).
+__device__ __attribute__((always_inline)) static inline void rmsd_grad_one(int n, const double *A, const double *B,
+                                                                           double r, double g, double *GA,
+                                                                           double *GB)
 {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_sets) return;
-    const int64_t p0 = off[s];
-    const int n = (int)(off[s + 1] - p0);
-    if (n <= 0) return;
-    const double *A = a + 3 * p0;
-    const double *B = b + 3 * p0;
-    double *GA = grad_a ? grad_a + 3 * p0 : nullptr;
-    double *GB = grad_b ? grad_b + 3 * p0 : nullptr;
-    const double r = rmsd[s];
-    const double g = grad_rmsd[s];
     if (!(r > 0.0) || g == 0.0) {
         for (int q = 0; q < 3 * n; q++) {
             if (GA) GA[q] = 0.0;
@@ -230,13 +211,105 @@ __global__ void __launch_bounds__(256) k_rmsd_bwd(const double *a, const double 
             u[d] = A[3 * i + d] - ca[d];
             v[d] = B[3 * i + d] - cb[d];
         }
+        double ga[3], gb[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             // (R^T B_i)_d and (R A_i)_d
             const double rtb = R[0][d] * v[0] + R[1][d] * v[1] + R[2][d] * v[2];
             const double ra = R[d][0] * u[0] + R[d][1] * u[1] + R[d][2] * u[2];
-            if (GA) GA[3 * i + d] = scale * (u[d] - rtb);
-            if (GB) GB[3 * i + d] = scale * (v[d] - ra);
+            ga[d] = scale * (u[d] - rtb);
+            gb[d] = scale * (v[d] - ra);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (GA) GA[3 * i + d] = ga[d];
+            if (GB) GB[3 * i + d] = gb[d];
+        }
+    }
+}
+
+// One wave per tile of RG_SETS consecutive sets (one set per lane). When the
+// tile's points fit, they are staged in LDS with coalesced loads (and the
+// gradients leave through LDS with coalesced stores); otherwise each lane
+// reads its own set from global memory.
+#define RG_SETS 64
+#define RG_CAP 768   // points per tile held in LDS (12 per set on average)
+
+// Set s owns points [off[s], off[s+1]) of a and b, 3 doubles per point.
+__global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const double *b, const int64_t *off,
+                                                      long long n_sets, double *rmsd, double *rot)
+{
+    __shared__ double la[3 * RG_CAP], lb[3 * RG_CAP];
+    const int lane = threadIdx.x;
+    const long long s0 = (long long)blockIdx.x * RG_SETS;
+    if (s0 >= n_sets) return;
+    const long long s1 = s0 + RG_SETS < n_sets ? s0 + RG_SETS : n_sets;
+    const int64_t P0 = off[s0], P1 = off[s1];
+    const bool staged = P1 - P0 <= RG_CAP;
+    if (staged) {
+        const int m = (int)(3 * (P1 - P0));
+        for (int q = lane; q < m; q += RG_SETS) {
+            la[q] = a[3 * P0 + q];
+            lb[q] = b[3 * P0 + q];
+        }
+        wave_sync();
+    }
+    const long long s = s0 + lane;
+    if (s >= s1) return;
+    const int64_t p0 = off[s];
+    const int n = (int)(off[s + 1] - p0);
+    double *rt = rot ? rot + 9 * s : nullptr;
+    if (n <= 0) {
+        rmsd[s] = 0.0;
+        if (rt)
+            for (int q = 0; q < 9; q++) rt[q] = (q % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    double r;
+    if (staged) rmsd_one(n, la + 3 * (p0 - P0), lb + 3 * (p0 - P0), &r, rt);
+    else rmsd_one(n, a + 3 * p0, b + 3 * p0, &r, rt);
+    rmsd[s] = r;
+}
+
+__global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const double *b, const int64_t *off,
+                                                      long long n_sets, const double *rmsd, const double *grad_rmsd,
+                                                      double *grad_a, double *grad_b)
+{
+    __shared__ double la[3 * RG_CAP], lb[3 * RG_CAP];
+    const int lane = threadIdx.x;
+    const long long s0 = (long long)blockIdx.x * RG_SETS;
+    if (s0 >= n_sets) return;
+    const long long s1 = s0 + RG_SETS < n_sets ? s0 + RG_SETS : n_sets;
+    const int64_t P0 = off[s0], P1 = off[s1];
+    const bool staged = P1 - P0 <= RG_CAP;
+    const int m = (int)(3 * (P1 - P0));
+    if (staged) {
+        for (int q = lane; q < m; q += RG_SETS) {
+            la[q] = a[3 * P0 + q];
+            lb[q] = b[3 * P0 + q];
+        }
+        wave_sync();
+    }
+    const long long s = s0 + lane;
+    if (s < s1) {
+        const int64_t p0 = off[s];
+        const int n = (int)(off[s + 1] - p0);
+        if (n > 0) {
+            if (staged) {
+                // gradients replace the points in LDS
+                double *A = la + 3 * (p0 - P0), *B = lb + 3 * (p0 - P0);
+                rmsd_grad_one(n, A, B, rmsd[s], grad_rmsd[s], A, B);
+            } else {
+                rmsd_grad_one(n, a + 3 * p0, b + 3 * p0, rmsd[s], grad_rmsd[s], grad_a ? grad_a + 3 * p0 : nullptr,
+                              grad_b ? grad_b + 3 * p0 : nullptr);
+            }
+        }
+    }
+    if (staged) {
+        wave_sync();
+        for (int q = lane; q < m; q += RG_SETS) {
+            if (grad_a) grad_a[3 * P0 + q] = la[q];
+            if (grad_b) grad_b[3 * P0 + q] = lb[q];
         }
     }
 }

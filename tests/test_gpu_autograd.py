@@ -145,3 +145,27 @@ def test_host_tensors_are_refused():
     a = torch.zeros(2, 3, 3, dtype=torch.float64)
     with pytest.raises(TypeError):
         superposition_rmsd(a, a)
+
+
+def test_large_sets_take_the_unstaged_path():
+    from oracle.kabsch_torch import kabsch_rmsd_grad
+
+    # tiles of 64 sets with more than 768 points in all are read straight from
+    # global memory; mix them with small-set tiles that are staged in LDS
+    big_a, big_b, big_off = pairs(70, 11, k_min=20, k_max=40, collinear=0.0)
+    small_a, small_b, small_off = pairs(130, 12)
+    a = np.concatenate([big_a, small_a])
+    b = np.concatenate([big_b, small_b])
+    off = np.concatenate([big_off, small_off[1:] + big_off[-1]])
+    w = np.random.default_rng(3).uniform(-1.0, 1.0, size=len(off) - 1)
+    r, ga, gb = run(a, b, off, weights=w)
+    for s in range(len(off) - 1):
+        A, B = a[off[s]:off[s + 1]], b[off[s]:off[s + 1]]
+        # the SVD-based checker's |A|^2 + |B|^2 - 2 S loses digits on sets far
+        # from the origin: the RMSD is checked against its 50-digit value
+        want = exact_rmsd(A, B)
+        assert abs(r[s] - want) <= 1e-13 * max(1.0, want), s
+        _, ta, tb = kabsch_rmsd_grad(A, B)
+        x = np.concatenate([ga[off[s]:off[s + 1]], gb[off[s]:off[s + 1]]]).ravel()
+        y = w[s] * np.concatenate([ta, tb]).ravel()
+        assert np.linalg.norm(x - y) <= 1e-10 * np.linalg.norm(y), s
