@@ -56,6 +56,33 @@ def hit_digest(t, mi, atoms):
     return h
 
 
+def _mix64_np(x):
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def pair_digests(pair, atoms, sizes, n_templates, n_pairs):
+    """Per-pair hit counts and sums (mod 2^64) of hit_digest over the hits
+    (pair index, atom rows), vectorised over hits."""
+    pair = np.asarray(pair, dtype=np.int64)
+    t = pair % n_templates
+    mi = pair // n_templates
+    n = np.asarray(sizes, dtype=np.int64)[t]
+    with np.errstate(over="ignore"):
+        h = _mix64_np(((t.astype(np.uint64) & np.uint64(0xFFFFFFFF)) << np.uint64(32))
+                      ^ (mi.astype(np.uint64) & np.uint64(0xFFFFFFFF)))
+        for j in range(int(n.max()) if len(n) else 0):
+            sel = n > j
+            a = atoms[sel, j].astype(np.int64).astype(np.uint64) & np.uint64(0xFFFFFFFF)
+            h[sel] = _mix64_np(h[sel] ^ a)
+    counts = np.bincount(pair, minlength=n_pairs).astype(np.int64)
+    dg = np.zeros(n_pairs, dtype=np.uint64)
+    np.add.at(dg, pair, h)
+    return counts, dg
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -200,16 +227,9 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
         out["gpu_vs_cpu"] = value / cpu_rate
         log(f"{name}: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads ({reps} reps)")
         # parity of the hit set (before the best_match / max_candidates policies)
-        T = len(tpls)
-        gpu_dg = [0] * n_pairs
-        gpu_hc = np.zeros(n_pairs, dtype=np.int64)
-        for e in np.nonzero(res.flag == 1)[0]:
-            p = int(res.pair[e])
-            m, t = divmod(p, T)
-            gpu_dg[p] = (gpu_dg[p] + hit_digest(t, m, res.atoms[e, :sizes[t]])) & _M64
-            gpu_hc[p] += 1
-        parity = bool(np.array_equal(gpu_hc, hc) and
-                      all(int(a) == int(b) for a, b in zip(gpu_dg, dg)) and
+        sel = res.flag == 1
+        gpu_hc, gpu_dg = pair_digests(res.pair[sel], res.atoms[sel], sizes, len(tpls), n_pairs)
+        parity = bool(np.array_equal(gpu_hc, hc) and np.array_equal(gpu_dg, dg.astype(np.uint64)) and
                       np.array_equal(res.pair_matches, cm))
         out["hit_set_parity"] = parity
         log(f"{name}: hit-set parity with the CPU oracle: {parity}")
@@ -237,6 +257,136 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
                     out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
         flops_all = flops_join + (27.0 * work["complete_atoms"] + C_EIG * work["complete"]) * f
         out["flops_alg"] = flops_all
+    return out
+
+
+def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
+    """Library-scale batched search: 64 assemblies of 100,000 atoms x 7,000
+    templates, streamed through jam_query in chunks of ``chunk`` assemblies
+    (one batched call each, every (template, molecule) pair of the chunk).
+
+    ``value`` counts the search with inputs and the resulting events resident
+    in HBM; ``with_event_copy`` adds the copy of every event to host memory
+    (pageable numpy arrays, over PCIe); ``drop_in`` is ``Jess.query_batch``
+    (Python hit objects) on the first chunk. The CPU baseline is the C
+    restatement on the first assembly x all templates, and the hit set of that
+    assembly is checked against it."""
+    import ctypes
+
+    from pyjess_amd import Jess, synthetic, _native
+
+    t0 = time.time()
+    mols, tpls, kw = synthetic.workload("library-64", seed=seed, scale=scale)
+    n_atoms = sum(len(m) for m in mols)
+    T = len(tpls)
+    n_pairs = len(mols) * T
+    log(f"library: {len(mols)} assemblies ({n_atoms} atoms) x {T} templates, generated in {time.time() - t0:.1f}s")
+    qargs = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    jess = Jess(tpls)
+    ctx = _native.context()
+    L = _native.lib()
+    tset = jess._device_templates(ctx)
+    chunks = [mols[i:i + chunk] for i in range(0, len(mols), chunk)]
+    msets = [_native.MoleculeSet(ctx, c) for c in chunks]
+    params = _native.Params(*qargs, 0, 1)
+
+    def device_pass():
+        n_ev = n_cm = 0
+        stage = np.zeros(5)
+        for ms in msets:
+            h = ctypes.c_void_p()
+            with ctx.lock:
+                _native._check(L.jam_query(ctx.handle, tset.handle, ms.handle, ctypes.byref(params),
+                                           ctypes.byref(h)))
+                try:
+                    n_ev += L.jam_result_n_events(h)
+                    t = np.zeros(5)
+                    _native._check(L.jam_result_timings(h, _native._ptr(t)))
+                    stage += t
+                finally:
+                    L.jam_result_free(h)
+        return n_ev, stage
+
+    for _ in range(warmup):
+        device_pass()
+    _native.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n_events, stage = device_pass()
+    _native.synchronize()
+    dev_s = (time.perf_counter() - t0) / steps
+    value = n_pairs / dev_s
+    log(f"library: device-resident {dev_s * 1000:.1f} ms/step, {value:,.0f} queries/s, {n_events} events; "
+        f"device ms: candidates {stage[0]:.1f} kd {stage[1]:.1f} join {stage[2]:.1f} super {stage[3]:.1f} "
+        f"total {stage[4]:.1f}")
+
+    # the same with every event copied to host memory
+    t0 = time.perf_counter()
+    first = None
+    for ms in msets:
+        r = _native.query(tset, ms, *qargs, 0, True)
+        if first is None:
+            first = r
+        del r
+    copy_s = time.perf_counter() - t0
+    log(f"library: with the event copy {copy_s * 1000:.1f} ms, {n_pairs / copy_s:,.0f} queries/s")
+
+    # the drop-in batched API on the first chunk (Python hit objects)
+    t0 = time.perf_counter()
+    hits = jess.query_batch(chunks[0], *qargs, max_candidates=kw["max_candidates"])
+    drop_s = time.perf_counter() - t0
+    n_hits0 = sum(len(h) for h in hits)
+    del hits
+    log(f"library: query_batch on {len(chunks[0])} assemblies {drop_s * 1000:.1f} ms "
+        f"({len(chunks[0]) * T / drop_s:,.0f} queries/s, {n_hits0} hits)")
+
+    out = {
+        "value": value,
+        "unit": "queries/s",
+        "ms_per_step": dev_s * 1000.0,
+        "steps": steps,
+        "config": {"workload": "library-64", "molecules": len(mols), "atoms": n_atoms, "templates": T,
+                   "pairs": n_pairs, "chunk": chunk, "seed": seed, "rmsd_threshold": kw["rmsd_threshold"],
+                   "distance_cutoff": kw["distance_cutoff"], "max_dynamic_distance": kw["max_dynamic_distance"],
+                   "max_candidates": kw["max_candidates"]},
+        "events": int(n_events),
+        "device_ms": {"candidates": stage[0], "kd_order": stage[1], "join": stage[2], "superposition": stage[3],
+                      "total": stage[4]},
+        "with_event_copy": {"value": n_pairs / copy_s, "unit": "queries/s", "seconds": copy_s},
+        "drop_in": {"value": len(chunks[0]) * T / drop_s, "unit": "queries/s", "molecules": len(chunks[0]),
+                    "hits": n_hits0, "seconds": drop_s},
+        "cpu_baseline": None,
+        "hit_set_parity": None,
+    }
+    if cpu:
+        from oracle import oracle
+
+        threads = cpu_share()
+        mv = [oracle.MolView.of(mols[0])]
+        tv = oracle.TplView(tpls)
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            cm, hc, dg = oracle.batch(mv, tv, *qargs, n_threads=threads)
+            reps += 1
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        cpu_elapsed = time.perf_counter() - t0
+        cpu_rate = reps * T / cpu_elapsed
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"the first assembly ({len(mols[0])} atoms) x all {T} templates, run {reps} times in "
+                      f"{cpu_elapsed:.1f}s on {threads} threads (C restatement of the reference algorithm, "
+                      f"gcc -O3)"}
+        out["gpu_vs_cpu"] = value / cpu_rate
+        sizes = [len(t) for t in tpls]
+        sel = (first.flag == 1) & (first.pair < T)
+        gpu_hc, gpu_dg = pair_digests(first.pair[sel], first.atoms[sel], sizes, T, T)
+        parity = bool(np.array_equal(gpu_hc, hc) and np.array_equal(gpu_dg, dg.astype(np.uint64)) and
+                      np.array_equal(first.pair_matches[:T], cm))
+        out["hit_set_parity"] = parity
+        log(f"library: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads; hit-set parity on the "
+            f"first assembly: {parity}")
     return out
 
 
@@ -377,7 +527,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="plain-8k", help="plain-2k | plain-8k | batch-4k | rmsd-grad-1m")
+    ap.add_argument("--workload", default="plain-8k",
+                    help="plain-2k | plain-8k | batch-4k | rmsd-grad-1m | library-64")
+    ap.add_argument("--library-chunk", type=int, default=8, help="assemblies per batched call of library-64")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch workload's molecules")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of each CPU baseline")
@@ -402,6 +554,16 @@ def main():
                "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                "dtype": "f64", "data": "synthetic", "config": dict(d["config"], parallelism="1 GPU"),
                "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"], "differentiation": d}
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "library-64":
+        d = measure_library(args.scale, args.seed, args.library_chunk, args.steps, args.warmup, args.cpu_seconds,
+                            not args.no_cpu)
+        out = {"metric": METRIC, "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": dict(d["config"], parallelism="1 GPU"), "roofline": None,
+               "cpu_baseline": d["cpu_baseline"], "library": d}
         print(json.dumps(out), flush=True)
         return
     main_run = measure(args.workload, args.scale, args.seed, args.steps, args.warmup, args.cpu_seconds,

@@ -27,6 +27,7 @@ extern "C" {
 #define JAM_EMODE (-3)      /* template uses a match mode this build does not implement */
 #define JAM_EDEVICE (-4)    /* HIP runtime error or no usable gfx950 device */
 #define JAM_ETOOBIG (-5)    /* template larger than JAM_MAX_TEMPLATE_ATOMS */
+#define JAM_ESTALE (-6)     /* a result's events were overwritten by a later query */
 
 #define JAM_MAX_TEMPLATE_ATOMS 32
 
@@ -118,8 +119,11 @@ void jam_molecules_free(jam_molecules *m);
  * matches whose rotation has a NaN, which pyjess skips with a warning), each
  * with its pair, its rank in the reference enumeration order of that pair,
  * the matched atoms (molecule-local indices, template order), RMSD, rotation
- * and centroids. Query.__next__'s policies (best_match, max_candidates) are
- * applied by the caller over events sorted by (pair scan order, rank). */
+ * and centroids. The events are ordered as pyjess reports them: by molecule,
+ * then template in scan order, then rank; Query.__next__'s policies
+ * (best_match, max_candidates) are applied by the caller in one pass over
+ * them. They stay in device memory, owned by the context, until the next
+ * jam_query on that context: jam_result_copy fetches them before then. */
 int32_t jam_query(jam_context *ctx, const jam_templates *t, const jam_molecules *m,
                   const jam_params *params, jam_result **out);
 
@@ -128,8 +132,11 @@ int32_t jam_query(jam_context *ctx, const jam_templates *t, const jam_molecules 
 int64_t jam_result_n_events(const jam_result *r);
 int32_t jam_result_stride(const jam_result *r);   /* atoms per event row */
 int32_t jam_result_n_pairs(const jam_result *r);  /* n_molecules * n_templates */
-/* Copies the events out. Any pointer may be NULL to skip that field.
- * pair = molecule * n_templates + template. flag: 1 hit, 2 NaN rotation. */
+/* Copies the events from device memory into the caller's host buffers
+ * (n_events rows each). Any pointer may be NULL to skip that field.
+ * pair = molecule * n_templates + template. flag: 1 hit, 2 NaN rotation.
+ * JAM_ESTALE once another jam_query has run on the result's context; the
+ * result must not outlive its context. */
 int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
                         double *rmsd, double *rotation9, double *c0_3, double *c1_3,
                         int32_t *atoms_stride);

@@ -193,6 +193,63 @@ def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const i
         (<list>out[mi]).append(h)
 
 
+def select_events(const int[:] pair, const long long[:] rank, const int[:] flag, const double[:] rmsd,
+                  bint best_match, long long mc):
+    """Indices of the events that ``Query.__next__`` reaches
+    (``_jess.pyx:2238-2276``), in order: the hits it yields and the
+    NaN-rotation matches it warns about.
+
+    The events arrive grouped by pair in report order, ranks increasing
+    within a pair. The candidate counter restarts at every template and
+    after every yielded hit; when it exceeds ``mc`` (-1: no limit) the rest of
+    the template is skipped after the current match. With ``best_match``,
+    the first match of least RMSD (strict ``<``) is kept per template."""
+    cdef Py_ssize_t n = pair.shape[0], i = 0, j, e, w = 0, best
+    out = np.empty(max(n, 1), dtype=np.int64)
+    cdef long long[:] o = out
+    cdef int p
+    cdef long long rk, c, seg
+    cdef double best_rmsd
+    while i < n:
+        p = pair[i]
+        j = i
+        while j < n and pair[j] == p:
+            j += 1
+        if best_match:
+            best = -1
+            best_rmsd = math.inf
+            for e in range(i, j):
+                rk = rank[e]
+                if mc != -1 and rk > mc + 1:
+                    break
+                if rmsd[e] < best_rmsd:
+                    if flag[e] == 2:
+                        o[w] = e        # warning only
+                        w += 1
+                    else:
+                        best = e
+                        best_rmsd = rmsd[e]
+                if mc != -1 and rk == mc + 1:
+                    break
+            if best >= 0:
+                o[w] = best
+                w += 1
+        else:
+            seg = 0
+            for e in range(i, j):
+                c = rank[e] - seg
+                if mc != -1 and c > mc + 1:
+                    break
+                o[w] = e
+                w += 1
+                if flag[e] != 2:
+                    seg = rank[e] + 1
+                if mc != -1 and c == mc + 1:
+                    break
+        i = j
+    return out[:w]
+
+
 cdef class HitIterator:
     """Iteration of `Query` over its hits in pyjess order; NaN-rotation
     entries (None) become warnings, as in ``Query.__next__``

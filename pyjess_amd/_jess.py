@@ -14,7 +14,6 @@ from __future__ import annotations
 
 import functools
 import io
-import math
 import warnings
 from typing import Iterable, List, Optional, Sequence
 
@@ -34,6 +33,7 @@ def _warn_nan():
 
 
 from ._hits import Block as _Block, Hit, HitIterator as _HitIterator, build_hits as _build_hits  # noqa: E402
+from ._hits import select_events as _select_events  # noqa: E402
 
 
 class Query(_HitIterator):
@@ -95,87 +95,39 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
             max_candidates: int) -> List[list]:
     """Hits per molecule, in pyjess order, from the device events.
 
-    Templates are scanned in reverse insertion order (``Jess_addTemplate``
-    prepends, ``patches/Jess.c.patch:63-72``). Within a template the events are
-    taken by rank, the position of the complete match in the reference
-    enumeration. The candidate counter of ``Query.__next__`` restarts at every
-    template and after every yielded hit; when it exceeds ``max_candidates``
-    the rest of the template is skipped (after the current match, which is
-    still reported if it is a hit). NaN rotations produce ``None`` entries,
-    turned into warnings when iterated.
+    The device delivers the events in report order: by molecule, then
+    template in scan order (reverse insertion order: ``Jess_addTemplate``
+    prepends, ``patches/Jess.c.patch:63-72``), then rank, the position of the
+    complete match in the reference enumeration. The candidate counter of
+    ``Query.__next__`` restarts at every template and after every yielded
+    hit; when it exceeds ``max_candidates`` the rest of the template is
+    skipped (after the current match, which is still reported if it is a
+    hit). NaN rotations produce ``None`` entries, turned into warnings when
+    iterated.
     """
     T = len(jess)
     out: List[list] = [[] for _ in molecules]
-    if len(res.rmsd) == 0:
+    n = len(res.rmsd)
+    if n == 0:
         return out
     mc = max_candidates
-    mol_of = res.pair // T
-    tpl_of = res.pair % T
-    # molecule, then scan position of the template (reverse order), then rank
-    order = np.lexsort((res.rank, (T - 1) - tpl_of, mol_of))
+    pair = np.ascontiguousarray(res.pair, dtype=np.int32)
     # the candidate counter of a template never exceeds its number of complete
     # matches, so a cap at or above the largest one never cuts anything
     if mc != -1 and len(res.pair_matches) and mc >= int(res.pair_matches.max()):
         mc = -1
     if not best_match and mc == -1:
         # no policy can drop an event: every event is reported, in order
-        chosen = order
+        chosen = np.arange(n, dtype=np.int64)
     else:
-        chosen = _select(res, order, best_match, mc)
+        chosen = _select_events(pair, np.ascontiguousarray(res.rank, dtype=np.int64),
+                                np.ascontiguousarray(res.flag, dtype=np.int32),
+                                np.ascontiguousarray(res.rmsd, dtype=np.float64), best_match, mc)
     block = _Block(res.rotation, res.c0, res.c1, res.atoms)
-    _build_hits(out, np.asarray(chosen, dtype=np.int64), mol_of.astype(np.int32), tpl_of.astype(np.int32),
+    _build_hits(out, chosen, (pair // T).astype(np.int32), (pair % T).astype(np.int32),
                 np.ascontiguousarray(res.flag, dtype=np.int32), np.ascontiguousarray(res.rmsd, dtype=np.float64),
                 block, list(jess._templates), list(molecules))
     return out
-
-
-def _select(res, order, best_match: bool, mc: int) -> list:
-    """Events that pyjess reaches under ``best_match`` / ``max_candidates``, in
-    order: hits it yields and NaN-rotation matches it warns about."""
-    T_pairs = res.pair
-    ranks = res.rank.tolist()
-    flags = res.flag.tolist()
-    rmsds = res.rmsd.tolist()
-    pairs = T_pairs.tolist()
-    chosen = []
-    n = len(order)
-    i = 0
-    order = order.tolist()
-    while i < n:
-        p = pairs[order[i]]
-        j = i
-        while j < n and pairs[order[j]] == p:
-            j += 1
-        events = order[i:j]
-        i = j
-        if best_match:
-            best = None
-            best_rmsd = math.inf
-            for e in events:
-                rank = ranks[e]
-                if mc != -1 and rank > mc + 1:
-                    break
-                if rmsds[e] < best_rmsd:
-                    if flags[e] == 2:
-                        chosen.append(e)            # warning only
-                    else:
-                        best, best_rmsd = e, rmsds[e]
-                if mc != -1 and rank == mc + 1:
-                    break
-            if best is not None:
-                chosen.append(best)
-        else:
-            seg = 0
-            for e in events:
-                c = ranks[e] - seg
-                if mc != -1 and c > mc + 1:
-                    break
-                chosen.append(e)
-                if flags[e] != 2:
-                    seg = ranks[e] + 1
-                if mc != -1 and c == mc + 1:
-                    break
-    return chosen
 
 
 class Jess:

@@ -23,6 +23,7 @@ JAM_EINVAL = -2
 JAM_EMODE = -3
 JAM_EDEVICE = -4
 JAM_ETOOBIG = -5
+JAM_ESTALE = -6
 MAX_TEMPLATE_ATOMS = 32
 
 _SYMBOLS = (
@@ -352,10 +353,11 @@ def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_
     h = ctypes.c_void_p()
     with ctx.lock:
         _check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h)))
-    try:
-        return Result(h)
-    finally:
-        L.jam_result_free(h)
+        try:
+            # the events live in the context's device buffers until the next query
+            return Result(h)
+        finally:
+            L.jam_result_free(h)
 
 
 def superpose_batch(a: np.ndarray, b: np.ndarray, device: int = 0):

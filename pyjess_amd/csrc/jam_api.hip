@@ -119,6 +119,9 @@ struct jam_context {
     int nsets_cap = 0, npairs_cap = 0;
     int64_t items_cap = 0;
     int event_stride = 0;
+    // incremented by every query: a result's events live in e_* until the
+    // next query on the context
+    uint64_t query_gen = 0;
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
 };
@@ -178,9 +181,11 @@ struct jam_molecules {
 struct jam_result {
     int n_pairs = 0;
     int stride = 1;
-    std::vector<int32_t> pair, flag, atoms;
-    std::vector<int64_t> rank, pair_cm;
-    std::vector<double> rmsd, rot, c0, c1;
+    // the events stay in the context's device buffers (copied out on demand)
+    jam_context *ctx = nullptr;
+    uint64_t gen = 0;
+    int64_t n_events = 0;
+    std::vector<int64_t> pair_cm;
     double ms[5] = {0, 0, 0, 0, 0};
     double expand_ms = 0.0;     // summed duration of the k_join launches
     int64_t level_partials[JAM_MAX_LEVELS] = {};   // partial matches expanded per join level
@@ -679,6 +684,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     *out = nullptr;
     HIP_TRY(hipSetDevice(c->device));
     std::unique_ptr<jam_result> R(new jam_result());
+    R->ctx = c;
+    R->gen = ++c->query_gen;
     const int T = tp->n_templates, M = ms->n_mols, U = tp->n_types;
     const int64_t n_pairs64 = (int64_t)T * M;
     if (n_pairs64 > INT32_MAX / 64) return fail(JAM_ETOOBIG, "too many (template, molecule) pairs");
@@ -897,6 +904,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         // the level positions (`bits` bits each, `per` levels per 64-bit key),
         // the last levels first, then the pair
         if ((rc = grow_sort(c, n_done)) != JAM_OK) return rc;
+        if ((rc = grow_events(c, n_done, stride)) != JAM_OK) return rc;
+        size_t scan_bytes = 0;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int *)nullptr, (int *)nullptr,
+                                                 (int)std::max<int64_t>(n_done, 1), s));
+        HIP_TRY(c->s_tmp.reserve(scan_bytes));
         int cur = 0;
         const unsigned gs = (unsigned)((n_done + 255) / 256);
         if (n_done > 0) {
@@ -917,7 +929,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                     c->s_perm[cur].as<int>(), c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, cnt_l * bits, s));
                 cur ^= 1;
             }
-            hipLaunchKernelGGL(jam::k_pairkey, dim3(gs), dim3(256), 0, s, jn, c->s_perm[cur].as<const int>(),
+            hipLaunchKernelGGL(jam::k_pairkey, dim3(gs), dim3(256), 0, s, jn, T, c->s_perm[cur].as<const int>(),
                                (long long)n_done, c->s_pkey[0].as<unsigned int>());
             int pbits = 1;
             while ((1ll << pbits) < (long long)n_pairs) pbits++;
@@ -929,7 +941,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         }
         const int *perm = c->s_perm[cur].as<const int>();
 
-        // ---- superposition of every complete match, events
+        // ---- superposition of every complete match; events in that order
         DevEvents ev;
         ev.cap = c->event_cap;
         ev.stride = stride;
@@ -942,22 +954,39 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         ev.c1 = c->e_c1.as<double>();
         ev.atoms = c->e_atoms.as<int>();
         ev.count = c->e_count.as<unsigned long long>();
+        // marks and their scan reuse the (now free) 64-bit sort keys
+        int *sel = c->s_key[0].as<int>();
+        int *pos = c->s_key[1].as<int>();
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ev.count, 1ll);
         if (n_done > 0) {
+            const long long *first = c->p_first.as<long long>();
             hipLaunchKernelGGL(jam::k_first, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
                                c->p_first.as<long long>());
-            hipLaunchKernelGGL(jam::k_super, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
-                               (long long)n_done, (const long long *)c->p_first.as<long long>());
+            hipLaunchKernelGGL(jam::k_super<false>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
+                               (long long)n_done, first, sel, (const int *)pos);
+            size_t tb = c->s_tmp.bytes;
+            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->s_tmp.ptr, tb, (const int *)sel, pos, (int)n_done, s));
+            hipLaunchKernelGGL(jam::k_event_total, dim3(1), dim3(64), 0, s, (const int *)sel, (const int *)pos,
+                               (long long)n_done, ev.count);
+            hipLaunchKernelGGL(jam::k_super<true>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
+                               (long long)n_done, first, sel, (const int *)pos);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[4], s));
         unsigned long long ne_dev = 0;
         HIP_TRY(hipMemcpyAsync(&ne_dev, ev.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if ((int64_t)ne_dev > c->event_cap) {
-            if ((rc = grow_events(c, (int64_t)ne_dev, stride)) != JAM_OK) return rc;
-            continue;
+        if (n_done > 0) {
+            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min(1024, (n_pairs + 255) / 256)), dim3(256), 0, s,
+                               (unsigned long long *)c->p_cm.ptr, (long long)n_pairs);
+            hipLaunchKernelGGL(jam::k_pair_matches, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
+                               (const long long *)c->p_first.as<long long>(), c->p_cm.as<long long>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(R->pair_cm.data(), c->p_cm.ptr, sizeof(long long) * n_pairs,
+                                   hipMemcpyDeviceToHost, s));
         }
+        HIP_TRY(hipStreamSynchronize(s));
+        if ((int64_t)ne_dev > c->event_cap || (int64_t)ne_dev > n_done)
+            return fail(JAM_EDEVICE, "event count exceeds the complete matches");
         float ms_k[5] = {0, 0, 0, 0, 0};
         HIP_TRY(hipEventElapsedTime(&ms_k[0], c->ev[0], c->ev[1]));
         HIP_TRY(hipEventElapsedTime(&ms_k[1], c->ev[1], c->ev[2]));
@@ -975,34 +1004,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         R->expand_launches = c->profile ? n_launched : 0;
         for (int q = 0; q < JAM_MAX_LEVELS; q++) R->level_partials[q] = 0;
         for (int L = 1; L < JAM_MAX_LEVELS && L <= n_launched; L++) R->level_partials[L] = (int64_t)jc[L - 1];
-        const int64_t ne = (int64_t)ne_dev;
-        R->pair.resize(ne);
-        R->rank.resize(ne);
-        R->flag.resize(ne);
-        R->rmsd.resize(ne);
-        R->rot.resize(9 * ne);
-        R->c0.resize(3 * ne);
-        R->c1.resize(3 * ne);
-        R->atoms.resize(ne * stride);
-        if (ne > 0) {
-            HIP_TRY(hipMemcpyAsync(R->pair.data(), ev.pair, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->rank.data(), ev.rank, sizeof(long long) * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->flag.data(), ev.flag, sizeof(int) * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->rmsd.data(), ev.rmsd, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->rot.data(), ev.rot, sizeof(double) * 9 * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->c0.data(), ev.c0, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->c1.data(), ev.c1, sizeof(double) * 3 * ne, hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipMemcpyAsync(R->atoms.data(), ev.atoms, sizeof(int) * ne * stride, hipMemcpyDeviceToHost, s));
-        }
-        if (n_done > 0) {
-            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min(1024, (n_pairs + 255) / 256)), dim3(256), 0, s,
-                               (unsigned long long *)c->p_cm.ptr, (long long)n_pairs);
-            hipLaunchKernelGGL(jam::k_pair_matches, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
-                               (const long long *)c->p_first.as<long long>(), c->p_cm.as<long long>());
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(R->pair_cm.data(), c->p_cm.ptr, sizeof(long long) * n_pairs,
-                                   hipMemcpyDeviceToHost, s));
-        }
+        R->n_events = (int64_t)ne_dev;
         HIP_TRY(hipStreamSynchronize(s));
         *out = R.release();
         return JAM_OK;
@@ -1014,7 +1016,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
 // results
 // ---------------------------------------------------------------------------
 
-extern "C" int64_t jam_result_n_events(const jam_result *r) { return r ? (int64_t)r->rmsd.size() : 0; }
+extern "C" int64_t jam_result_n_events(const jam_result *r) { return r ? r->n_events : 0; }
 
 extern "C" int32_t jam_result_stride(const jam_result *r) { return r ? r->stride : 0; }
 
@@ -1024,15 +1026,25 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
                                    double *rmsd, double *rot, double *c0, double *c1, int32_t *atoms)
 {
     if (!r) return fail(JAM_EINVAL, "NULL result");
-    const size_t n = r->rmsd.size();
-    if (pair) std::memcpy(pair, r->pair.data(), sizeof(int32_t) * n);
-    if (rank) std::memcpy(rank, r->rank.data(), sizeof(int64_t) * n);
-    if (flag) std::memcpy(flag, r->flag.data(), sizeof(int32_t) * n);
-    if (rmsd) std::memcpy(rmsd, r->rmsd.data(), sizeof(double) * n);
-    if (rot) std::memcpy(rot, r->rot.data(), sizeof(double) * 9 * n);
-    if (c0) std::memcpy(c0, r->c0.data(), sizeof(double) * 3 * n);
-    if (c1) std::memcpy(c1, r->c1.data(), sizeof(double) * 3 * n);
-    if (atoms) std::memcpy(atoms, r->atoms.data(), sizeof(int32_t) * r->atoms.size());
+    const size_t n = (size_t)r->n_events;
+    if (n == 0) return JAM_OK;
+    jam_context *c = r->ctx;
+    if (!c || c->query_gen != r->gen)
+        return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    auto get = [&](void *dst, const DBuf &src, size_t bytes) -> hipError_t {
+        return dst ? hipMemcpyAsync(dst, src.ptr, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    HIP_TRY(get(pair, c->e_pair, sizeof(int32_t) * n));
+    HIP_TRY(get(rank, c->e_rank, sizeof(int64_t) * n));
+    HIP_TRY(get(flag, c->e_flag, sizeof(int32_t) * n));
+    HIP_TRY(get(rmsd, c->e_rmsd, sizeof(double) * n));
+    HIP_TRY(get(rot, c->e_rot, sizeof(double) * 9 * n));
+    HIP_TRY(get(c0, c->e_c0, sizeof(double) * 3 * n));
+    HIP_TRY(get(c1, c->e_c1, sizeof(double) * 3 * n));
+    HIP_TRY(get(atoms, c->e_atoms, sizeof(int32_t) * (size_t)r->stride * n));
+    HIP_TRY(hipStreamSynchronize(s));
     return JAM_OK;
 }
 

@@ -15,10 +15,12 @@
 //                  of the kd-ordered set whose box meets the join box, all 64
 //                  lanes testing one candidate each against every annulus and
 //                  check(); children appended in any order.
-//   k_sortkey      complete matches sorted by (pair, level positions): the
-//                  reference's depth-first order and each match's rank.
-//   k_super        thread per complete match: FP64 superposition, RMSD
-//                  threshold, event compaction with the match's rank.
+//   k_sortkey      complete matches sorted by (molecule, template in scan
+//                  order, level positions): the reference's depth-first
+//                  order and each match's rank.
+//   k_super        thread per complete match: FP64 superposition and RMSD
+//                  threshold, then (after a scan of the marks) the events in
+//                  that same order, each with the match's rank.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -511,10 +513,18 @@ __global__ void __launch_bounds__(256) k_sortkey(DevTpls tp, DevPairs pr, DevJoi
     key[s] = k;
 }
 
-__global__ void __launch_bounds__(256) k_pairkey(DevJoin jn, const int *perm, long long n, unsigned int *key)
+// Last sort key: the molecule, then the template in scan order (Jess scans
+// templates in reverse insertion order, patches/Jess.c.patch:63-72), so the
+// complete matches end up grouped by pair in the order the caller reports
+// them.
+__global__ void __launch_bounds__(256) k_pairkey(DevJoin jn, int n_templates, const int *perm, long long n,
+                                                 unsigned int *key)
 {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) key[s] = (unsigned int)jn.done_pair[perm[s]];
+    if (s >= n) return;
+    const int p = jn.done_pair[perm[s]];
+    const int t = p % n_templates;
+    key[s] = (unsigned int)(p - t + (n_templates - 1 - t));
 }
 
 // first sorted position of every pair with complete matches
@@ -540,14 +550,14 @@ __global__ void __launch_bounds__(256) k_pair_matches(DevJoin jn, const int *per
 // superposition + events
 // ---------------------------------------------------------------------------
 
-// Thread per complete match, in reference order: superposition, RMSD
-// threshold, event with the match's rank within its pair.
-__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevJoin jn,
-                                               DevEvents ev, QueryParams qp, const int *perm, long long n_matches,
-                                               const long long *first)
+// Superposition of complete match s (sorted position): pair, molecule-local
+// atoms in template order, RMSD, rotation, centroids.
+__device__ __attribute__((always_inline)) static inline int super_of(DevMols ms, DevTpls tp, DevSets st,
+                                                                     DevPairs pr, DevJoin jn, const int *perm,
+                                                                     long long s, int atoms[JAM_MAX_LEVELS],
+                                                                     int *n_out, double *rmsd, double rot[9],
+                                                                     double c0[3], double c1[3])
 {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_matches) return;
     const int i = perm[s];
     const int pair = jn.done_pair[i];
     const int T = tp.n_templates;
@@ -558,18 +568,41 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets s
     const unsigned char *ord = pr.ord + (size_t)pair * JAM_MAX_LEVELS;
     const int *row = jn.done_ent + (long long)i * jn.stride;
     // molecule-local atom of template atom k: placed at the level l with ord[l] == k
-    int atoms[JAM_MAX_LEVELS];
     for (int l = 0; l < n; l++) atoms[ord[l]] = (l == 0) ? st.list[row[0]] : st.katom[row[l]];
     auto pa = [&](int k, int d) { return ms.xyz[3 * (size_t)(abase + atoms[k]) + d]; };
     auto pb = [&](int k, int d) { return tp.pos[3 * (size_t)(tbase + k) + d]; };
+    superpose(n, pa, pb, rmsd, rot, c0, c1);
+    *n_out = n;
+    return pair;
+}
+
+// Thread per complete match, in reference order. Pass 1 (EMIT = false)
+// marks the matches within the RMSD threshold (sel[s] = 1); after an
+// exclusive scan of sel into pos, pass 2 (EMIT = true) recomputes the
+// superposition of the marked ones and writes their events at pos[s], so the
+// events keep the order of the sorted matches without a sort of their own.
+// Each event carries the match's rank within its pair.
+template <bool EMIT>
+__global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevJoin jn,
+                                               DevEvents ev, QueryParams qp, const int *perm, long long n_matches,
+                                               const long long *first, int *sel, const int *pos)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_matches) return;
+    if (EMIT && !sel[s]) return;
+    int atoms[JAM_MAX_LEVELS];
+    int n;
     double rmsd, rot[9], c0[3], c1[3];
-    superpose(n, pa, pb, &rmsd, rot, c0, c1);
-    if (!(rmsd <= qp.rmsd_threshold)) return;
+    const int pair = super_of(ms, tp, st, pr, jn, perm, s, atoms, &n, &rmsd, rot, c0, c1);
+    if (!EMIT) {
+        sel[s] = (rmsd <= qp.rmsd_threshold) ? 1 : 0;
+        return;
+    }
     bool nan = false;
 #pragma unroll
     for (int k = 0; k < 9; k++) nan |= isnan(rot[k]);
-    const unsigned long long e = atomicAdd(ev.count, 1ull);
-    if (e >= (unsigned long long)ev.cap) return;
+    const long long e = pos[s];
+    if (e >= ev.cap) return;
     ev.pair[e] = pair;
     ev.rank[e] = s - first[pair];
     ev.flag[e] = nan ? 2 : 1;
@@ -582,6 +615,12 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets s
         ev.c1[3 * e + d] = c1[d];
     }
     for (int k = 0; k < ev.stride; k++) ev.atoms[e * (size_t)ev.stride + k] = k < n ? atoms[k] : -1;
+}
+
+// number of events: the scan's last position plus the last mark
+__global__ void k_event_total(const int *sel, const int *pos, long long n, unsigned long long *count)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = n > 0 ? (unsigned long long)(pos[n - 1] + sel[n - 1]) : 0ull;
 }
 
 // Standalone batched superposition: sets of k points, a/b as [set][point][xyz].

@@ -61,6 +61,8 @@ _PARENT = {
 SYMMETRIC = {("ASP", "OD1"), ("ASP", "OD2"), ("GLU", "OE1"), ("GLU", "OE2"), ("ARG", "NH1"),
              ("ARG", "NH2"), ("ASN", "OD1"), ("ASN", "ND2"), ("GLN", "OE1"), ("GLN", "NE2")}
 
+_SYM_TYPES = sorted({rn for rn, _ in SYMMETRIC})
+
 CHAIN_IDS = "ABCDEFGHIJKLMNOPQRSTUVWXYZ"
 
 
@@ -234,9 +236,16 @@ def _residue_starts(mol: Molecule) -> np.ndarray:
 
 def planted_template(rng: np.random.Generator, mol: Molecule, n_residues: int, atoms_per_residue,
                      jitter: float = 0.3, radius: float = 10.0, p_flexible: float = 0.3,
-                     tpl_id: str = "PLNT", starts: Optional[np.ndarray] = None) -> Template:
+                     tpl_id: str = "PLNT", starts: Optional[np.ndarray] = None,
+                     p_symmetric: float = 0.0) -> Template:
     """Atoms of ``n_residues`` residues whose CA lie within ``radius`` of a random
-    seed residue's CA, copied from ``mol`` with N(0, jitter) noise."""
+    seed residue's CA, copied from ``mol`` with N(0, jitter) noise.
+
+    With ``p_symmetric`` > 0, that fraction of the residues (on average) carry
+    both atoms of a symmetric pair (carboxylate O, amide O/N, guanidinium N) in
+    match mode 1, so each matches either atom of the pair: a residue of a type
+    that has such a pair is made symmetric with probability
+    ``p_symmetric / 0.25`` (5 of the 20 types have one)."""
     if starts is None:
         starts = _residue_starts(mol)
     ca_idx = np.array([s + 1 for s in starts])  # N, CA, ...: CA is the 2nd atom
@@ -250,50 +259,74 @@ def planted_template(rng: np.random.Generator, mol: Molecule, n_residues: int, a
             break
     chosen = [seed] + list(rng.choice(near, size=min(len(near), n_residues - 1), replace=False))
     atoms = []
+    p_sym_type = min(1.0, p_symmetric / 0.25)
     for r in chosen:
         res_atoms = _residue_atoms(mol, int(starts[r]))
         k = int(atoms_per_residue() if callable(atoms_per_residue) else atoms_per_residue)
-        # side-chain atoms first (they carry the chemistry), then backbone
-        side = [a for a in res_atoms if bytes(mol.names[a]) not in (b"_N__", b"_C__", b"_O__")]
-        pool = side if len(side) >= k else res_atoms
-        pick = sorted(rng.choice(len(pool), size=min(k, len(pool)), replace=False))
-        for j in pick:
-            a = pool[j]
+        rn = bytes(mol.resnames[res_atoms[0]][:3]).decode()
+        pair = [a for a in res_atoms if (rn, bytes(mol.names[a]).decode().strip("_")) in SYMMETRIC]
+        if p_symmetric > 0.0 and pair and rng.random() < p_sym_type:
+            # the symmetric pair, then other atoms of the residue
+            rest = [a for a in res_atoms if a not in pair]
+            extra = sorted(rng.choice(len(rest), size=min(max(k - 2, 0), len(rest)), replace=False))
+            picked = sorted(pair + [rest[j] for j in extra])
+            sym = True
+        else:
+            # side-chain atoms first (they carry the chemistry), then backbone
+            side = [a for a in res_atoms if bytes(mol.names[a]) not in (b"_N__", b"_C__", b"_O__")]
+            pool = side if len(side) >= k else res_atoms
+            picked = [pool[j] for j in sorted(rng.choice(len(pool), size=min(k, len(pool)), replace=False))]
+            sym = False
+        for a in picked:
             nm = bytes(mol.names[a]).decode().strip("_")
-            rn = bytes(mol.resnames[a][:3]).decode()
             x, y, z = (mol.xyz[a] + rng.normal(scale=jitter, size=3)).tolist()
+            mode = 1 if sym and a in pair else _mode_for(rn, nm, rng, p_flexible)
             atoms.append(TemplateAtom(
                 chain_id=chr(mol.chains[a, 1]), residue_number=int(mol.resseq[a]),
                 x=round(x, 3), y=round(y, 3), z=round(z, 3), residue_names=[rn], atom_names=[nm],
-                distance_weight=0.0, match_mode=_mode_for(rn, nm, rng, p_flexible)))
+                distance_weight=0.0, match_mode=mode))
     return Template(atoms, id=tpl_id)
 
 
 def random_template(rng: np.random.Generator, n_residues: int, atoms_per_residue, radius: float = 8.0,
-                    p_flexible: float = 0.3, tpl_id: str = "RAND") -> Template:
+                    p_flexible: float = 0.3, tpl_id: str = "RAND", p_symmetric: float = 0.0) -> Template:
     """Residue types drawn uniformly, residue centres uniform in a ball of
-    ``radius``, atoms 1.5 A steps around their centre."""
+    ``radius``, atoms 1.5 A steps around their centre. A fraction
+    ``p_symmetric`` of the residues is drawn from the types with a symmetric
+    atom pair instead, and carries that pair in match mode 1."""
     atoms = []
     for r in range(n_residues):
-        rn = RESIDUES[int(rng.integers(len(RESIDUES)))]
+        sym = p_symmetric > 0.0 and rng.random() < p_symmetric
+        if sym:
+            rn = _SYM_TYPES[int(rng.integers(len(_SYM_TYPES)))]
+        else:
+            rn = RESIDUES[int(rng.integers(len(RESIDUES)))]
         names = ["CA"] + SIDECHAINS[rn]
         centre = _unit(rng, 1)[0] * radius * rng.random() ** (1 / 3)
         k = int(atoms_per_residue() if callable(atoms_per_residue) else atoms_per_residue)
-        pick = sorted(rng.choice(len(names), size=min(k, len(names)), replace=False))
+        if sym:
+            pair = [j for j, nm in enumerate(names) if (rn, nm) in SYMMETRIC]
+            rest = [j for j in range(len(names)) if j not in pair]
+            extra = rng.choice(len(rest), size=min(max(k - 2, 0), len(rest)), replace=False)
+            pick = sorted(pair + [rest[j] for j in extra])
+        else:
+            pair = []
+            pick = sorted(rng.choice(len(names), size=min(k, len(names)), replace=False))
         p = centre.copy()
         for j in pick:
             p = p + 1.5 * _unit(rng, 1)[0]
             nm = names[j]
+            mode = 1 if j in pair else _mode_for(rn, nm, rng, p_flexible)
             atoms.append(TemplateAtom(
                 chain_id="A", residue_number=100 + r, x=round(p[0], 3), y=round(p[1], 3),
                 z=round(p[2], 3), residue_names=[rn], atom_names=[nm], distance_weight=0.0,
-                match_mode=_mode_for(rn, nm, rng, p_flexible)))
+                match_mode=mode))
     return Template(atoms, id=tpl_id)
 
 
 def template_set(rng: np.random.Generator, mols: Sequence[Molecule], n_templates: int,
                  planted_fraction: float, residues: Tuple[int, int], atoms_per_residue: Tuple[int, int],
-                 jitter: float = 0.3, p_flexible: float = 0.3) -> List[Template]:
+                 jitter: float = 0.3, p_flexible: float = 0.3, p_symmetric: float = 0.0) -> List[Template]:
     """``n_templates`` templates, a fraction planted in random molecules of ``mols``."""
     out = []
     starts = [_residue_starts(m) for m in mols]
@@ -304,9 +337,10 @@ def template_set(rng: np.random.Generator, mols: Sequence[Molecule], n_templates
         if i < n_planted:
             mi = int(rng.integers(len(mols)))
             out.append(planted_template(rng, mols[mi], nres, apr, jitter=jitter, p_flexible=p_flexible,
-                                        tpl_id=f"P{i:05d}", starts=starts[mi]))
+                                        tpl_id=f"P{i:05d}", starts=starts[mi], p_symmetric=p_symmetric))
         else:
-            out.append(random_template(rng, nres, apr, p_flexible=p_flexible, tpl_id=f"R{i:05d}"))
+            out.append(random_template(rng, nres, apr, p_flexible=p_flexible, tpl_id=f"R{i:05d}",
+                                       p_symmetric=p_symmetric))
     order = rng.permutation(len(out))
     return [out[i] for i in order]
 
@@ -324,9 +358,14 @@ def workload(name: str, seed: int = 0, scale: float = 1.0):
                    x 2-3 atoms, 30% planted.
     ``batch-4k``   4,096 proteins, lognormal sizes (median 4,000 atoms, clipped
                    to 1,000-40,000) x 200 templates of 3-6 residues, 10% planted.
-    ``scale`` shrinks the molecule count of the batch workload (tests).
-    All use rmsd_threshold 2.0, distance_cutoff = max_dynamic_distance = 1.5,
-    max_candidates 10,000.
+    ``library-64`` 64 assemblies of 100,000 atoms in 20 chains (residue numbers
+                   restarting per chain) x 7,000 templates of 2-8 residues x
+                   2-3 atoms, 10% planted, 20% of template residues carrying a
+                   symmetric atom pair in mode 1; distance_cutoff =
+                   max_dynamic_distance = 2.5, max_candidates 100,000.
+    ``scale`` shrinks the molecule count of the batch and library workloads
+    (tests). Unless stated, rmsd_threshold 2.0, distance_cutoff =
+    max_dynamic_distance = 1.5, max_candidates 10,000.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     kw = dict(rmsd_threshold=2.0, distance_cutoff=1.5, max_dynamic_distance=1.5, max_candidates=10000)
@@ -343,6 +382,12 @@ def workload(name: str, seed: int = 0, scale: float = 1.0):
         sizes = np.clip(np.round(np.exp(rng.normal(math.log(4000), 0.6, size=n))), 1000, 40000).astype(int)
         mols = _batch_molecules(rng, sizes)
         tpls = template_set(rng, mols, 200, 0.1, (3, 6), (2, 3))
+        return mols, tpls, kw
+    if name == "library-64":
+        n = max(1, int(round(64 * scale)))
+        mols = [protein_with_atoms(rng, 100000, n_chains=20, mol_id=f"L{i:03d}") for i in range(n)]
+        tpls = template_set(rng, mols, 7000, 0.1, (2, 8), (2, 3), p_symmetric=0.2)
+        kw.update(distance_cutoff=2.5, max_dynamic_distance=2.5, max_candidates=100000)
         return mols, tpls, kw
     raise ValueError(f"unknown workload {name!r}")
 

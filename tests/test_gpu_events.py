@@ -1,0 +1,64 @@
+"""The event stream of jam_query: delivered in report order (molecule,
+template in scan order, rank), kept in device memory until the next query on
+the context, and refused with JAM_ESTALE after that."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from pyjess_amd import Jess, synthetic
+from pyjess_amd import _native
+
+pytestmark = pytest.mark.gpu
+
+
+def device():
+    if not _native.device_available():
+        pytest.skip("no GPU")
+
+
+def test_events_arrive_in_report_order():
+    device()
+    mols, tpls, kw = synthetic.workload("batch-4k", seed=3, scale=6 / 4096)
+    jess = Jess(tpls)
+    res = jess._device_query(mols, kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"],
+                             0, True)
+    T = len(tpls)
+    assert len(res.pair) > 100
+    scan = (res.pair // T) * T + (T - 1 - res.pair % T)     # molecule, then reverse template order
+    assert np.all(np.diff(scan) >= 0)
+    same = np.diff(scan) == 0
+    assert np.all(np.diff(res.rank)[same] > 0)
+    # every event's rank is below its pair's number of complete matches
+    assert np.all(res.rank < res.pair_matches[res.pair])
+
+
+def test_result_is_stale_after_the_next_query():
+    device()
+    mols, tpls, kw = synthetic.workload("plain-2k", seed=1)
+    jess = Jess(tpls)
+    ctx = _native.context()
+    tset = jess._device_templates(ctx)
+    mset = _native.MoleculeSet(ctx, mols)
+    L = _native.lib()
+    p = _native.Params(kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"], 0, 1)
+    h1, h2 = ctypes.c_void_p(), ctypes.c_void_p()
+    with ctx.lock:
+        _native._check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h1)))
+        try:
+            n = L.jam_result_n_events(h1)
+            assert n > 0
+            first = np.zeros(n, dtype=np.float64)
+            assert L.jam_result_copy(h1, None, None, None, _native._ptr(first), None, None, None, None) == 0
+            _native._check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h2)))
+            try:
+                again = np.zeros(n, dtype=np.float64)
+                assert L.jam_result_copy(h1, None, None, None, _native._ptr(again), None, None, None,
+                                         None) == _native.JAM_ESTALE
+                assert L.jam_result_copy(h2, None, None, None, _native._ptr(again), None, None, None, None) == 0
+                assert np.array_equal(first, again)
+            finally:
+                L.jam_result_free(h2)
+        finally:
+            L.jam_result_free(h1)

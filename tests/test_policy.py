@@ -1,4 +1,4 @@
-"""The host-side policy of the product (events sorted by rank, pyjess_amd._jess._policy)
+"""The host-side policy of the product (events in report order, pyjess_amd._jess._policy)
 against the literal loop of Query.__next__ (_jess.pyx:2197-2283) restated in the
 oracle, on random streams of complete matches with hits, NaN rotations and
 sub-threshold matches, for every best_match / max_candidates combination."""
@@ -44,8 +44,8 @@ def events_of(stream, thr, n_templates):
         pos[m.template] = rank + 1
         if m.rmsd <= thr:
             rows.append((m.template, rank, 2 if m.nan else 1, m.rmsd, m.rotation, m.atoms))
-    rng = np.random.default_rng(len(rows))
-    rows = [rows[i] for i in rng.permutation(len(rows))]   # device order is arbitrary
+    # the stream is in scan order with ranks increasing: the order in which
+    # the device delivers its events
     res = types.SimpleNamespace()
     res.pair = np.array([r[0] for r in rows], dtype=np.int32)
     res.rank = np.array([r[1] for r in rows], dtype=np.int64)
