@@ -41,8 +41,15 @@ class Block:
 
 
 @cython.freelist(1024)
+@cython.no_gc
 cdef class Hit:
-    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``)."""
+    """A hit between a query molecule and a template (``_jess.pyx:2286-2621``).
+
+    Hits are not tracked by the cyclic garbage collector: a hit refers only to
+    its result block, template and molecule, none of which refers back to
+    hits, and a query can return millions of them (tracking them makes the
+    collector's passes over young objects the dominant cost of building the
+    list)."""
 
     cdef readonly double rmsd
     cdef object _block
@@ -172,17 +179,26 @@ cdef class Hit:
 
 
 def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const int[:] tpl_of,
-               const int[:] flag, const double[:] rmsd, object block, list templates, list molecules):
-    """Appends to out[molecule] the hit of every chosen event (None for a NaN
-    rotation), in the given order."""
+               const int[:] flag, const double[:] rmsd, object block, list templates, list molecules,
+               bint keep_none=True):
+    """Appends to out[molecule] the hit of every chosen event, in the given
+    order. A NaN rotation appends None when ``keep_none`` (the iterator warns
+    when it reaches it) and is skipped otherwise. Returns the number of NaN
+    rotations met per molecule."""
     cdef Py_ssize_t i, e, n = chosen.shape[0]
     cdef int mi
     cdef Hit h
+    cdef list lst
+    nan = np.zeros(len(out), dtype=np.int64)
+    cdef long long[:] nn = nan
     for i in range(n):
         e = chosen[i]
         mi = mol_of[e]
+        lst = <list>out[mi]
         if flag[e] == 2:
-            (<list>out[mi]).append(None)
+            nn[mi] += 1
+            if keep_none:
+                lst.append(None)
             continue
         h = Hit.__new__(Hit)
         h.rmsd = rmsd[e]
@@ -190,7 +206,8 @@ def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const i
         h._row = e
         h._template = templates[tpl_of[e]]
         h._molecule = molecules[mi]
-        (<list>out[mi]).append(h)
+        lst.append(h)
+    return nan
 
 
 def select_events(const int[:] pair, const long long[:] rank, const int[:] flag, const double[:] rmsd,

@@ -88,11 +88,11 @@ class Query(_HitIterator):
             return []
         res = self.jess._device_query([self.molecule], self.rmsd_threshold, self._distance_cutoff,
                                       self._max_dynamic_distance, self._ignore_chain, self._reorder)
-        return _policy(self.jess, [self.molecule], res, self.best_match, self._max_candidates)[0]
+        return _policy(self.jess, [self.molecule], res, self.best_match, self._max_candidates)[0][0]
 
 
 def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", best_match: bool,
-            max_candidates: int) -> List[list]:
+            max_candidates: int, keep_none: bool = True):
     """Hits per molecule, in pyjess order, from the device events.
 
     The device delivers the events in report order: by molecule, then
@@ -103,13 +103,14 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
     hit; when it exceeds ``max_candidates`` the rest of the template is
     skipped (after the current match, which is still reported if it is a
     hit). NaN rotations produce ``None`` entries, turned into warnings when
-    iterated.
+    iterated (``keep_none``), or are dropped. Returns the hit lists and the
+    number of NaN rotations per molecule.
     """
     T = len(jess)
     out: List[list] = [[] for _ in molecules]
     n = len(res.rmsd)
     if n == 0:
-        return out
+        return out, np.zeros(len(molecules), dtype=np.int64)
     mc = max_candidates
     pair = np.ascontiguousarray(res.pair, dtype=np.int32)
     # the candidate counter of a template never exceeds its number of complete
@@ -124,10 +125,11 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
                                 np.ascontiguousarray(res.flag, dtype=np.int32),
                                 np.ascontiguousarray(res.rmsd, dtype=np.float64), best_match, mc)
     block = _Block(res.rotation, res.c0, res.c1, res.atoms)
-    _build_hits(out, chosen, (pair // T).astype(np.int32), (pair % T).astype(np.int32),
-                np.ascontiguousarray(res.flag, dtype=np.int32), np.ascontiguousarray(res.rmsd, dtype=np.float64),
-                block, list(jess._templates), list(molecules))
-    return out
+    nan = _build_hits(out, chosen, (pair // T).astype(np.int32), (pair % T).astype(np.int32),
+                      np.ascontiguousarray(res.flag, dtype=np.int32),
+                      np.ascontiguousarray(res.rmsd, dtype=np.float64), block, list(jess._templates),
+                      list(molecules), keep_none)
+    return out, nan
 
 
 class Jess:
@@ -221,12 +223,9 @@ class Jess:
             return [[] for _ in molecules]
         res = self._device_query(molecules, rmsd_threshold, distance_cutoff, max_dynamic_distance,
                                  _IGNORE[ignore_chain], reorder)
-        out = _policy(self, molecules, res, best_match, mc)
-        hits = []
-        for lst in out:
-            if any(h is None for h in lst):
-                _warn_nan()
-            hits.append([h for h in lst if h is not None])
+        hits, nan = _policy(self, molecules, res, best_match, mc, keep_none=False)
+        for _ in range(int(np.count_nonzero(nan))):
+            _warn_nan()
         return hits
 
 

@@ -110,6 +110,11 @@ struct jam_context {
     int front_stride = 0, done_stride = 0;
     int join_grid[3] = {0, 0, 0};   // resident workgroups of k_join<8/16/32>
     int kd_grid[12] = {};           // resident workgroups of the kd kernels
+    // the exact kd pass of each size tier runs on a stream of its own, from
+    // the moment the tier's fast pass has listed its tied sets, beside the
+    // fast passes of the other tiers (all streams at the default priority)
+    hipStream_t kd_side[KD_TIERS] = {};
+    hipEvent_t kd_fork[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
     int64_t sort_cap = 0;
@@ -266,6 +271,11 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
     for (auto &ev : c->xev) HIP_TRY(hipEventCreate(&ev));
+    for (int k = 0; k < KD_TIERS; k++) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->kd_side[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->kd_fork[k], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->kd_joined[k], hipEventDisableTiming));
+    }
     *out = c.release();
     return JAM_OK;
 }
@@ -287,6 +297,12 @@ extern "C" void jam_context_free(jam_context *ctx)
         if (ev) (void)hipEventDestroy(ev);
     for (auto &ev : ctx->xev)
         if (ev) (void)hipEventDestroy(ev);
+    for (int k = 0; k < KD_TIERS; k++) {
+        if (ctx->kd_side[k]) (void)hipStreamSynchronize(ctx->kd_side[k]);
+        if (ctx->kd_fork[k]) (void)hipEventDestroy(ctx->kd_fork[k]);
+        if (ctx->kd_joined[k]) (void)hipEventDestroy(ctx->kd_joined[k]);
+        if (ctx->kd_side[k]) (void)hipStreamDestroy(ctx->kd_side[k]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -812,25 +828,63 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_THREADS, d);
             }
             auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
-            hipLaunchKernelGGL(jam::k_kdwave, grid(0), dim3(JAM_WAVE), 0, s, dm, dt, st, bucket, nb, tied, nt);
-            hipLaunchKernelGGL((jam::k_kdblock<128>), grid(1), dim3(128), 0, s, dm, dt, st, bucket + ns, nb + 1,
-                               tied + ns, nt + 1);
-            hipLaunchKernelGGL((jam::k_kdblock<256>), grid(2), dim3(256), 0, s, dm, dt, st, bucket + 2 * ns, nb + 2,
-                               tied + 2 * ns, nt + 2);
-            hipLaunchKernelGGL((jam::k_kdblock<512>), grid(3), dim3(512), 0, s, dm, dt, st, bucket + 3 * ns, nb + 3,
-                               tied + 3 * ns, nt + 3);
-            hipLaunchKernelGGL((jam::k_kdblock<1024>), grid(4), dim3(1024), 0, s, dm, dt, st, bucket + 4 * ns,
-                               nb + 4, tied + 4 * ns, nt + 4);
-            hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, s, dm, dt, st, scr, bucket + 5 * ns,
-                               nb + 5, tied + 5 * ns, nt + 5);
-            hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, s, dm, dt, st, tied, nt);
-            hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, s, dm, dt, st, tied + ns, nt + 1);
-            hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, s, dm, dt, st, tied + 2 * ns, nt + 2);
-            hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, s, dm, dt, st, tied + 3 * ns, nt + 3);
-            hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, s, dm, dt, st, tied + 4 * ns,
-                               nt + 4);
-            hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_THREADS), 0, s, dm, dt, st, scr, tied + 5 * ns,
-                               nt + 5);
+            // largest tier first: its exact pass, the longest, starts earliest
+            for (int k = KD_TIERS - 1; k >= 0; k--) {
+                const size_t o = (size_t)k * ns;
+                switch (k) {
+                case 0:
+                    hipLaunchKernelGGL(jam::k_kdwave, grid(0), dim3(JAM_WAVE), 0, s, dm, dt, st, bucket, nb, tied,
+                                       nt);
+                    break;
+                case 1:
+                    hipLaunchKernelGGL((jam::k_kdblock<128>), grid(1), dim3(128), 0, s, dm, dt, st, bucket + o,
+                                       nb + 1, tied + o, nt + 1);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((jam::k_kdblock<256>), grid(2), dim3(256), 0, s, dm, dt, st, bucket + o,
+                                       nb + 2, tied + o, nt + 2);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL((jam::k_kdblock<512>), grid(3), dim3(512), 0, s, dm, dt, st, bucket + o,
+                                       nb + 3, tied + o, nt + 3);
+                    break;
+                case 4:
+                    hipLaunchKernelGGL((jam::k_kdblock<1024>), grid(4), dim3(1024), 0, s, dm, dt, st, bucket + o,
+                                       nb + 4, tied + o, nt + 4);
+                    break;
+                default:
+                    hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, s, dm, dt, st, scr, bucket + o,
+                                       nb + 5, tied + o, nt + 5);
+                    break;
+                }
+                hipStream_t x = c->kd_side[k];
+                HIP_TRY(hipEventRecord(c->kd_fork[k], s));
+                HIP_TRY(hipStreamWaitEvent(x, c->kd_fork[k], 0));
+                switch (k) {
+                case 0:
+                    hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, tied, nt);
+                    break;
+                case 1:
+                    hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, x, dm, dt, st, tied + o, nt + 1);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, x, dm, dt, st, tied + o, nt + 2);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, x, dm, dt, st, tied + o, nt + 3);
+                    break;
+                case 4:
+                    hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, tied + o,
+                                       nt + 4);
+                    break;
+                default:
+                    hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_THREADS), 0, x, dm, dt, st, scr,
+                                       tied + o, nt + 5);
+                    break;
+                }
+                HIP_TRY(hipEventRecord(c->kd_joined[k], x));
+            }
+            for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[2], s));

@@ -225,6 +225,10 @@ __global__ void __launch_bounds__(256) k_level0(DevPairs pr, DevJoin jn)
 #ifndef JOIN_UNROLL
 #define JOIN_UNROLL 1
 #endif
+// most waves sharing one row's blocks when the frontier is small
+#ifndef JOIN_SPLIT
+#define JOIN_SPLIT 16
+#endif
 // sets of at most JOIN_SMALL blocks skip the block-box test: their candidates
 // are loaded at once
 #ifndef JOIN_SMALL
@@ -275,7 +279,7 @@ __device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &stag
 template <int MAXN>
 __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                 const DevJoin &jn, const QueryParams &qp, JoinLds<MAXN> &S, long long i,
-                                int &staged)
+                                int part, int parts, int &staged)
 {
     const int lane = lane_id();
     const int L = jn.level;
@@ -288,7 +292,7 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
     const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
     if (L >= n) {
         // a one-atom template is complete at level 0
-        if (lane == 0) {
+        if (lane == 0 && part == 0) {
             const unsigned long long r = atomicAdd(jn.n_done, 1ull);
             if ((long long)r < jn.done_cap) {
                 jn.done_pair[r] = p;
@@ -364,6 +368,7 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
     const int ignore = qp.ignore_chain;
     const long long bbase = (long long)offL / JAM_WAVE + setL;
     const bool last = (L == n - 1);
+    int seen = 0;   // in-box blocks met so far; this wave takes every parts-th one
     for (int g = 0; g * JAM_WAVE < nblk; g++) {
         // blocks of this group whose box meets the join box (all of them for
         // small sets: the test is a prefilter only)
@@ -379,6 +384,17 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
                        lo[2] > bb[5]);
             }
             bm = ballot(ok);
+        }
+        if (parts > 1) {
+            // keep the in-box blocks whose running index is part mod parts
+            unsigned long long keep = 0ull, rest = bm;
+            while (rest) {
+                const int b = first_bit(rest);
+                rest &= rest - 1;
+                if (seen % parts == part) keep |= 1ull << b;
+                seen++;
+            }
+            bm = keep;
         }
         while (bm) {
             // up to JOIN_UNROLL blocks per round, their loads in flight together
@@ -472,9 +488,14 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
     if (n_in > jn.in_cap) n_in = jn.in_cap;
     int staged = 0;
     const long long nw = (long long)gridDim.x * JOIN_WAVES;
-    for (long long i = (long long)blockIdx.x * JOIN_WAVES + w; i < n_in; i += nw) {
+    // a frontier smaller than the resident waves is spread thinner: each row's
+    // in-box blocks are shared by `parts` waves
+    const int parts = (n_in > 0 && n_in < nw) ? (int)min((long long)JOIN_SPLIT, nw / n_in) : 1;
+    const long long items = n_in * parts;
+    for (long long it = (long long)blockIdx.x * JOIN_WAVES + w; it < items; it += nw) {
         wave_sync();
-        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, staged);
+        const long long i = it / parts;
+        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, (int)(it - i * parts), parts, staged);
     }
     join_flush(jn, S[w], staged);
 }

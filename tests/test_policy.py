@@ -72,7 +72,7 @@ def test_policy_matches_query_loop(seed, best_match, max_candidates):
     jess = Jess([two_atom_template(i) for i in range(n_t)])
     mol = synthetic.protein(np.random.Generator(np.random.PCG64(0)), 400)
     mc = -1 if max_candidates is None else max_candidates
-    got = _policy(jess, [mol], events_of(stream, thr, n_t), best_match, mc)[0]
+    got = _policy(jess, [mol], events_of(stream, thr, n_t), best_match, mc)[0][0]
     got_hits = [h for h in got if h is not None]
     assert len(got) - len(got_hits) == warns
     assert [(jess._templates.index(h.template()), tuple(int(a.serial) - 1 for a in h.atoms(False)))
