@@ -323,8 +323,10 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
     # the same with every event copied to host memory
     t0 = time.perf_counter()
     first = None
+    partials = np.zeros(32, dtype=np.int64)
     for ms in msets:
         r = _native.query(tset, ms, *qargs, 0, True)
+        partials += r.level_partials
         if first is None:
             first = r
         del r
@@ -350,6 +352,7 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
                    "distance_cutoff": kw["distance_cutoff"], "max_dynamic_distance": kw["max_dynamic_distance"],
                    "max_candidates": kw["max_candidates"]},
         "events": int(n_events),
+        "join_level_partials": [int(x) for x in partials[1:] if x > 0],
         "device_ms": {"candidates": stage[0], "kd_order": stage[1], "join": stage[2], "superposition": stage[3],
                       "total": stage[4]},
         "with_event_copy": {"value": n_pairs / copy_s, "unit": "queries/s", "seconds": copy_s},

@@ -241,21 +241,22 @@ struct JoinLds {
     double r2lo[MAXN], r2hi[MAXN];
     int pch[MAXN], pres[MAXN];
     int tch[MAXN], tres[MAXN];
-    int ent[MAXN];
-    // children waiting to be written: parent row and the new entry
+    // children waiting to be written: parent row and the new entry, for the
+    // next frontier (sp, se) and for the complete matches (dp, de)
     int sp[JOIN_STAGE], se[JOIN_STAGE];
+    int dp[JOIN_STAGE], de[JOIN_STAGE];
 };
 
-// Writes the wave's staged children to the next frontier: one atomic for all
-// of them, then the rows (parent's entries + the new one) by all lanes.
-template <int MAXN>
-__device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
+// Writes `staged` rows (parent row par[k] + the new entry ent[k]) to a row
+// buffer: one atomic for all of them, then the rows by all lanes.
+__device__ static void join_write(const DevJoin &jn, const int *par, const int *ent, int &staged,
+                                  unsigned long long *count, long long cap, int *out_pair, int *out_ent)
 {
     if (staged == 0) return;
     wave_sync();
     const int lane = lane_id();
     unsigned long long b0 = 0;
-    if (lane == 0) b0 = atomicAdd(jn.n_out, (unsigned long long)staged);
+    if (lane == 0) b0 = atomicAdd(count, (unsigned long long)staged);
     const long long base = (long long)__shfl(b0, 0, JAM_WAVE);
     const int L = jn.level;
     const int w = L + 1;
@@ -263,23 +264,35 @@ __device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &stag
     for (int idx = lane; idx < total; idx += JAM_WAVE) {
         const int k = idx / w, l = idx - k * w;
         const long long r = base + k;
-        if (r >= jn.out_cap) {
+        if (r >= cap) {
             if (l == 0) atomicOr(jn.overflow, 1u);
             continue;
         }
-        const int par = S.sp[k];
-        jn.out_ent[r * jn.stride + l] = l < L ? jn.in_ent[(long long)par * jn.stride + l] : S.se[k];
-        if (l == 0) jn.out_pair[r] = jn.in_pair[par];
+        const int pr = par[k];
+        out_ent[r * jn.stride + l] = l < L ? jn.in_ent[(long long)pr * jn.stride + l] : ent[k];
+        if (l == 0) out_pair[r] = jn.in_pair[pr];
     }
     wave_sync();
     staged = 0;
+}
+
+template <int MAXN>
+__device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
+{
+    join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
+}
+
+template <int MAXN>
+__device__ static void join_flush_done(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
+{
+    join_write(jn, S.dp, S.de, staged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
 }
 
 // One wave extends partial i of level L-1 by the candidates of level L.
 template <int MAXN>
 __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                 const DevJoin &jn, const QueryParams &qp, JoinLds<MAXN> &S, long long i,
-                                int part, int parts, int &staged)
+                                int part, int parts, int &staged, int &dstaged)
 {
     const int lane = lane_id();
     const int L = jn.level;
@@ -321,7 +334,6 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
         } else {
             en = st.kent[e];
         }
-        S.ent[lane] = e;
         S.px[lane] = en.x;
         S.py[lane] = en.y;
         S.pz[lane] = en.z;
@@ -448,20 +460,13 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
                 const int cnt = __popcll(mask);
                 const int r = rank_below(mask);
                 if (last) {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(jn.n_done, (unsigned long long)cnt);
-                    base = __shfl(base, 0, JAM_WAVE);
-                    const long long slot = (long long)base + r;
+                    // complete matches, staged like the children
+                    if (dstaged + cnt > JOIN_STAGE) join_flush_done(jn, S, dstaged);
                     if (ok) {
-                        if (slot < jn.done_cap) {
-                            int *out = jn.done_ent + slot * jn.stride;
-                            for (int l = 0; l < L; l++) out[l] = S.ent[l];
-                            out[L] = offL + q;
-                            jn.done_pair[slot] = p;
-                        } else {
-                            atomicOr(jn.overflow, 1u);
-                        }
+                        S.dp[dstaged + r] = (int)i;
+                        S.de[dstaged + r] = offL + q;
                     }
+                    dstaged += cnt;
                 } else {
                     if (staged + cnt > JOIN_STAGE) join_flush(jn, S, staged);
                     if (ok) {
@@ -486,7 +491,7 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
     const int w = threadIdx.x / JAM_WAVE;
     long long n_in = (long long)*jn.n_in;
     if (n_in > jn.in_cap) n_in = jn.in_cap;
-    int staged = 0;
+    int staged = 0, dstaged = 0;
     const long long nw = (long long)gridDim.x * JOIN_WAVES;
     // a frontier smaller than the resident waves is spread thinner: each row's
     // in-box blocks are shared by `parts` waves
@@ -495,9 +500,10 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
     for (long long it = (long long)blockIdx.x * JOIN_WAVES + w; it < items; it += nw) {
         wave_sync();
         const long long i = it / parts;
-        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, (int)(it - i * parts), parts, staged);
+        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, (int)(it - i * parts), parts, staged, dstaged);
     }
     join_flush(jn, S[w], staged);
+    join_flush_done(jn, S[w], dstaged);
 }
 
 // ---------------------------------------------------------------------------
