@@ -432,9 +432,12 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
                 if (u >= nb) break;
                 const int q = blk[u] * JAM_WAVE + lane;
                 bool ok = q < cntL;
-                if (ok) {
-                    const SetEnt c = cand[u];
-                    for (int j = 0; j < L; j++) {
+                const SetEnt c = cand[u];
+                for (int j = 0; j < L; j++) {
+                    // most in-box candidates fail an early annulus: stop as
+                    // soon as no lane of the wave is still in
+                    if (ballot(ok) == 0ull) break;
+                    if (ok) {
                         // _Annulus_po: squared distance summed x, y, z from 0.0
                         double tx = S.px[j] - c.x, ty = S.py[j] - c.y, tz = S.pz[j] - c.z;
                         double s2 = 0.0;
