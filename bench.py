@@ -10,11 +10,16 @@ HBM. The metric is template x molecule queries per second.
 
 The headline ``value`` is the plain drop-in on one 8,000-atom protein x 500
 templates; the ``batched`` object is the batched form on 4,096 molecules x 200
-templates; ``differentiation`` is the differentiable superposition RMSD
-(forward + backward) over 1M point-set pairs. Each carries a CPU baseline (the C restatement of the reference
-algorithm on the host's cores, hit-set parity checked) and the roofline of the
-dominant kernel, the join's k_join launches (one per level), timed with HIP events on the
-library's stream in as many steps again after the timed ones.
+templates; ``library`` is the library-scale form (64 assemblies of 100,000
+atoms x 7,000 templates, in chunks of assemblies through the C ABI, events
+left in HBM; the event copy to host memory and the drop-in ``query_batch``
+are reported beside it); ``differentiation`` is the differentiable
+superposition RMSD (forward + backward) over 1M point-set pairs. Each carries
+a CPU baseline (the C restatement of the reference algorithm on the host's
+cores, hit-set parity checked; torch autograd through an SVD for the RMSD)
+and the search legs the roofline of the dominant kernel, the join's k_join
+launches (one per level), timed with HIP events on the library's stream in as
+many steps again after the timed ones.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -533,6 +538,8 @@ def main():
     ap.add_argument("--workload", default="plain-8k",
                     help="plain-2k | plain-8k | batch-4k | rmsd-grad-1m | library-64")
     ap.add_argument("--library-chunk", type=int, default=8, help="assemblies per batched call of library-64")
+    ap.add_argument("--library-scale", type=float, default=1.0,
+                    help="fraction of the 64 assemblies of the library-scale measurement (0: skip it)")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch workload's molecules")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of each CPU baseline")
@@ -577,6 +584,12 @@ def main():
                           not args.no_cpu, 32)
         batched["vs_plain"] = batched["value"] / main_run["value"]
 
+    library = None
+    if args.library_scale > 0 and args.workload != "batch-4k":
+        library = measure_library(args.library_scale, args.seed, args.library_chunk, 1, 1, args.cpu_seconds,
+                                  not args.no_cpu)
+        library["vs_plain"] = library["value"] / main_run["value"]
+
     diff = None
     if args.diff_sets > 0 and args.workload != "batch-4k":
         diff = measure_diff(args.diff_sets, args.seed, max(3, args.batch_steps), 1, args.cpu_seconds,
@@ -607,6 +620,7 @@ def main():
         "complete_matches": main_run["complete_matches"],
         "join_level_partials": main_run["join_level_partials"],
         "batched": batched,
+        "library": library,
         "differentiation": diff,
     }
     print(json.dumps(out), flush=True)
