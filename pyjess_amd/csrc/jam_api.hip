@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -127,6 +128,10 @@ struct jam_context {
     // incremented by every query: a result's events live in e_* until the
     // next query on the context
     uint64_t query_gen = 0;
+    // pinned host staging for event copies: two halves of JAM_STAGE_BYTES,
+    // device-to-host copies into one half while the other is unpacked
+    void *hstage = nullptr;
+    hipEvent_t stage_ev[2] = {};
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
 };
@@ -296,6 +301,9 @@ extern "C" void jam_context_free(jam_context *ctx)
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto &ev : ctx->xev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+    for (auto &ev : ctx->stage_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (int k = 0; k < KD_TIERS; k++) {
         if (ctx->kd_side[k]) (void)hipStreamSynchronize(ctx->kd_side[k]);
@@ -1076,6 +1084,37 @@ extern "C" int32_t jam_result_stride(const jam_result *r) { return r ? r->stride
 
 extern "C" int32_t jam_result_n_pairs(const jam_result *r) { return r ? r->n_pairs : 0; }
 
+namespace {
+
+constexpr size_t JAM_STAGE_BYTES = size_t(64) << 20;
+
+// memcpy spread over a few host threads when it is large
+void host_copy(void *dst, const void *src, size_t bytes)
+{
+    const size_t per = size_t(16) << 20;
+    const unsigned nt = (unsigned)std::min<size_t>(8, bytes / per);
+    if (nt < 2) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t part = (bytes + nt - 1) / nt;
+    for (unsigned k = 0; k < nt; k++) {
+        const size_t b0 = k * part, b1 = std::min(bytes, b0 + part);
+        if (b0 < b1)
+            th.emplace_back([=] { std::memcpy((char *)dst + b0, (const char *)src + b0, b1 - b0); });
+    }
+    for (auto &x : th) x.join();
+}
+
+struct CopyPiece {
+    void *dst;
+    const char *src;
+    size_t bytes;
+};
+
+}  // namespace
+
 extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
                                    double *rmsd, double *rot, double *c0, double *c1, int32_t *atoms)
 {
@@ -1087,18 +1126,59 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
         return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    auto get = [&](void *dst, const DBuf &src, size_t bytes) -> hipError_t {
-        return dst ? hipMemcpyAsync(dst, src.ptr, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+    if (!c->hstage) {
+        HIP_TRY(hipHostMalloc(&c->hstage, 2 * JAM_STAGE_BYTES, hipHostMallocDefault));
+        for (auto &ev : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    // the fields, cut into pieces of at most one staging half
+    std::vector<CopyPiece> pieces;
+    auto add = [&](void *dst, const DBuf &src, size_t bytes) {
+        if (!dst) return;
+        for (size_t o = 0; o < bytes; o += JAM_STAGE_BYTES)
+            pieces.push_back({(char *)dst + o, (const char *)src.ptr + o, std::min(JAM_STAGE_BYTES, bytes - o)});
     };
-    HIP_TRY(get(pair, c->e_pair, sizeof(int32_t) * n));
-    HIP_TRY(get(rank, c->e_rank, sizeof(int64_t) * n));
-    HIP_TRY(get(flag, c->e_flag, sizeof(int32_t) * n));
-    HIP_TRY(get(rmsd, c->e_rmsd, sizeof(double) * n));
-    HIP_TRY(get(rot, c->e_rot, sizeof(double) * 9 * n));
-    HIP_TRY(get(c0, c->e_c0, sizeof(double) * 3 * n));
-    HIP_TRY(get(c1, c->e_c1, sizeof(double) * 3 * n));
-    HIP_TRY(get(atoms, c->e_atoms, sizeof(int32_t) * (size_t)r->stride * n));
-    HIP_TRY(hipStreamSynchronize(s));
+    add(pair, c->e_pair, sizeof(int32_t) * n);
+    add(rank, c->e_rank, sizeof(int64_t) * n);
+    add(flag, c->e_flag, sizeof(int32_t) * n);
+    add(rmsd, c->e_rmsd, sizeof(double) * n);
+    add(rot, c->e_rot, sizeof(double) * 9 * n);
+    add(c0, c->e_c0, sizeof(double) * 3 * n);
+    add(c1, c->e_c1, sizeof(double) * 3 * n);
+    add(atoms, c->e_atoms, sizeof(int32_t) * (size_t)r->stride * n);
+    // batches of consecutive pieces filling one half each; the copy of batch
+    // k + 1 runs while batch k is unpacked from the other half
+    struct Batch {
+        size_t first, last, half;
+    };
+    std::vector<Batch> batches;
+    for (size_t i = 0; i < pieces.size();) {
+        size_t used = 0, j = i;
+        while (j < pieces.size() && used + pieces[j].bytes <= JAM_STAGE_BYTES) used += pieces[j++].bytes;
+        batches.push_back({i, j, batches.size() & 1});
+        i = j;
+    }
+    auto issue = [&](const Batch &b) -> hipError_t {
+        char *h = (char *)c->hstage + b.half * JAM_STAGE_BYTES;
+        size_t o = 0;
+        for (size_t i = b.first; i < b.last; i++) {
+            hipError_t e = hipMemcpyAsync(h + o, pieces[i].src, pieces[i].bytes, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) return e;
+            o += pieces[i].bytes;
+        }
+        return hipEventRecord(c->stage_ev[b.half], s);
+    };
+    if (!batches.empty()) HIP_TRY(issue(batches[0]));
+    for (size_t k = 0; k < batches.size(); k++) {
+        if (k + 1 < batches.size()) HIP_TRY(issue(batches[k + 1]));
+        const Batch &b = batches[k];
+        HIP_TRY(hipEventSynchronize(c->stage_ev[b.half]));
+        const char *h = (const char *)c->hstage + b.half * JAM_STAGE_BYTES;
+        size_t o = 0;
+        for (size_t i = b.first; i < b.last; i++) {
+            host_copy(pieces[i].dst, h + o, pieces[i].bytes);
+            o += pieces[i].bytes;
+        }
+    }
     return JAM_OK;
 }
 
