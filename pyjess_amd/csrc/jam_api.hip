@@ -833,7 +833,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 c->kd_grid[8] = resident_grid(jam::k_kdexact<256>, 256, d);
                 c->kd_grid[9] = resident_grid(jam::k_kdexact<512>, 512, d);
                 c->kd_grid[10] = resident_grid(jam::k_kdexact<1024>, 1024, d);
-                c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_THREADS, d);
+                c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_EXACT_THREADS, d);
             }
             auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
             // largest tier first: its exact pass, the longest, starts earliest
@@ -886,7 +886,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                                        nt + 4);
                     break;
                 default:
-                    hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_THREADS), 0, x, dm, dt, st, scr,
+                    hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr,
                                        tied + o, nt + 5);
                     break;
                 }

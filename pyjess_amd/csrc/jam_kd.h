@@ -47,6 +47,9 @@ namespace jam {
 #define KD_BIG 1024
 #define KD_TIERS 6
 #define KD_THREADS 256
+// threads of the exact pass over a set larger than KD_BIG: its rounds are
+// latency-bound loops over the positions, so more threads per set
+#define KD_EXACT_THREADS 1024
 // global scratch of sets larger than KD_BIG, per set entry (padded to a power
 // of two for the fast path's sorts: 3 doubles, 9 ints and 1 byte per padded
 // entry, at most 2 * 61 bytes per entry)
@@ -936,11 +939,11 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
 }
 
 // exact path of the sets of `list`
-__global__ void __launch_bounds__(KD_THREADS) k_kdbig_exact(DevMols ms, DevTpls tp, DevSets st,
-                                                            unsigned char *scr, const int *list,
-                                                            const unsigned int *count)
+__global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, DevTpls tp, DevSets st,
+                                                                  unsigned char *scr, const int *list,
+                                                                  const unsigned int *count)
 {
-    __shared__ int partial[KD_THREADS];
+    __shared__ int partial[KD_EXACT_THREADS];
     __shared__ int flag[2];
     if (*st.total > st.cap) return;
     const int cnt = (int)*count;
@@ -965,8 +968,8 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig_exact(DevMols ms, DevTpls 
         w.stride = (int)stride;
         kd_load_global(ms, st, n, off, abase, w.c, stride);
         int *order = st.kidx + off;
-        kd_exact<KD_THREADS>(w, n, order, partial, flag);
-        kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
+        kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag);
+        kd_emit_order<KD_EXACT_THREADS>(ms, st, s, n, off, abase, order);
         __syncthreads();
     }
 }
