@@ -102,7 +102,7 @@ struct jam_context {
     hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
     bool profile = false;       // record xev (each record costs stream time)
     // per-query workspace
-    DBuf cnt, off, list, kidx, kent, katom, bbox, total, maxcnt, kscr, kd_bucket, kd_tied, kd_count;
+    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, total, maxcnt, kscr, kd_bucket, kd_tied, kd_count;
     DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
     // join: two frontiers (pair + stride entries per row), complete matches,
     // counters (rows per level, complete matches, overflow flag)
@@ -515,6 +515,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         cap = std::max<int64_t>(cap, 1024);
         HIP_TRY(c->list.reserve(sizeof(int) * cap));
         HIP_TRY(c->kidx.reserve(sizeof(int) * cap));
+        HIP_TRY(c->l2k.reserve(sizeof(int) * cap));
         HIP_TRY(c->kent.reserve(sizeof(SetEnt) * cap));
         HIP_TRY(c->katom.reserve(sizeof(int) * cap));
         HIP_TRY(c->kscr.reserve((size_t)KD_SCRATCH_BYTES_PER_ENTRY * cap));
@@ -757,6 +758,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         st.off = c->off.as<int64_t>();
         st.list = c->list.as<int>();
         st.kidx = c->kidx.as<int>();
+        st.l2k = c->l2k.as<int>();
         st.kent = c->kent.as<SetEnt>();
         st.katom = c->katom.as<int>();
         st.bbox = c->bbox.as<double>();
@@ -926,7 +928,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             jn.out_cap = c->front_cap;
             jn.out_pair = c->j_pair[0].as<int>();
             jn.out_ent = c->j_ent[0].as<int>();
-            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, pr, jn);
+            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, st, pr, jn);
             HIP_TRY(hipGetLastError());
             for (int L = 1; L <= levels; L++) {
                 const int bi = (L - 1) & 1, bo = L & 1;
@@ -983,7 +985,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             for (int l0 = ((nt - 1) / per) * per; l0 >= 0; l0 -= per) {
                 const int cnt_l = std::min(per, nt - l0);
                 // levels [l0, l0 + cnt_l) packed into cnt_l * bits bits
-                hipLaunchKernelGGL(jam::k_sortkey, dim3(gs), dim3(256), 0, s, dt, pr, jn,
+                hipLaunchKernelGGL(jam::k_sortkey, dim3(gs), dim3(256), 0, s, st, dt, pr, jn,
                                    c->s_perm[cur].as<const int>(), (long long)n_done, l0, cnt_l, bits,
                                    c->s_key[0].as<unsigned long long>());
                 HIP_TRY(hipcub::DeviceRadixSort::SortPairs(

@@ -72,7 +72,8 @@ struct DevSets {
     int *cnt;                 // per set
     int64_t *off;             // per set: exclusive scan of cnt
     int *list;                // reference-order candidate list (molecule-local atom)
-    int *kidx;                // scratch: kd permutation (sets larger than the LDS paths)
+    int *kidx;                // kd position -> list position (set-local), per entry
+    int *l2k;                 // list position -> kd position (set-local), per entry
     SetEnt *kent;             // kd DFS order
     int *katom;               // kd DFS order: molecule-local atom
     double *bbox;             // 6 per block: lo xyz, hi xyz

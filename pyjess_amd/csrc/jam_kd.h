@@ -121,6 +121,8 @@ __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st,
         en.res = ms.resseq[g];
         st.kent[off + p] = en;
         st.katom[off + p] = atom;
+        st.kidx[off + p] = e;
+        st.l2k[off + e] = p;
     }
     double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
     double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
@@ -931,7 +933,8 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
         w.side = (unsigned char *)(ib + 9 * n2);
         w.stride = n2;
         kd_load_global(ms, st, n, off, abase, w.c, (size_t)n2);
-        int *order = st.kidx + off;
+        // the arrangement after the work arrays: 61 * n2 + 4 * n <= 128 * n bytes
+        int *order = (int *)((unsigned char *)(ib + 9 * n2) + n2);
         if (kd_fast<KD_THREADS>(w, n, order, partial, flag)) kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
         else if (threadIdx.x == 0) tied[atomicAdd(n_tied, 1u)] = s;
         __syncthreads();
@@ -967,7 +970,7 @@ __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, De
         w.wr = ib + 8 * stride;
         w.stride = (int)stride;
         kd_load_global(ms, st, n, off, abase, w.c, stride);
-        int *order = st.kidx + off;
+        int *order = ib + 9 * stride;   // the arrangement after the work arrays (64 * n bytes)
         kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag);
         kd_emit_order<KD_EXACT_THREADS>(ms, st, s, n, off, abase, order);
         __syncthreads();

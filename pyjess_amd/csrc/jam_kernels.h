@@ -209,15 +209,18 @@ __global__ void __launch_bounds__(256) k_items(DevPairs pr)
 // reference's enumeration order and each match's rank.
 
 // Level-0 rows: the candidates of every pair's level-0 list.
-__global__ void __launch_bounds__(256) k_level0(DevPairs pr, DevJoin jn)
+__global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin jn)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *jn.n_out = (unsigned long long)pr.n_items;
     if (i >= pr.n_items) return;
     const int p = pr.item_pair[i];
     const int pos0 = (int)(i - pr.item_base[p]) + pr.n1[p];
+    const int off0 = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off;
     jn.out_pair[i] = p;
-    jn.out_ent[i * jn.stride] = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off + pos0;
+    // the row holds kd entries at every level; level 0 walks its list in the
+    // reference order, so its entry is the kd position of list entry pos0
+    jn.out_ent[i * jn.stride] = off0 + st.l2k[off0 + pos0];
 }
 
 #define JOIN_WAVES 4
@@ -323,17 +326,7 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
         // the atom placed at level `lane`
         const int e = row[lane];
         const int k = lv[lane].k;
-        SetEnt en;
-        if (lane == 0) {
-            const size_t g = (size_t)(ms.atom_off[p / T] + st.list[e]);
-            en.x = ms.xyz[3 * g + 0];
-            en.y = ms.xyz[3 * g + 1];
-            en.z = ms.xyz[3 * g + 2];
-            en.ch = ms.chain[g];
-            en.res = ms.resseq[g];
-        } else {
-            en = st.kent[e];
-        }
+        const SetEnt en = st.kent[e];
         S.px[lane] = en.x;
         S.py[lane] = en.y;
         S.pz[lane] = en.z;
@@ -523,7 +516,7 @@ __global__ void __launch_bounds__(256) k_iota(int *v, long long n)
 // the match's atoms in their sets (list position at level 0, kd position
 // above), `bits` bits each, the earliest level most significant; levels past
 // the template's size count as 0 (all matches of a pair have the same size).
-__global__ void __launch_bounds__(256) k_sortkey(DevTpls tp, DevPairs pr, DevJoin jn, const int *perm,
+__global__ void __launch_bounds__(256) k_sortkey(DevSets st, DevTpls tp, DevPairs pr, DevJoin jn, const int *perm,
                                                  long long n, int l0, int per, int bits, unsigned long long *key)
 {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,7 +530,8 @@ __global__ void __launch_bounds__(256) k_sortkey(DevTpls tp, DevPairs pr, DevJoi
     for (int q = 0; q < per; q++) {
         const int l = l0 + q;
         unsigned long long pos = 0;
-        if (l < nt) pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
+        if (l == 0) pos = (unsigned long long)st.kidx[row[0]];
+        else if (l < nt) pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
         k = (k << bits) | pos;
     }
     key[s] = k;
@@ -598,7 +592,7 @@ __device__ __attribute__((always_inline)) static inline int super_of(DevMols ms,
     const unsigned char *ord = pr.ord + (size_t)pair * JAM_MAX_LEVELS;
     const int *row = jn.done_ent + (long long)i * jn.stride;
     // molecule-local atom of template atom k: placed at the level l with ord[l] == k
-    for (int l = 0; l < n; l++) atoms[ord[l]] = (l == 0) ? st.list[row[0]] : st.katom[row[l]];
+    for (int l = 0; l < n; l++) atoms[ord[l]] = st.katom[row[l]];
     auto pa = [&](int k, int d) { return ms.xyz[3 * (size_t)(abase + atoms[k]) + d]; };
     auto pb = [&](int k, int d) { return tp.pos[3 * (size_t)(tbase + k) + d]; };
     superpose(n, pa, pb, rmsd, rot, c0, c1);
