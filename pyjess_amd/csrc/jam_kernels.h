@@ -295,16 +295,14 @@ __device__ static void join_flush_done(const DevJoin &jn, JoinLds<MAXN> &S, int 
 template <int MAXN>
 __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                 const DevJoin &jn, const QueryParams &qp, JoinLds<MAXN> &S, long long i,
-                                int part, int parts, int &staged, int &dstaged)
+                                int p, int e_lane, int part, int parts, int &staged, int &dstaged)
 {
     const int lane = lane_id();
     const int L = jn.level;
-    const int p = jn.in_pair[i];
     const int T = tp.n_templates;
     const int t = p % T;
     const int tbase = tp.atom_off[t];
     const int n = tp.atom_off[t + 1] - tbase;
-    const int *row = jn.in_ent + i * jn.stride;
     const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
     if (L >= n) {
         // a one-atom template is complete at level 0
@@ -312,7 +310,7 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
             const unsigned long long r = atomicAdd(jn.n_done, 1ull);
             if ((long long)r < jn.done_cap) {
                 jn.done_pair[r] = p;
-                jn.done_ent[r * jn.stride] = row[0];
+                jn.done_ent[r * jn.stride] = e_lane;
             } else {
                 atomicOr(jn.overflow, 1u);
             }
@@ -324,9 +322,8 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
     if (lane < L) {
         // the atom placed at level `lane`
-        const int e = row[lane];
         const int k = lv[lane].k;
-        const SetEnt en = st.kent[e];
+        const SetEnt en = st.kent[e_lane];
         S.px[lane] = en.x;
         S.py[lane] = en.y;
         S.pz[lane] = en.z;
@@ -493,10 +490,27 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
     // in-box blocks are shared by `parts` waves
     const int parts = (n_in > 0 && n_in < nw) ? (int)min((long long)JOIN_SPLIT, nw / n_in) : 1;
     const long long items = n_in * parts;
-    for (long long it = (long long)blockIdx.x * JOIN_WAVES + w; it < items; it += nw) {
+    // the pair and this lane's entry of the next row are loaded while the
+    // current one is processed
+    const int lane = lane_id();
+    const int L = jn.level;
+    int p_next = 0, e_next = 0;
+    long long it = (long long)blockIdx.x * JOIN_WAVES + w;
+    if (it < items) {
+        const long long r = it / parts;
+        p_next = jn.in_pair[r];
+        if (lane < L) e_next = jn.in_ent[r * jn.stride + lane];
+    }
+    for (; it < items; it += nw) {
+        const int p = p_next, e = e_next;
+        if (it + nw < items) {
+            const long long r = (it + nw) / parts;
+            p_next = jn.in_pair[r];
+            if (lane < L) e_next = jn.in_ent[r * jn.stride + lane];
+        }
         wave_sync();
         const long long i = it / parts;
-        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, (int)(it - i * parts), parts, staged, dstaged);
+        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, p, e, (int)(it - i * parts), parts, staged, dstaged);
     }
     join_flush(jn, S[w], staged);
     join_flush_done(jn, S[w], dstaged);
