@@ -116,6 +116,24 @@ def test_synthetic_workloads(name, reorder):
     assert_same_hits(mol, tpls, got, want)
 
 
+@pytest.mark.parametrize("ignore_chain,best_match,max_candidates",
+                         [(None, False, 100000), ("residues", False, 3), (None, True, 100000), ("atoms", True, 2)])
+def test_library_style(ignore_chain, best_match, max_candidates):
+    # the library-scale shape at a size the oracle replays in seconds: 20
+    # chains, templates of 2-8 residues with symmetric mode-1 atom pairs,
+    # 2.5 A cutoff
+    rng = np.random.Generator(np.random.PCG64(21))
+    mol = synthetic.protein_with_atoms(rng, 12000, n_chains=20, mol_id="LIBT")
+    tpls = synthetic.template_set(rng, [mol], 150, 0.2, (2, 8), (2, 3), p_symmetric=0.2)
+    assert any(a.match_mode == 1 for t in tpls for a in t)
+    got = list(Jess(tpls).query(mol, 2.0, 2.5, 2.5, max_candidates=max_candidates, ignore_chain=ignore_chain,
+                                best_match=best_match))
+    want = oracle.hits(mol, tpls, 2.0, 2.5, 2.5, max_candidates=max_candidates, ignore_chain=ignore_chain,
+                       best_match=best_match)
+    assert len(want) > 50
+    assert_same_hits(mol, tpls, got, want)
+
+
 @pytest.mark.parametrize("ignore_chain", [None, "residues", "atoms"])
 def test_multichain_flexible(ignore_chain):
     rng = np.random.Generator(np.random.PCG64(5))
