@@ -296,8 +296,9 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
     params = _native.Params(*qargs, 0, 1)
 
     def device_pass():
-        n_ev = n_cm = 0
+        n_ev = 0
         stage = np.zeros(5)
+        jms, jl = 0.0, 0
         for ms in msets:
             h = ctypes.c_void_p()
             with ctx.lock:
@@ -308,18 +309,28 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
                     t = np.zeros(5)
                     _native._check(L.jam_result_timings(h, _native._ptr(t)))
                     stage += t
+                    x, nl = ctypes.c_double(0.0), ctypes.c_int32(0)
+                    _native._check(L.jam_result_join_kernels(h, ctypes.byref(x), ctypes.byref(nl)))
+                    jms += x.value
+                    jl += nl.value
                 finally:
                     L.jam_result_free(h)
-        return n_ev, stage
+        return n_ev, stage, jms, jl
 
     for _ in range(warmup):
         device_pass()
     _native.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        n_events, stage = device_pass()
+        n_events, stage, _, _ = device_pass()
     _native.synchronize()
     dev_s = (time.perf_counter() - t0) / steps
+    # the k_join launches timed with HIP events in one more pass
+    ctx.set_profiling(True)
+    try:
+        _, _, join_ms, join_launches = device_pass()
+    finally:
+        ctx.set_profiling(False)
     value = n_pairs / dev_s
     log(f"library: device-resident {dev_s * 1000:.1f} ms/step, {value:,.0f} queries/s, {n_events} events; "
         f"device ms: candidates {stage[0]:.1f} kd {stage[1]:.1f} join {stage[2]:.1f} super {stage[3]:.1f} "
@@ -363,6 +374,9 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
         "with_event_copy": {"value": n_pairs / copy_s, "unit": "queries/s", "seconds": copy_s},
         "drop_in": {"value": len(chunks[0]) * T / drop_s, "unit": "queries/s", "molecules": len(chunks[0]),
                     "hits": n_hits0, "seconds": drop_s},
+        "k_join_ms": join_ms,
+        "k_join_launches": join_launches,
+        "roofline": None,
         "cpu_baseline": None,
         "hit_set_parity": None,
     }
@@ -387,6 +401,28 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
                       f"{cpu_elapsed:.1f}s on {threads} threads (C restatement of the reference algorithm, "
                       f"gcc -O3)"}
         out["gpu_vs_cpu"] = value / cpu_rate
+        # algorithmic FP64 work of the join (8 flops per annulus test of an
+        # in-box candidate) from an instrumented oracle run on the first
+        # assembly x every tenth template, scaled to the workload
+        wt = tpls[::10]
+        w = oracle.work(mols[0], oracle.TplView(wt), kw["distance_cutoff"], kw["max_dynamic_distance"])
+        flops_join = 8.0 * w["inbox_tests"] * (T / len(wt)) * len(mols)
+        per_launch = flops_join / max(join_launches, 1)
+        avg_ms = join_ms / max(join_launches, 1)
+        achieved = per_launch / (avg_ms / 1000.0) / 1e12
+        out["roofline"] = {
+            "bound": "mfma", "kernel": "k_join", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+            "per_launch_flops": per_launch, "avg_launch_ms": avg_ms, "launches_per_step": join_launches,
+            "note": "FP64 vector ALU work of the exact annulus tests (8 flops x level x in-box candidate, "
+                    f"oracle run on the first assembly x {len(wt)} templates, scaled) against the 78.6 "
+                    "TFLOP/s FP64 peak"}
+        prof = os.path.join(ROOT, "profiles", "k_join.library-64.pmc.md")
+        if os.path.exists(prof):
+            for line in open(prof):
+                if line.startswith("traffic_bytes_per_launch:"):
+                    out["roofline"]["traffic"] = float(line.split(":")[1])
+                    out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
         sizes = [len(t) for t in tpls]
         sel = (first.flag == 1) & (first.pair < T)
         gpu_hc, gpu_dg = pair_digests(first.pair[sel], first.atoms[sel], sizes, T, T)
@@ -572,7 +608,7 @@ def main():
         out = {"metric": METRIC, "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-               "config": dict(d["config"], parallelism="1 GPU"), "roofline": None,
+               "config": dict(d["config"], parallelism="1 GPU"), "roofline": d["roofline"],
                "cpu_baseline": d["cpu_baseline"], "library": d}
         print(json.dumps(out), flush=True)
         return
