@@ -481,7 +481,10 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
                                                               DevJoin jn, QueryParams qp)
 {
     __shared__ JoinLds<MAXN> S[JOIN_WAVES];
-    const int w = threadIdx.x / JAM_WAVE;
+    // the wave index is uniform across the wave: saying so lets the compiler
+    // keep everything derived from the row (pair, template, level headers) in
+    // scalar registers and read it with scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / JAM_WAVE);
     long long n_in = (long long)*jn.n_in;
     if (n_in > jn.in_cap) n_in = jn.in_cap;
     int staged = 0, dstaged = 0;
