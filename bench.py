@@ -569,7 +569,7 @@ def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain-8k",
                     help="plain-2k | plain-8k | batch-4k | rmsd-grad-1m | library-64")
@@ -603,6 +603,9 @@ def main():
         print(json.dumps(out), flush=True)
         return
     if args.workload == "library-64":
+        # about 1.4 s per step on one MI355X
+        args.steps = min(args.steps, 3)
+        args.warmup = min(args.warmup, 1)
         d = measure_library(args.scale, args.seed, args.library_chunk, args.steps, args.warmup, args.cpu_seconds,
                             not args.no_cpu)
         out = {"metric": METRIC, "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": args.steps,
@@ -612,6 +615,8 @@ def main():
                "cpu_baseline": d["cpu_baseline"], "library": d}
         print(json.dumps(out), flush=True)
         return
+    if args.workload == "batch-4k":
+        args.steps = min(args.steps, 10)
     main_run = measure(args.workload, args.scale, args.seed, args.steps, args.warmup, args.cpu_seconds,
                        not args.no_cpu, None if args.workload != "batch-4k" else 32)
     batched = None
