@@ -109,7 +109,7 @@ struct jam_context {
     DBuf j_pair[2], j_ent[2], d_pair, d_ent, j_count;
     int64_t front_cap = 0, done_cap = 0;
     int front_stride = 0, done_stride = 0;
-    int join_grid[3] = {0, 0, 0};   // resident workgroups of k_join<8/16/32>
+    int join_grid[4] = {0, 0, 0, 0};   // resident workgroups of the join kernel <8/16/24/32>
     int kd_grid[12] = {};           // resident workgroups of the kd kernels
     // the exact kd pass of each size tier runs on a stream of its own, from
     // the moment the tier's fast pass has listed its tied sets, beside the
@@ -665,11 +665,13 @@ int resident_grid(K kernel, int threads, int device)
     return std::max(1, per_cu) * std::max(1, cus);
 }
 
+#define JOIN_KERNEL(M) (jam::k_join<M, JOIN_TW>)
+
 template <int MAXN>
 int join_grid(int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, jam::k_join<MAXN>, JOIN_WAVES * JAM_WAVE, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, JOIN_KERNEL(MAXN), JOIN_WAVES * JAM_WAVE, 0) !=
             hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return 0;
@@ -679,9 +681,12 @@ int join_grid(int device)
 // resident workgroups of the k_join variant for rows of `stride` entries
 int join_workgroups(jam_context *c, int stride)
 {
-    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : 2;
+    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : stride <= 24 ? 2 : 3;
     if (c->join_grid[v] == 0) {
-        c->join_grid[v] = v == 0 ? join_grid<8>(c->device) : v == 1 ? join_grid<16>(c->device) : join_grid<32>(c->device);
+        c->join_grid[v] = v == 0   ? join_grid<8>(c->device)
+                          : v == 1 ? join_grid<16>(c->device)
+                          : v == 2 ? join_grid<24>(c->device)
+                                   : join_grid<32>(c->device);
         if (c->join_grid[v] == 0) c->join_grid[v] = 1024;
     }
     return c->join_grid[v];
@@ -691,11 +696,12 @@ int join_workgroups(jam_context *c, int stride)
 void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, const DevTpls &dt, const DevSets &st,
                  const DevPairs &pr, const DevJoin &jn, const QueryParams &qp)
 {
-    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : 2;
+    const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : stride <= 24 ? 2 : 3;
     const dim3 g((unsigned)join_workgroups(c, stride)), b(JOIN_WAVES * JAM_WAVE);
-    if (v == 0) hipLaunchKernelGGL((jam::k_join<8>), g, b, 0, s, dm, dt, st, pr, jn, qp);
-    else if (v == 1) hipLaunchKernelGGL((jam::k_join<16>), g, b, 0, s, dm, dt, st, pr, jn, qp);
-    else hipLaunchKernelGGL((jam::k_join<32>), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    if (v == 0) hipLaunchKernelGGL(JOIN_KERNEL(8), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    else if (v == 1) hipLaunchKernelGGL(JOIN_KERNEL(16), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    else if (v == 2) hipLaunchKernelGGL(JOIN_KERNEL(24), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    else hipLaunchKernelGGL(JOIN_KERNEL(32), g, b, 0, s, dm, dt, st, pr, jn, qp);
 }
 
 }  // namespace

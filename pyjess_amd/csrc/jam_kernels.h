@@ -9,12 +9,12 @@
 //                  that order plus one bounding box per 64 entries.
 //   k_pairs        thread per (template, molecule) pair: level order and the
 //                  number of level-0 partial matches.
-//   k_join         one launch per join level, a wave per partial match of
-//                  level L-1: the candidates of level L
-//                  (patches/Scanner.c.patch:232-334) from the 64-entry blocks
-//                  of the kd-ordered set whose box meets the join box, all 64
-//                  lanes testing one candidate each against every annulus and
-//                  check(); children appended in any order.
+//   k_join         one launch per join level, a team of 16 lanes per partial
+//                  match of level L-1 (four per wave): the candidates of
+//                  level L (patches/Scanner.c.patch:232-334) from the
+//                  64-entry blocks of the kd-ordered set whose box meets the
+//                  join box, each lane testing one candidate against every
+//                  annulus and check(); children appended in any order.
 //   k_sortkey      complete matches sorted by (molecule, template in scan
 //                  order, level positions): the reference's depth-first
 //                  order and each match's rank.
@@ -223,10 +223,19 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
     jn.out_ent[i * jn.stride] = off0 + st.l2k[off0 + pos0];
 }
 
+#ifndef JOIN_WAVES
 #define JOIN_WAVES 4
-#define JOIN_STAGE 256
-#ifndef JOIN_UNROLL
-#define JOIN_UNROLL 1
+#endif
+#ifndef JOIN_STAGE
+#define JOIN_STAGE 128
+#endif
+// lanes per team (one partial match per team) and the candidate loads a
+// team keeps in flight
+#ifndef JOIN_TW
+#define JOIN_TW 16
+#endif
+#ifndef JOIN_TEAM_LOADS
+#define JOIN_TEAM_LOADS 2
 #endif
 // most waves sharing one row's blocks when the frontier is small
 #ifndef JOIN_SPLIT
@@ -237,18 +246,6 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
 #ifndef JOIN_SMALL
 #define JOIN_SMALL 4
 #endif
-
-template <int MAXN>
-struct JoinLds {
-    double px[MAXN], py[MAXN], pz[MAXN];
-    double r2lo[MAXN], r2hi[MAXN];
-    int pch[MAXN], pres[MAXN];
-    int tch[MAXN], tres[MAXN];
-    // children waiting to be written: parent row and the new entry, for the
-    // next frontier (sp, se) and for the complete matches (dp, de)
-    int sp[JOIN_STAGE], se[JOIN_STAGE];
-    int dp[JOIN_STAGE], de[JOIN_STAGE];
-};
 
 // Writes `staged` rows (parent row par[k] + the new entry ent[k]) to a row
 // buffer: one atomic for all of them, then the rows by all lanes.
@@ -262,75 +259,103 @@ __device__ static void join_write(const DevJoin &jn, const int *par, const int *
     if (lane == 0) b0 = atomicAdd(count, (unsigned long long)staged);
     const long long base = (long long)__shfl(b0, 0, JAM_WAVE);
     const int L = jn.level;
-    const int w = L + 1;
-    const int total = staged * w;
+    // the parent's L entries, then each row's new entry and pair (two
+    // passes, so that no load selects between LDS and global memory)
+    const int total = staged * L;
     for (int idx = lane; idx < total; idx += JAM_WAVE) {
-        const int k = idx / w, l = idx - k * w;
+        const int k = idx / L, l = idx - k * L;
+        const long long r = base + k;
+        if (r >= cap) continue;
+        out_ent[r * jn.stride + l] = jn.in_ent[(long long)par[k] * jn.stride + l];
+    }
+    for (int k = lane; k < staged; k += JAM_WAVE) {
         const long long r = base + k;
         if (r >= cap) {
-            if (l == 0) atomicOr(jn.overflow, 1u);
+            atomicOr(jn.overflow, 1u);
             continue;
         }
-        const int pr = par[k];
-        out_ent[r * jn.stride + l] = l < L ? jn.in_ent[(long long)pr * jn.stride + l] : ent[k];
-        if (l == 0) out_pair[r] = jn.in_pair[pr];
+        out_ent[r * jn.stride + L] = ent[k];
+        out_pair[r] = jn.in_pair[par[k]];
     }
     wave_sync();
     staged = 0;
 }
 
-template <int MAXN>
-__device__ static void join_flush(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
-{
-    join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
-}
+// ---------------------------------------------------------------------------
+// the join, one launch per level: a wave extends JAM_WAVE / TW partial
+// matches at once, one per team of TW lanes, so that the dependent loads of
+// several rows are in flight together; each team tests its row's candidates
+// TW at a time against every annulus and check()
+// ---------------------------------------------------------------------------
 
 template <int MAXN>
-__device__ static void join_flush_done(const DevJoin &jn, JoinLds<MAXN> &S, int &staged)
-{
-    join_write(jn, S.dp, S.de, staged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
-}
+struct JoinTeamLds {
+    double px[MAXN], py[MAXN], pz[MAXN];
+    double r2lo[MAXN], r2hi[MAXN];
+    int pch[MAXN], pres[MAXN];
+    int tch[MAXN], tres[MAXN];
+};
 
-// One wave extends partial i of level L-1 by the candidates of level L.
-template <int MAXN>
-__device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
-                                const DevJoin &jn, const QueryParams &qp, JoinLds<MAXN> &S, long long i,
-                                int p, int e_lane, int part, int parts, int &staged, int &dstaged)
+template <int MAXN, int TW>
+struct JoinWaveLds {
+    JoinTeamLds<MAXN> tm[JAM_WAVE / TW];
+    // children (sp, se) and complete matches (dp, de) waiting to be written:
+    // parent row and the new entry
+    int sp[JOIN_STAGE], se[JOIN_STAGE];
+    int dp[JOIN_STAGE], de[JOIN_STAGE];
+};
+
+template <int MAXN, int TW>
+__device__ static void join_team(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
+                                 const DevJoin &jn, const QueryParams &qp, JoinWaveLds<MAXN, TW> &S,
+                                 bool active, long long it, int parts, int &staged, int &dstaged)
 {
+    constexpr unsigned long long TMASK = (TW == 64) ? ~0ull : ((1ull << TW) - 1ull);
     const int lane = lane_id();
+    const int team = lane / TW, tl = lane % TW;
+    JoinTeamLds<MAXN> &M = S.tm[team];
     const int L = jn.level;
+    const long long i = active ? it / parts : 0;
+    const int part = active ? (int)(it - i * parts) : 0;
+    const int p = active ? jn.in_pair[i] : 0;
     const int T = tp.n_templates;
     const int t = p % T;
     const int tbase = tp.atom_off[t];
     const int n = tp.atom_off[t + 1] - tbase;
     const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
-    if (L >= n) {
+    if (active && L >= n && tl == 0 && part == 0) {
         // a one-atom template is complete at level 0
-        if (lane == 0 && part == 0) {
-            const unsigned long long r = atomicAdd(jn.n_done, 1ull);
-            if ((long long)r < jn.done_cap) {
-                jn.done_pair[r] = p;
-                jn.done_ent[r * jn.stride] = e_lane;
-            } else {
-                atomicOr(jn.overflow, 1u);
-            }
+        const unsigned long long r = atomicAdd(jn.n_done, 1ull);
+        if ((long long)r < jn.done_cap) {
+            jn.done_pair[r] = p;
+            jn.done_ent[r * jn.stride] = jn.in_ent[i * jn.stride];
+        } else {
+            atomicOr(jn.overflow, 1u);
         }
-        return;
     }
-    const PairLevel hL = lv[L];
-    const int kL = hL.k, setL = hL.set, cntL = hL.cnt, offL = hL.off;
+    const bool live = active && L < n;
+    int kL = 0, setL = 0, cntL = 0, offL = 0;
+    if (live) {
+        const PairLevel hL = lv[L];
+        kL = hL.k;
+        setL = hL.set;
+        cntL = hL.cnt;
+        offL = hL.off;
+    }
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
-    if (lane < L) {
-        // the atom placed at level `lane`
-        const int k = lv[lane].k;
-        const SetEnt en = st.kent[e_lane];
-        S.px[lane] = en.x;
-        S.py[lane] = en.y;
-        S.pz[lane] = en.z;
-        S.pch[lane] = en.ch;
-        S.pres[lane] = en.res;
-        S.tch[lane] = tp.chain[tbase + k];
-        S.tres[lane] = tp.resseq[tbase + k];
+    for (int j = tl; j < L; j += TW) {
+        if (!live) break;
+        // the atom placed at level j
+        const int e = jn.in_ent[i * jn.stride + j];
+        const int k = lv[j].k;
+        const SetEnt en = st.kent[e];
+        M.px[j] = en.x;
+        M.py[j] = en.y;
+        M.pz[j] = en.z;
+        M.pch[j] = en.ch;
+        M.pres[j] = en.res;
+        M.tch[j] = tp.chain[tbase + k];
+        M.tres[j] = tp.resseq[tbase + k];
         // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
         // Annulus_reuse patches/Annulus.c.patch:107-140
         const double d = tp.dist[tp.dist_off[t] + k * n + kL];
@@ -345,178 +370,162 @@ __device__ static void join_one(const DevMols &ms, const DevTpls &tp, const DevS
         }
         if (a < 0.0) a = 0.0;
         if (b < 0.0) b = 0.0;
-        S.r2lo[lane] = a * a;
-        S.r2hi[lane] = b * b;
-        lo[0] = en.x - b;
-        lo[1] = en.y - b;
-        lo[2] = en.z - b;
-        hi[0] = en.x + b;
-        hi[1] = en.y + b;
-        hi[2] = en.z + b;
+        M.r2lo[j] = a * a;
+        M.r2hi[j] = b * b;
+        lo[0] = fmax(lo[0], en.x - b);
+        lo[1] = fmax(lo[1], en.y - b);
+        lo[2] = fmax(lo[2], en.z - b);
+        hi[0] = fmin(hi[0], en.x + b);
+        hi[1] = fmin(hi[1], en.y + b);
+        hi[2] = fmin(hi[2], en.z + b);
     }
-    const int nblk = (cntL + JAM_WAVE - 1) / JAM_WAVE;
-    if (nblk > JOIN_SMALL) {
-        // join box (Join_computeBox, patches/Join.c.patch:154-196)
-        for (int sh = 32; sh >= 1; sh >>= 1) {
+    // join box of the team (Join_computeBox, patches/Join.c.patch:154-196)
+    for (int sh = TW / 2; sh >= 1; sh >>= 1) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
-                hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
-            }
+        for (int d = 0; d < 3; d++) {
+            lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
+            hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
         }
     }
-    const int tchL = tp.chain[tbase + kL], tresL = tp.resseq[tbase + kL];
+    const int tchL = live ? tp.chain[tbase + kL] : 0, tresL = live ? tp.resseq[tbase + kL] : 0;
     wave_sync();
     const int ignore = qp.ignore_chain;
     const long long bbase = (long long)offL / JAM_WAVE + setL;
     const bool last = (L == n - 1);
-    int seen = 0;   // in-box blocks met so far; this wave takes every parts-th one
-    for (int g = 0; g * JAM_WAVE < nblk; g++) {
+    const int nblk = live ? (cntL + JAM_WAVE - 1) / JAM_WAVE : 0;
+    const int ngroups = (nblk + TW - 1) / TW;
+    int seen = 0;   // in-box blocks met so far; this team takes every parts-th one
+    for (int g = 0;; g++) {
+        if (ballot(g < ngroups) == 0ull) break;
         // blocks of this group whose box meets the join box (all of them for
         // small sets: the test is a prefilter only)
-        unsigned long long bm;
-        if (nblk <= JOIN_SMALL) {
-            bm = (nblk >= 64) ? ~0ull : ((1ull << nblk) - 1ull);
-        } else {
-            const int b = g * JAM_WAVE + lane;
-            bool ok = false;
-            if (b < nblk) {
+        bool inb = false;
+        const int b = g * TW + tl;
+        if (g < ngroups && b < nblk) {
+            if (nblk <= JOIN_SMALL) {
+                inb = true;
+            } else {
                 const double *bb = st.bbox + 6 * (bbase + b);
-                ok = !(bb[0] > hi[0] || lo[0] > bb[3] || bb[1] > hi[1] || lo[1] > bb[4] || bb[2] > hi[2] ||
-                       lo[2] > bb[5]);
+                inb = !(bb[0] > hi[0] || lo[0] > bb[3] || bb[1] > hi[1] || lo[1] > bb[4] || bb[2] > hi[2] ||
+                        lo[2] > bb[5]);
             }
-            bm = ballot(ok);
         }
+        unsigned long long bm = (ballot(inb) >> (team * TW)) & TMASK;
         if (parts > 1) {
             // keep the in-box blocks whose running index is part mod parts
             unsigned long long keep = 0ull, rest = bm;
             while (rest) {
-                const int b = first_bit(rest);
+                const int x = first_bit(rest);
                 rest &= rest - 1;
-                if (seen % parts == part) keep |= 1ull << b;
+                if (seen % parts == part) keep |= 1ull << x;
                 seen++;
             }
             bm = keep;
         }
-        while (bm) {
-            // up to JOIN_UNROLL blocks per round, their loads in flight together
-            int blk[JOIN_UNROLL];
-            int nb = 0;
+        for (;;) {
+            const bool has = bm != 0ull;
+            if (ballot(has) == 0ull) break;
+            int blk = 0;
+            if (has) {
+                blk = g * TW + first_bit(bm);
+                bm &= bm - 1;
+            }
+            // the block's 64 entries, TW at a time, JOIN_TEAM_LOADS loads in
+            // flight together
+            constexpr int NU = JAM_WAVE / TW;
+            constexpr int NL = JOIN_TEAM_LOADS < NU ? JOIN_TEAM_LOADS : NU;
+            SetEnt cand[NL];
 #pragma unroll
-            for (int u = 0; u < JOIN_UNROLL; u++) {
-                blk[u] = -1;
-                if (bm) {
-                    blk[u] = g * JAM_WAVE + first_bit(bm);
-                    bm &= bm - 1;
-                    nb++;
+            for (int u = 0; u < NU; u++) {
+                if (u % NL == 0) {
+#pragma unroll
+                    for (int v = 0; v < NL; v++) {
+                        const int q = blk * JAM_WAVE + (u + v) * TW + tl;
+                        if (has && u + v < NU && q < cntL) cand[v] = st.kent[offL + q];
+                    }
                 }
-            }
-            SetEnt cand[JOIN_UNROLL];
-#pragma unroll
-            for (int u = 0; u < JOIN_UNROLL; u++) {
-                const int q = blk[u] * JAM_WAVE + lane;
-                if (blk[u] >= 0 && q < cntL) cand[u] = st.kent[offL + q];
-            }
-#pragma unroll
-            for (int u = 0; u < JOIN_UNROLL; u++) {
-                if (u >= nb) break;
-                const int q = blk[u] * JAM_WAVE + lane;
-                bool ok = q < cntL;
-                const SetEnt c = cand[u];
+                const int q = blk * JAM_WAVE + u * TW + tl;
+                bool ok = has && q < cntL;
+                const SetEnt c = cand[u % NL];
                 for (int j = 0; j < L; j++) {
                     // most in-box candidates fail an early annulus: stop as
                     // soon as no lane of the wave is still in
                     if (ballot(ok) == 0ull) break;
                     if (ok) {
                         // _Annulus_po: squared distance summed x, y, z from 0.0
-                        double tx = S.px[j] - c.x, ty = S.py[j] - c.y, tz = S.pz[j] - c.z;
+                        double tx = M.px[j] - c.x, ty = M.py[j] - c.y, tz = M.pz[j] - c.z;
                         double s2 = 0.0;
                         s2 += tx * tx;
                         s2 += ty * ty;
                         s2 += tz * tz;
-                        if (s2 < S.r2lo[j] || s2 > S.r2hi[j]) ok = false;
+                        if (s2 < M.r2lo[j] || s2 > M.r2hi[j]) ok = false;
                         // TessTemplate_check
-                        const bool same_chain = S.pch[j] == c.ch;
-                        const bool tsame_chain = S.tch[j] == tchL;
+                        const bool same_chain = M.pch[j] == c.ch;
+                        const bool tsame_chain = M.tch[j] == tchL;
                         if (ignore == 0) {
                             if (same_chain != tsame_chain) ok = false;
                             if (!same_chain) continue;
                         }
-                        const bool same_res = S.pres[j] == c.res;
-                        const bool tsame_res = S.tres[j] == tresL;
+                        const bool same_res = M.pres[j] == c.res;
+                        const bool tsame_res = M.tres[j] == tresL;
                         if (same_res != tsame_res) ok = false;
                         if (ignore != 2 && same_res && !same_chain) ok = false;
                     }
                 }
-                const unsigned long long mask = ballot(ok);
-                if (mask == 0ull) continue;
-                const int cnt = __popcll(mask);
-                const int r = rank_below(mask);
-                if (last) {
-                    // complete matches, staged like the children
-                    if (dstaged + cnt > JOIN_STAGE) join_flush_done(jn, S, dstaged);
-                    if (ok) {
-                        S.dp[dstaged + r] = (int)i;
-                        S.de[dstaged + r] = offL + q;
-                    }
-                    dstaged += cnt;
-                } else {
-                    if (staged + cnt > JOIN_STAGE) join_flush(jn, S, staged);
-                    if (ok) {
+                const unsigned long long cm = ballot(ok && !last), dm = ballot(ok && last);
+                if (cm) {
+                    const int cnt = __popcll(cm);
+                    if (staged + cnt > JOIN_STAGE) join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap,
+                                                              jn.out_pair, jn.out_ent);
+                    if (ok && !last) {
+                        const int r = rank_below(cm);
                         S.sp[staged + r] = (int)i;
                         S.se[staged + r] = offL + q;
                     }
                     staged += cnt;
                 }
+                if (dm) {
+                    const int cnt = __popcll(dm);
+                    if (dstaged + cnt > JOIN_STAGE) join_write(jn, S.dp, S.de, dstaged, jn.n_done, jn.done_cap,
+                                                               jn.done_pair, jn.done_ent);
+                    if (ok && last) {
+                        const int r = rank_below(dm);
+                        S.dp[dstaged + r] = (int)i;
+                        S.de[dstaged + r] = offL + q;
+                    }
+                    dstaged += cnt;
+                }
             }
         }
     }
+    wave_sync();
 }
 
-// One launch per join level: waves loop over the input rows (the grid is what
-// stays resident on the device); the output frontier's counter is also the
-// next level's input size, so levels run back to back without the host.
-template <int MAXN>
+// Waves loop over the input rows (the grid is what stays resident on the
+// device); the output frontier's counter is also the next level's input
+// size, so levels run back to back without the host.
+template <int MAXN, int TW>
 __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
                                                               DevJoin jn, QueryParams qp)
 {
-    __shared__ JoinLds<MAXN> S[JOIN_WAVES];
-    // the wave index is uniform across the wave: saying so lets the compiler
-    // keep everything derived from the row (pair, template, level headers) in
-    // scalar registers and read it with scalar loads
+    constexpr int NTEAM = JAM_WAVE / TW;
+    __shared__ JoinWaveLds<MAXN, TW> SW[JOIN_WAVES];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / JAM_WAVE);
+    const int team = lane_id() / TW;
     long long n_in = (long long)*jn.n_in;
     if (n_in > jn.in_cap) n_in = jn.in_cap;
     int staged = 0, dstaged = 0;
-    const long long nw = (long long)gridDim.x * JOIN_WAVES;
-    // a frontier smaller than the resident waves is spread thinner: each row's
-    // in-box blocks are shared by `parts` waves
-    const int parts = (n_in > 0 && n_in < nw) ? (int)min((long long)JOIN_SPLIT, nw / n_in) : 1;
+    const long long slots = (long long)gridDim.x * JOIN_WAVES * NTEAM;
+    // a frontier smaller than the team slots is spread thinner: each row's
+    // in-box blocks are shared by `parts` teams
+    const int parts = (n_in > 0 && n_in < slots) ? (int)min((long long)JOIN_SPLIT, slots / n_in) : 1;
     const long long items = n_in * parts;
-    // the pair and this lane's entry of the next row are loaded while the
-    // current one is processed
-    const int lane = lane_id();
-    const int L = jn.level;
-    int p_next = 0, e_next = 0;
-    long long it = (long long)blockIdx.x * JOIN_WAVES + w;
-    if (it < items) {
-        const long long r = it / parts;
-        p_next = jn.in_pair[r];
-        if (lane < L) e_next = jn.in_ent[r * jn.stride + lane];
+    for (long long base = ((long long)blockIdx.x * JOIN_WAVES + w) * NTEAM; base < items; base += slots) {
+        const long long it = base + team;
+        join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, staged, dstaged);
     }
-    for (; it < items; it += nw) {
-        const int p = p_next, e = e_next;
-        if (it + nw < items) {
-            const long long r = (it + nw) / parts;
-            p_next = jn.in_pair[r];
-            if (lane < L) e_next = jn.in_ent[r * jn.stride + lane];
-        }
-        wave_sync();
-        const long long i = it / parts;
-        join_one<MAXN>(ms, tp, st, pr, jn, qp, S[w], i, p, e, (int)(it - i * parts), parts, staged, dstaged);
-    }
-    join_flush(jn, S[w], staged);
-    join_flush_done(jn, S[w], dstaged);
+    join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
+    join_write(jn, SW[w].dp, SW[w].de, dstaged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
 }
 
 // ---------------------------------------------------------------------------
