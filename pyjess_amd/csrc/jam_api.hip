@@ -844,8 +844,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_EXACT_THREADS, d);
             }
             auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
-            // largest tier first: its exact pass, the longest, starts earliest
-            for (int k = KD_TIERS - 1; k >= 0; k--) {
+            auto launch_fast = [&](int k) {
                 const size_t o = (size_t)k * ns;
                 switch (k) {
                 case 0:
@@ -873,9 +872,9 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                                        nb + 5, tied + o, nt + 5);
                     break;
                 }
-                hipStream_t x = c->kd_side[k];
-                HIP_TRY(hipEventRecord(c->kd_fork[k], s));
-                HIP_TRY(hipStreamWaitEvent(x, c->kd_fork[k], 0));
+            };
+            auto launch_exact = [&](int k, hipStream_t x) {
+                const size_t o = (size_t)k * ns;
                 switch (k) {
                 case 0:
                     hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, tied, nt);
@@ -898,7 +897,14 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                                        tied + o, nt + 5);
                     break;
                 }
-                HIP_TRY(hipEventRecord(c->kd_joined[k], x));
+            };
+            // largest tier first: its exact pass, the longest, starts earliest
+            for (int k = KD_TIERS - 1; k >= 0; k--) {
+                launch_fast(k);
+                HIP_TRY(hipEventRecord(c->kd_fork[k], s));
+                HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
+                launch_exact(k, c->kd_side[k]);
+                HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
             }
             for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
             HIP_TRY(hipGetLastError());
