@@ -308,7 +308,8 @@ struct JoinWaveLds {
 template <int MAXN, int TW>
 __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                  const DevJoin &jn, const QueryParams &qp, JoinWaveLds<MAXN, TW> &S,
-                                 bool active, long long it, int parts, int &staged, int &dstaged)
+                                 bool active, long long it, int parts, int p, const int (&ent)[(MAXN + TW - 1) / TW],
+                                 int &staged, int &dstaged)
 {
     constexpr unsigned long long TMASK = (TW == 64) ? ~0ull : ((1ull << TW) - 1ull);
     const int lane = lane_id();
@@ -317,7 +318,6 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
     const int L = jn.level;
     const long long i = active ? it / parts : 0;
     const int part = active ? (int)(it - i * parts) : 0;
-    const int p = active ? jn.in_pair[i] : 0;
     const int T = tp.n_templates;
     const int t = p % T;
     const int tbase = tp.atom_off[t];
@@ -343,10 +343,12 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         offL = hL.off;
     }
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
-    for (int j = tl; j < L; j += TW) {
-        if (!live) break;
+#pragma unroll
+    for (int jj = 0; jj < (MAXN + TW - 1) / TW; jj++) {
+        const int j = tl + jj * TW;
+        if (!live || j >= L) break;
         // the atom placed at level j
-        const int e = jn.in_ent[i * jn.stride + j];
+        const int e = ent[jj];
         const int k = lv[j].k;
         const SetEnt en = st.kent[e];
         M.px[j] = en.x;
@@ -520,9 +522,34 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevT
     // in-box blocks are shared by `parts` teams
     const int parts = (n_in > 0 && n_in < slots) ? (int)min((long long)JOIN_SPLIT, slots / n_in) : 1;
     const long long items = n_in * parts;
-    for (long long base = ((long long)blockIdx.x * JOIN_WAVES + w) * NTEAM; base < items; base += slots) {
+    // the pair and this lane's entries of the team's next row are loaded
+    // while the current one is processed
+    constexpr int NE = (MAXN + TW - 1) / TW;
+    const int tl = lane_id() % TW;
+    const int L = jn.level;
+    int p_next = 0, e_next[NE];
+    auto fetch = [&](long long it) {
+#pragma unroll
+        for (int jj = 0; jj < NE; jj++) e_next[jj] = 0;
+        if (it >= items) return;
+        const long long r = it / parts;
+        p_next = jn.in_pair[r];
+#pragma unroll
+        for (int jj = 0; jj < NE; jj++) {
+            const int j = tl + jj * TW;
+            if (j < L) e_next[jj] = jn.in_ent[r * jn.stride + j];
+        }
+    };
+    long long base = ((long long)blockIdx.x * JOIN_WAVES + w) * NTEAM;
+    if (base < items) fetch(base + team);
+    for (; base < items; base += slots) {
         const long long it = base + team;
-        join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, staged, dstaged);
+        const int p = p_next;
+        int e[NE];
+#pragma unroll
+        for (int jj = 0; jj < NE; jj++) e[jj] = e_next[jj];
+        if (base + slots < items) fetch(base + slots + team);
+        join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, p, e, staged, dstaged);
     }
     join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
     join_write(jn, SW[w].dp, SW[w].de, dstaged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
