@@ -102,8 +102,8 @@ struct jam_context {
     hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
     bool profile = false;       // record xev (each record costs stream time)
     // per-query workspace
-    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, total, maxcnt, kscr, kd_bucket, kd_tied, kd_count;
-    DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair, n_items;
+    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, kscr, kd_bucket, kd_tied, kd_count;
+    DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair;
     // join: two frontiers (pair + stride entries per row), complete matches,
     // counters (rows per level, complete matches, overflow flag)
     DBuf j_pair[2], j_ent[2], d_pair, d_ent, j_count;
@@ -119,7 +119,7 @@ struct jam_context {
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
     int64_t sort_cap = 0;
-    DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms, e_count;
+    DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms;
     DBuf scan_tmp, scan_tmp2;
     int64_t set_cap = 0, blk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
@@ -132,6 +132,9 @@ struct jam_context {
     // device-to-host copies into one half while the other is unpacked
     void *hstage = nullptr;
     hipEvent_t stage_ev[2] = {};
+    // pinned host words the device's counters are read back into
+    unsigned long long *hsmall = nullptr;
+    int64_t hsmall_cap = 0;
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
 };
@@ -303,6 +306,7 @@ extern "C" void jam_context_free(jam_context *ctx)
     for (auto &ev : ctx->xev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->hstage) (void)hipHostFree(ctx->hstage);
+    if (ctx->hsmall) (void)hipHostFree(ctx->hsmall);
     for (auto &ev : ctx->stage_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (int k = 0; k < KD_TIERS; k++) {
@@ -526,8 +530,6 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         c->blk_cap = std::max(blk, blk_need);
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
     }
-    HIP_TRY(c->total.reserve(sizeof(int64_t)));
-    HIP_TRY(c->maxcnt.reserve(sizeof(int)));
     HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * 2 * KD_TIERS));
     return JAM_OK;
 }
@@ -539,11 +541,23 @@ int32_t grow_pairs(jam_context *c, int n_pairs)
     HIP_TRY(c->p_lvl.reserve(sizeof(PairLevel) * (size_t)JAM_MAX_LEVELS * n_pairs));
     HIP_TRY(c->p_nchunk.reserve(sizeof(int) * (size_t)n_pairs));
     HIP_TRY(c->p_base.reserve(sizeof(long long) * (size_t)n_pairs));
-    HIP_TRY(c->p_cm.reserve(sizeof(long long) * (size_t)n_pairs));
+    HIP_TRY(c->p_cm.reserve(sizeof(long long) * ((size_t)n_pairs + 1)));   // [0]: event count
     HIP_TRY(c->p_first.reserve(sizeof(long long) * (size_t)n_pairs));
     HIP_TRY(c->p_n1.reserve(sizeof(int) * (size_t)n_pairs));
-    HIP_TRY(c->n_items.reserve(sizeof(long long)));
     c->npairs_cap = n_pairs;
+    return JAM_OK;
+}
+
+// pinned host words for counter read-backs
+int32_t grow_hsmall(jam_context *c, int64_t words)
+{
+    if (words <= c->hsmall_cap) return JAM_OK;
+    words = std::max<int64_t>(words, 1024);
+    if (c->hsmall) HIP_TRY(hipHostFree(c->hsmall));
+    c->hsmall = nullptr;
+    c->hsmall_cap = 0;
+    HIP_TRY(hipHostMalloc((void **)&c->hsmall, sizeof(unsigned long long) * (size_t)words, hipHostMallocDefault));
+    c->hsmall_cap = words;
     return JAM_OK;
 }
 
@@ -620,7 +634,6 @@ int32_t grow_events(jam_context *c, int64_t cap, int stride)
         c->event_cap = cap;
         c->event_stride = stride;
     }
-    HIP_TRY(c->e_count.reserve(sizeof(unsigned long long)));
     return JAM_OK;
 }
 
@@ -642,7 +655,7 @@ __global__ void k_total_items(const int *cnt, const long long *off, long long n,
 __global__ void k_items_any(DevPairs pr)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pr.n_pairs) return;
+    if (p >= pr.n_pairs || *pr.n_items > pr.items_cap) return;   // does not fit: grown and run again
     const int c = pr.nchunk[p];
     const long long b = pr.item_base[p];
     for (int i = 0; i < c; i++) pr.item_pair[b + i] = p;
@@ -706,6 +719,16 @@ void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, c
 
 }  // namespace
 
+// One device array of counters, read back in one copy after the join: rows
+// per join level, complete matches, the overflow flag, then the sizes phase A
+// finds (candidate-set entries, level-0 items, largest set).
+constexpr int CNT_DONE = JAM_MAX_LEVELS;
+constexpr int CNT_OVF = JAM_MAX_LEVELS + 1;
+constexpr int CNT_TOTAL = JAM_MAX_LEVELS + 2;
+constexpr int CNT_ITEMS = JAM_MAX_LEVELS + 3;
+constexpr int CNT_MAXCNT = JAM_MAX_LEVELS + 4;
+constexpr int CNT_WORDS = JAM_MAX_LEVELS + 5;
+
 extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_molecules *ms,
                              const jam_params *p, jam_result **out)
 {
@@ -744,6 +767,9 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     if ((rc = grow_sets(c, std::max<int64_t>(c->set_cap, 1 << 16), n_sets)) != JAM_OK) return rc;
     if ((rc = grow_events(c, std::max<int64_t>(c->event_cap, 1 << 12), stride)) != JAM_OK) return rc;
     if ((rc = grow_pairs(c, n_pairs)) != JAM_OK) return rc;
+    if ((rc = grow_items(c, std::max<int64_t>(c->items_cap, 4096))) != JAM_OK) return rc;
+    if ((rc = grow_hsmall(c, std::max<int64_t>(CNT_WORDS, (int64_t)n_pairs + 1))) != JAM_OK) return rc;
+    HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * CNT_WORDS));
     DevMols dm = ms->dev();
     DevTpls dt = tp->dev();
     size_t scan_sets = 0, scan_pairs = 0, scan_max = 0;
@@ -768,11 +794,13 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         st.kent = c->kent.as<SetEnt>();
         st.katom = c->katom.as<int>();
         st.bbox = c->bbox.as<double>();
-        st.total = c->total.as<int64_t>();
-        st.maxcnt = c->maxcnt.as<int>();
+        unsigned long long *cnts = c->j_count.as<unsigned long long>();
+        st.total = (int64_t *)(cnts + CNT_TOTAL);
+        st.maxcnt = (int *)(cnts + CNT_MAXCNT);
         DevPairs pr;
         pr.n_pairs = n_pairs;
-        pr.n_items = 0;
+        pr.n_items = (const long long *)(cnts + CNT_ITEMS);
+        pr.items_cap = c->items_cap;
         pr.ord = c->p_ord.as<unsigned char>();
         pr.lvl = c->p_lvl.as<PairLevel>();
         pr.nchunk = c->p_nchunk.as<int>();
@@ -793,22 +821,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_pairs, pr.nchunk, pr.item_base, n_pairs,
                                                  s));
         hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, pr.nchunk, pr.item_base, (long long)n_pairs,
-                           c->n_items.as<long long>());
-        int64_t totals[2] = {0, 0};
-        int maxcnt = 0;
-        HIP_TRY(hipMemcpyAsync(&totals[0], st.total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&totals[1], c->n_items.ptr, sizeof(long long), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&maxcnt, st.maxcnt, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (totals[0] >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 candidate-set entries in one query");
-        if (totals[0] > c->set_cap) {
-            if ((rc = grow_sets(c, totals[0], n_sets)) != JAM_OK) return rc;
-            continue;
-        }
-        const int64_t n0 = totals[1];
-        if ((rc = grow_items(c, n0)) != JAM_OK) return rc;
-        pr.n_items = n0;
-        pr.item_pair = c->i_pair.as<int>();
+                           (long long *)(cnts + CNT_ITEMS));
+        // no read-back here: the buffers keep the sizes of earlier queries,
+        // every kernel below checks the sizes on the device, and the host
+        // reads them with the join's counters, growing and running the query
+        // again when something did not fit
         hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
@@ -914,13 +931,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         // ---- the join: one launch per level, back to back; the counters
         // stay on the device until the end
         const int levels = std::max(1, stride - 1);
-        if ((rc = grow_front(c, std::max<int64_t>(c->front_cap, n0), stride)) != JAM_OK) return rc;
+        if ((rc = grow_front(c, std::max<int64_t>(c->front_cap, 1 << 14), stride)) != JAM_OK) return rc;
         if ((rc = grow_done(c, std::max<int64_t>(c->done_cap, 1 << 14), stride)) != JAM_OK) return rc;
-        HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * (JAM_MAX_LEVELS + 2)));
-        unsigned long long *fcount = c->j_count.as<unsigned long long>();   // rows per level
-        unsigned long long *n_done_dev = fcount + JAM_MAX_LEVELS;
-        unsigned int *overflow = (unsigned int *)(fcount + JAM_MAX_LEVELS + 1);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, fcount, (long long)(JAM_MAX_LEVELS + 2));
+        unsigned long long *fcount = cnts;   // rows per level
+        unsigned long long *n_done_dev = cnts + CNT_DONE;
+        unsigned int *overflow = (unsigned int *)(cnts + CNT_OVF);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, fcount, (long long)CNT_TOTAL);
         DevJoin jn{};
         jn.stride = c->front_stride;
         jn.n_done = n_done_dev;
@@ -933,14 +949,15 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             jn.done_pair = c->d_pair.as<int>();
             jn.done_ent = c->d_ent.as<int>();
         }
-        if (n0 > 0) {
+        {
             hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
             jn.level = 0;
             jn.n_out = fcount;
             jn.out_cap = c->front_cap;
             jn.out_pair = c->j_pair[0].as<int>();
             jn.out_ent = c->j_ent[0].as<int>();
-            hipLaunchKernelGGL(jam::k_level0, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, st, pr, jn);
+            const unsigned l0_blocks = (unsigned)std::min<int64_t>(4096, (c->items_cap + 255) / 256);
+            hipLaunchKernelGGL(jam::k_level0, dim3(l0_blocks), dim3(256), 0, s, st, pr, jn);
             HIP_TRY(hipGetLastError());
             for (int L = 1; L <= levels; L++) {
                 const int bi = (L - 1) & 1, bo = L & 1;
@@ -959,11 +976,32 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 if (c->profile) HIP_TRY(hipEventRecord(c->xev[2 * (L - 1) + 1], s));
             }
         }
-        unsigned long long jc[JAM_MAX_LEVELS + 2];
-        HIP_TRY(hipMemcpyAsync(jc, fcount, sizeof(jc), hipMemcpyDeviceToHost, s));
+        unsigned long long *jc = c->hsmall;
+        HIP_TRY(hipMemcpyAsync(jc, cnts, sizeof(unsigned long long) * CNT_WORDS, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        const unsigned int ovf = (unsigned int)(jc[JAM_MAX_LEVELS + 1] & 0xFFFFFFFFull);
-        const int64_t n_done = (int64_t)jc[JAM_MAX_LEVELS];
+        const int64_t total = (int64_t)jc[CNT_TOTAL];
+        const int64_t n0 = (int64_t)jc[CNT_ITEMS];
+        int maxcnt = 0;
+        std::memcpy(&maxcnt, jc + CNT_MAXCNT, sizeof(int));
+        if (total >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 candidate-set entries in one query");
+        bool regrow = false;
+        if (total > c->set_cap) {
+            if ((rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
+            regrow = true;
+        }
+        if (n0 > c->items_cap) {
+            if ((rc = grow_items(c, n0)) != JAM_OK) return rc;
+            regrow = true;
+        }
+        if (n0 > c->front_cap) {
+            if ((rc = grow_front(c, std::max<int64_t>(2 * c->front_cap, n0), stride)) != JAM_OK) return rc;
+            regrow = true;
+        }
+        if (regrow) continue;
+        const unsigned int ovf = (unsigned int)(jc[CNT_OVF] & 0xFFFFFFFFull);
+        const int64_t n_done = (int64_t)jc[CNT_DONE];
+        unsigned long long rows[JAM_MAX_LEVELS];   // the read-back area is reused below
+        std::memcpy(rows, jc, sizeof(rows));
         if (ovf || n_done > c->done_cap) {
             // rows that did not fit: grow to what was seen (deeper levels saw
             // truncated inputs, so this may take another round)
@@ -1029,11 +1067,15 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         ev.c0 = c->e_c0.as<double>();
         ev.c1 = c->e_c1.as<double>();
         ev.atoms = c->e_atoms.as<int>();
-        ev.count = c->e_count.as<unsigned long long>();
+        // the event count, then the complete matches of every pair: read back
+        // in one copy
+        unsigned long long *pcm = (unsigned long long *)c->p_cm.ptr;
+        ev.count = pcm;
         // marks and their scan reuse the (now free) 64-bit sort keys
         int *sel = c->s_key[0].as<int>();
         int *pos = c->s_key[1].as<int>();
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, ev.count, 1ll);
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min(1024, (n_pairs + 256) / 256)), dim3(256), 0, s, pcm,
+                           (long long)n_pairs + 1);
         if (n_done > 0) {
             const long long *first = c->p_first.as<long long>();
             hipLaunchKernelGGL(jam::k_first, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
@@ -1049,18 +1091,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[4], s));
-        unsigned long long ne_dev = 0;
-        HIP_TRY(hipMemcpyAsync(&ne_dev, ev.count, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
         if (n_done > 0) {
-            hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min(1024, (n_pairs + 255) / 256)), dim3(256), 0, s,
-                               (unsigned long long *)c->p_cm.ptr, (long long)n_pairs);
             hipLaunchKernelGGL(jam::k_pair_matches, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
-                               (const long long *)c->p_first.as<long long>(), c->p_cm.as<long long>());
+                               (const long long *)c->p_first.as<long long>(), (long long *)(pcm + 1));
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(R->pair_cm.data(), c->p_cm.ptr, sizeof(long long) * n_pairs,
-                                   hipMemcpyDeviceToHost, s));
         }
+        HIP_TRY(hipMemcpyAsync(c->hsmall, pcm, sizeof(unsigned long long) * ((size_t)n_pairs + 1),
+                               hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        const unsigned long long ne_dev = c->hsmall[0];
+        std::memcpy(R->pair_cm.data(), c->hsmall + 1, sizeof(long long) * (size_t)n_pairs);
         if ((int64_t)ne_dev > c->event_cap || (int64_t)ne_dev > n_done)
             return fail(JAM_EDEVICE, "event count exceeds the complete matches");
         float ms_k[5] = {0, 0, 0, 0, 0};
@@ -1070,7 +1110,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipEventElapsedTime(&ms_k[3], c->ev[3], c->ev[4]));
         HIP_TRY(hipEventElapsedTime(&ms_k[4], c->ev[0], c->ev[4]));
         for (int k = 0; k < 5; k++) R->ms[k] = ms_k[k];
-        const int n_launched = n0 > 0 ? levels : 0;
+        const int n_launched = levels;
         R->expand_ms = c->profile ? 0.0 : -1.0;
         for (int q = 0; c->profile && q < n_launched; q++) {
             float x = 0.f;
@@ -1079,7 +1119,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         }
         R->expand_launches = c->profile ? n_launched : 0;
         for (int q = 0; q < JAM_MAX_LEVELS; q++) R->level_partials[q] = 0;
-        for (int L = 1; L < JAM_MAX_LEVELS && L <= n_launched; L++) R->level_partials[L] = (int64_t)jc[L - 1];
+        for (int L = 1; L < JAM_MAX_LEVELS && L <= n_launched; L++) R->level_partials[L] = (int64_t)rows[L - 1];
         R->n_events = (int64_t)ne_dev;
         HIP_TRY(hipStreamSynchronize(s));
         *out = R.release();

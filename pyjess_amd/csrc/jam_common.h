@@ -91,7 +91,8 @@ struct PairLevel {
 
 struct DevPairs {
     int n_pairs;
-    long long n_items;
+    const long long *n_items; // level-0 partial matches in all (device scalar)
+    long long items_cap;      // capacity of item_pair
     unsigned char *ord;       // JAM_MAX_LEVELS per pair: level -> template atom
     PairLevel *lvl;           // JAM_MAX_LEVELS per pair
     int *nchunk;              // level-0 partial matches per pair

@@ -209,18 +209,25 @@ __global__ void __launch_bounds__(256) k_items(DevPairs pr)
 // reference's enumeration order and each match's rank.
 
 // Level-0 rows: the candidates of every pair's level-0 list.
+// The buffers were sized before the sizes were known: when the candidate
+// sets, the items or the frontier do not fit, no row is written (the join then
+// has nothing to do) and the host grows the buffers and runs the query again.
 __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin jn)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *jn.n_out = (unsigned long long)pr.n_items;
-    if (i >= pr.n_items) return;
-    const int p = pr.item_pair[i];
-    const int pos0 = (int)(i - pr.item_base[p]) + pr.n1[p];
-    const int off0 = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off;
-    jn.out_pair[i] = p;
-    // the row holds kd entries at every level; level 0 walks its list in the
-    // reference order, so its entry is the kd position of list entry pos0
-    jn.out_ent[i * jn.stride] = off0 + st.l2k[off0 + pos0];
+    const long long n = *pr.n_items;
+    const bool fits = *st.total <= st.cap && n <= pr.items_cap && n <= jn.out_cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *jn.n_out = fits ? (unsigned long long)n : 0ull;
+    if (!fits) return;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int p = pr.item_pair[i];
+        const int pos0 = (int)(i - pr.item_base[p]) + pr.n1[p];
+        const int off0 = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off;
+        jn.out_pair[i] = p;
+        // the row holds kd entries at every level; level 0 walks its list in
+        // the reference order, so its entry is the kd position of list entry pos0
+        jn.out_ent[i * jn.stride] = off0 + st.l2k[off0 + pos0];
+    }
 }
 
 #ifndef JOIN_WAVES
