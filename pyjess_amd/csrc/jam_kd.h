@@ -46,7 +46,9 @@ namespace jam {
 
 #define KD_BIG 1024
 #define KD_TIERS 6
+#ifndef KD_THREADS
 #define KD_THREADS 256
+#endif
 // threads of the exact pass over a set larger than KD_BIG: its rounds are
 // latency-bound loops over the positions, so more threads per set
 #define KD_EXACT_THREADS 1024

@@ -62,3 +62,29 @@ def test_result_is_stale_after_the_next_query():
                 L.jam_result_free(h2)
         finally:
             L.jam_result_free(h1)
+
+
+def test_fresh_context_grows_and_reruns():
+    # a new context starts with small buffers: the first large query finds out
+    # on the device that its sets, items or frontier do not fit, and is run
+    # again after the host grows them; small and large queries then alternate
+    device()
+    from oracle import oracle
+    from bench import pair_digests
+
+    ctx = _native.Context(0)
+    big_mols, big_tpls, kw = synthetic.workload("batch-4k", seed=5, scale=24 / 4096)
+    small_mols, small_tpls, _ = synthetic.workload("plain-2k", seed=5)
+    q = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    runs = [(small_mols, small_tpls), (big_mols, big_tpls), (small_mols, small_tpls), (big_mols, big_tpls)]
+    for mols, tpls in runs:
+        tset = _native.TemplateSet(ctx, tpls)
+        mset = _native.MoleculeSet(ctx, mols)
+        res = _native.query(tset, mset, *q, 0, True)
+        cm, hc, dg = oracle.batch([oracle.MolView.of(m) for m in mols], oracle.TplView(tpls), *q, n_threads=4)
+        T = len(tpls)
+        sel = res.flag == 1
+        gpu_hc, gpu_dg = pair_digests(res.pair[sel], res.atoms[sel], [len(t) for t in tpls], T, len(mols) * T)
+        assert np.array_equal(res.pair_matches, cm)
+        assert np.array_equal(gpu_hc, hc)
+        assert np.array_equal(gpu_dg, dg.astype(np.uint64))
