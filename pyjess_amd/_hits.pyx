@@ -40,7 +40,7 @@ class Block:
         self.atoms = atoms
 
 
-@cython.freelist(1024)
+@cython.freelist(8192)
 @cython.no_gc
 cdef class Hit:
     """A hit between a query molecule and a template (``_jess.pyx:2286-2621``).
