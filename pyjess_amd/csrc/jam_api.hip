@@ -890,37 +890,54 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                     break;
                 }
             };
-            auto launch_exact = [&](int k, hipStream_t x) {
+            // the exact pass of tier k over the sets its fast pass found tied,
+            // or (`all`) over every set of the tier
+            auto launch_exact = [&](int k, hipStream_t x, bool all) {
                 const size_t o = (size_t)k * ns;
+                const int *lst = all ? bucket + o : tied + o;
+                const unsigned int *lc = all ? nb + k : nt + k;
                 switch (k) {
                 case 0:
-                    hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, tied, nt);
+                    hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, lst, lc);
                     break;
                 case 1:
-                    hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, x, dm, dt, st, tied + o, nt + 1);
+                    hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, x, dm, dt, st, lst, lc);
                     break;
                 case 2:
-                    hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, x, dm, dt, st, tied + o, nt + 2);
+                    hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, x, dm, dt, st, lst, lc);
                     break;
                 case 3:
-                    hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, x, dm, dt, st, tied + o, nt + 3);
+                    hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, x, dm, dt, st, lst, lc);
                     break;
                 case 4:
-                    hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, tied + o,
-                                       nt + 4);
+                    hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, lst, lc);
                     break;
                 default:
                     hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr,
-                                       tied + o, nt + 5);
+                                       lst, lc);
                     break;
                 }
             };
+            // One molecule has few sets in the 512 and 1024 tiers, and their
+            // exact pass (hundreds of microseconds per set, all sets side by
+            // side) is the kd phase's longest chain: it runs on every set of
+            // those tiers at once, without waiting for a fast pass to list the
+            // tied ones. The exact pass gives the reference's order whether or
+            // not a set has ties.
+            const bool direct = (M == 1);
             // largest tier first: its exact pass, the longest, starts earliest
             for (int k = KD_TIERS - 1; k >= 0; k--) {
-                launch_fast(k);
-                HIP_TRY(hipEventRecord(c->kd_fork[k], s));
-                HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
-                launch_exact(k, c->kd_side[k]);
+                const bool all = direct && (k == 3 || k == 4);
+                if (all) {
+                    HIP_TRY(hipEventRecord(c->kd_fork[k], s));
+                    HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
+                    launch_exact(k, c->kd_side[k], true);
+                } else {
+                    launch_fast(k);
+                    HIP_TRY(hipEventRecord(c->kd_fork[k], s));
+                    HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
+                    launch_exact(k, c->kd_side[k], false);
+                }
                 HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
             }
             for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
