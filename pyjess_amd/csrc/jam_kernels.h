@@ -184,14 +184,6 @@ __global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs 
     }
 }
 
-__global__ void __launch_bounds__(256) k_items(DevPairs pr)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pr.n_pairs) return;
-    const int c = pr.nchunk[p];
-    const long long b = pr.item_base[p];
-    for (int i = 0; i < c; i++) pr.item_pair[b + i] = p;
-}
 
 // ---------------------------------------------------------------------------
 // join (the scanner), one tree level at a time

@@ -1,0 +1,37 @@
+"""The hit-set digest bench.py uses for parity (vectorised over hits) equals
+the oracle's own per-pair digest and the scalar definition, on a small
+synthetic batch (no GPU: the hits come from the oracle's replay)."""
+
+import numpy as np
+
+from oracle import oracle
+from pyjess_amd import synthetic
+
+import bench
+
+
+def test_vectorised_digest_matches_oracle_and_scalar_definition():
+    mols, tpls, kw = synthetic.workload("batch-4k", seed=2, scale=3 / 4096)
+    q = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    T = len(tpls)
+    cm, hc, dg = oracle.batch([oracle.MolView.of(m) for m in mols], oracle.TplView(tpls), *q, n_threads=2)
+    pairs, rows = [], []
+    width = max(len(t) for t in tpls)
+    for mi, m in enumerate(mols):
+        stream = oracle.query_stream(m, tpls, kw["distance_cutoff"], kw["max_dynamic_distance"])
+        for h in stream:
+            if h.rmsd <= kw["rmsd_threshold"] and not h.nan:
+                pairs.append(mi * T + h.template)
+                rows.append(list(h.atoms) + [-1] * (width - len(h.atoms)))
+    assert len(pairs) > 10
+    pairs = np.array(pairs, dtype=np.int32)
+    rows = np.array(rows, dtype=np.int32)
+    counts, digests = bench.pair_digests(pairs, rows, [len(t) for t in tpls], T, len(mols) * T)
+    assert np.array_equal(counts, hc)
+    assert np.array_equal(digests, dg.astype(np.uint64))
+    # the scalar definition, summed mod 2^64
+    want = [0] * (len(mols) * T)
+    for p, r in zip(pairs, rows):
+        m, t = divmod(int(p), T)
+        want[p] = (want[p] + bench.hit_digest(t, m, r[:len(tpls[t])])) & bench._M64
+    assert [int(x) for x in digests] == want
