@@ -587,8 +587,8 @@ def main():
                     help="point-set pairs of the differentiable-RMSD measurement (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
-        log(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-            "(queries are independent: the sharding over GPUs is described in DESIGN.md)")
+        log(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (queries are independent: "
+            "they would be dealt to GPUs by molecule, with no exchange during the search)")
         sys.exit(2)
 
     if args.workload == "rmsd-grad-1m":

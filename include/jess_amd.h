@@ -24,7 +24,7 @@ extern "C" {
 #define JAM_OK 0
 #define JAM_ENOMEM (-1)     /* host or device allocation failed */
 #define JAM_EINVAL (-2)     /* invalid argument */
-#define JAM_EMODE (-3)      /* template uses a match mode this build does not implement */
+#define JAM_EMODE (-3)      /* template uses a match mode other than 0, 1 and 3 */
 #define JAM_EDEVICE (-4)    /* HIP runtime error or no usable gfx950 device */
 #define JAM_ETOOBIG (-5)    /* template larger than JAM_MAX_TEMPLATE_ATOMS */
 #define JAM_ESTALE (-6)     /* a result's events were overwritten by a later query */

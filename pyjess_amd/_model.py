@@ -504,7 +504,7 @@ class Molecule:
         if format not in ("detect", "pdb", "cif"):
             raise ValueError(f"invalid value for `format` argument: {format!r}")
         if format == "cif":
-            raise ValueError("mmCIF parsing is not supported by this build")
+            raise ValueError("mmCIF parsing is not supported: load the structure from PDB text")
         if isinstance(file, (str, os.PathLike)):
             with open(file) as f:
                 return cls._load_pdb(f, id, date, name, ignore_endmdl, skip_hetatm)
