@@ -483,10 +483,10 @@ def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
         bwd_ms.append(ev[1].elapsed_time(ev[2]))
     fwd_ms, bwd_ms = float(np.median(fwd_ms)), float(np.median(bwd_ms))
     # algorithmic HBM bytes: forward reads a, b and the offsets, writes the
-    # RMSD; backward reads a, b, offsets, RMSD and upstream gradient, writes
-    # both gradients
-    fwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 8 * n_sets
-    bwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 16 * n_sets + 2 * 24 * n_pts
+    # RMSD and the rotation (kept for the backward); backward reads a, b,
+    # offsets, RMSD, rotation and upstream gradient, writes both gradients
+    fwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 8 * n_sets + 72 * n_sets
+    bwd_bytes = 2 * 24 * n_pts + 8 * (n_sets + 1) + 16 * n_sets + 72 * n_sets + 2 * 24 * n_pts
     bwd_gbs = bwd_bytes / (bwd_ms / 1000.0) / 1e9
     log(f"differentiation: {value:,.0f} matches/s forward+backward ({1000 * elapsed / steps:.3f} ms/step); "
         f"forward {fwd_ms:.3f} ms ({fwd_bytes / fwd_ms / 1e6:.0f} GB/s), backward {bwd_ms:.3f} ms "

@@ -188,13 +188,14 @@ int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, i
                          double *rmsd, double *rot9, void *stream);
 /* Gradient of sum_s grad_rmsd[s] * rmsd[s] with respect to a and b (device
  * arrays laid out like a and b; either may be NULL), given the rmsd of
- * jam_rmsd_forward. No counterpart in the reference, which has no gradients:
+ * jam_rmsd_forward and, optionally, its rotations (rot9; NULL recomputes
+ * them). No counterpart in the reference, which has no gradients:
  * d rmsd / d a_i = (A_i - R^T B_i) / (n rmsd) and
  * d rmsd / d b_i = (B_i - R A_i) / (n rmsd), A and B the centred points, R
  * the optimal proper rotation; zero where rmsd is 0. Asynchronous on `stream`. */
 int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
-                          const double *rmsd, const double *grad_rmsd, double *grad_a, double *grad_b,
-                          void *stream);
+                          const double *rmsd, const double *rot9, const double *grad_rmsd, double *grad_a,
+                          double *grad_b, void *stream);
 
 /* ---- misc ------------------------------------------------------------- */
 

@@ -151,7 +151,7 @@ def lib():
         L.jam_result_level_partials.restype = i32
         L.jam_rmsd_forward.argtypes = [vp, vp, vp, i64, vp, vp, vp]
         L.jam_rmsd_forward.restype = i32
-        L.jam_rmsd_backward.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp]
+        L.jam_rmsd_backward.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
         L.jam_rmsd_backward.restype = i32
         L.jam_context_set_profiling.argtypes = [vp, i32]
         L.jam_context_set_profiling.restype = i32

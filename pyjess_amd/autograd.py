@@ -52,16 +52,20 @@ class SuperpositionRMSD(torch.autograd.Function):
         _check_inputs(a, b, offsets)
         n_sets = offsets.numel() - 1
         rmsd = torch.empty(n_sets, dtype=torch.float64, device=a.device)
+        # the optimal rotations are kept for the backward pass when it will run
+        need_grad = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
+        rot = torch.empty((n_sets, 9), dtype=torch.float64, device=a.device) if need_grad else None
         L = _native.lib()
         with torch.cuda.device(a.device):
             stream = ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream)
-            _native._check(L.jam_rmsd_forward(_ptr(a), _ptr(b), _ptr(offsets), n_sets, _ptr(rmsd), None, stream))
-        ctx.save_for_backward(a, b, offsets, rmsd)
+            _native._check(L.jam_rmsd_forward(_ptr(a), _ptr(b), _ptr(offsets), n_sets, _ptr(rmsd), _ptr(rot),
+                                              stream))
+        ctx.save_for_backward(a, b, offsets, rmsd, rot)
         return rmsd
 
     @staticmethod
     def backward(ctx, grad_rmsd: torch.Tensor):
-        a, b, offsets, rmsd = ctx.saved_tensors
+        a, b, offsets, rmsd, rot = ctx.saved_tensors
         need_a, need_b = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         if not (need_a or need_b):
             return None, None, None
@@ -72,7 +76,7 @@ class SuperpositionRMSD(torch.autograd.Function):
         with torch.cuda.device(a.device):
             stream = ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream)
             _native._check(L.jam_rmsd_backward(_ptr(a), _ptr(b), _ptr(offsets), offsets.numel() - 1, _ptr(rmsd),
-                                               _ptr(g), _ptr(ga), _ptr(gb), stream))
+                                               _ptr(rot), _ptr(g), _ptr(ga), _ptr(gb), stream))
         return ga, gb, None
 
 

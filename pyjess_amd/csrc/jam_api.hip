@@ -223,15 +223,15 @@ extern "C" int32_t jam_rmsd_forward(const double *a, const double *b, const int6
 }
 
 extern "C" int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
-                                     const double *rmsd, const double *grad_rmsd, double *grad_a, double *grad_b,
-                                     void *stream)
+                                     const double *rmsd, const double *rot9, const double *grad_rmsd, double *grad_a,
+                                     double *grad_b, void *stream)
 {
     if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
     if (n_sets == 0) return JAM_OK;
     if (!a || !b || !off || !rmsd || !grad_rmsd || (!grad_a && !grad_b)) return fail(JAM_EINVAL, "NULL argument");
     const int tb = RG_SETS;
     hipLaunchKernelGGL(jam::k_rmsd_bwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
-                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, grad_rmsd, grad_a, grad_b);
+                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, rot9, grad_rmsd, grad_a, grad_b);
     HIP_TRY(hipGetLastError());
     return JAM_OK;
 }

@@ -188,10 +188,11 @@ __device__ __attribute__((always_inline)) static inline void rmsd_one(int n, con
 }
 
 // One set: the gradient of g * rmsd, written to GA / GB (which may alias A /
-// B: every point is read before its gradient is written).
+// B: every point is read before its gradient is written). rot: the set's
+// optimal rotation from the forward pass, or null to compute it again.
 __device__ __attribute__((always_inline)) static inline void rmsd_grad_one(int n, const double *A, const double *B,
-                                                                           double r, double g, double *GA,
-                                                                           double *GB)
+                                                                           double r, double g, const double *rot,
+                                                                           double *GA, double *GB)
 {
     if (!(r > 0.0) || g == 0.0) {
         for (int q = 0; q < 3 * n; q++) {
@@ -200,9 +201,34 @@ __device__ __attribute__((always_inline)) static inline void rmsd_grad_one(int n
         }
         return;
     }
-    double ca[3], cb[3], X[3][3], R[3][3];
-    centre_and_correlate(n, A, B, ca, cb, X);
-    optimal_rotation(X, R);
+    double ca[3], cb[3], R[3][3];
+    if (rot) {
+        // centroids only: the rotation is the forward pass's
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ca[d] = 0.0;
+            cb[d] = 0.0;
+        }
+        for (int i = 0; i < n; i++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                ca[d] += A[3 * i + d];
+                cb[d] += B[3 * i + d];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ca[d] /= (double)n;
+            cb[d] /= (double)n;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) R[j][k] = rot[3 * j + k];
+    } else {
+        double X[3][3];
+        centre_and_correlate(n, A, B, ca, cb, X);
+        optimal_rotation(X, R);
+    }
     const double scale = g / ((double)n * r);
     for (int i = 0; i < n; i++) {
         double u[3], v[3];
@@ -272,8 +298,8 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const dou
 }
 
 __global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const double *b, const int64_t *off,
-                                                      long long n_sets, const double *rmsd, const double *grad_rmsd,
-                                                      double *grad_a, double *grad_b)
+                                                      long long n_sets, const double *rmsd, const double *rot,
+                                                      const double *grad_rmsd, double *grad_a, double *grad_b)
 {
     __shared__ double la[3 * RG_CAP], lb[3 * RG_CAP];
     const int lane = threadIdx.x;
@@ -295,13 +321,14 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const dou
         const int64_t p0 = off[s];
         const int n = (int)(off[s + 1] - p0);
         if (n > 0) {
+            const double *rt = rot ? rot + 9 * s : nullptr;
             if (staged) {
                 // gradients replace the points in LDS
                 double *A = la + 3 * (p0 - P0), *B = lb + 3 * (p0 - P0);
-                rmsd_grad_one(n, A, B, rmsd[s], grad_rmsd[s], A, B);
+                rmsd_grad_one(n, A, B, rmsd[s], grad_rmsd[s], rt, A, B);
             } else {
-                rmsd_grad_one(n, a + 3 * p0, b + 3 * p0, rmsd[s], grad_rmsd[s], grad_a ? grad_a + 3 * p0 : nullptr,
-                              grad_b ? grad_b + 3 * p0 : nullptr);
+                rmsd_grad_one(n, a + 3 * p0, b + 3 * p0, rmsd[s], grad_rmsd[s], rt,
+                              grad_a ? grad_a + 3 * p0 : nullptr, grad_b ? grad_b + 3 * p0 : nullptr);
             }
         }
     }
