@@ -119,7 +119,7 @@ struct jam_context {
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
     int64_t sort_cap = 0;
-    DBuf e_pair, e_rank, e_flag, e_rmsd, e_rot, e_c0, e_c1, e_atoms;
+    DBuf e_all;  // the events of the last query, event_layout(count, stride)
     DBuf scan_tmp, scan_tmp2;
     int64_t set_cap = 0, blk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
@@ -623,14 +623,7 @@ int32_t grow_events(jam_context *c, int64_t cap, int stride)
     if (cap > c->event_cap || stride > c->event_stride) {
         cap = std::max<int64_t>(cap, std::max<int64_t>(c->event_cap, 1024));
         stride = std::max(stride, c->event_stride);
-        HIP_TRY(c->e_pair.reserve(sizeof(int) * cap));
-        HIP_TRY(c->e_rank.reserve(sizeof(long long) * cap));
-        HIP_TRY(c->e_flag.reserve(sizeof(int) * cap));
-        HIP_TRY(c->e_rmsd.reserve(sizeof(double) * cap));
-        HIP_TRY(c->e_rot.reserve(sizeof(double) * 9 * cap));
-        HIP_TRY(c->e_c0.reserve(sizeof(double) * 3 * cap));
-        HIP_TRY(c->e_c1.reserve(sizeof(double) * 3 * cap));
-        HIP_TRY(c->e_atoms.reserve(sizeof(int) * cap * stride));
+        HIP_TRY(c->e_all.reserve(event_layout(cap, stride).bytes));
         c->event_cap = cap;
         c->event_stride = stride;
     }
@@ -1031,9 +1024,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         }
         HIP_TRY(hipEventRecord(c->ev[3], s));
 
-        // ---- reference order of the complete matches: LSD radix passes over
-        // the level positions (`bits` bits each, `per` levels per 64-bit key),
-        // the last levels first, then the pair
+        // ---- reference order of the complete matches: the order of the key
+        // (pair, level 0 position, level 1 position, ...), by rank counting
+        // when the matches are few, else by LSD radix passes over the level
+        // positions (`bits` bits each, `per` levels per 64-bit key), the last
+        // levels first, then the pair
         if ((rc = grow_sort(c, n_done)) != JAM_OK) return rc;
         if ((rc = grow_events(c, n_done, stride)) != JAM_OK) return rc;
         size_t scan_bytes = 0;
@@ -1042,9 +1037,35 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(c->s_tmp.reserve(scan_bytes));
         int cur = 0;
         const unsigned gs = (unsigned)((n_done + 255) / 256);
-        if (n_done > 0) {
-            int bits = 1;
-            while ((1ll << bits) <= (long long)maxcnt) bits++;
+        int pbits = 1;
+        while ((1ll << pbits) < (long long)n_pairs) pbits++;
+        int bits = 1;
+        while ((1ll << bits) <= (long long)maxcnt) bits++;
+        const int words = (pbits + stride * bits + 63) / 64;
+        if (n_done > 0 && n_done <= SORT_RANK_MAX && words <= SORT_RANK_WORDS) {
+            // rank counting: same permutation as the radix passes below
+            HIP_TRY(c->s_key[0].reserve(sizeof(unsigned long long) * (size_t)n_done * words));
+            unsigned long long *fk = c->s_key[0].as<unsigned long long>();
+            int *rank = c->s_perm[1].as<int>();
+            const unsigned tiles = (unsigned)((n_done + 255) / 256);
+            switch (words) {
+#define JAM_RANK_SORT(W)                                                                                        \
+    case W:                                                                                                     \
+        hipLaunchKernelGGL(jam::k_fullkey<W>, dim3(gs), dim3(256), 0, s, st, dt, pr, jn, (long long)n_done, stride, \
+                           bits, pbits, fk, rank);                                                              \
+        hipLaunchKernelGGL(jam::k_rank<W>, dim3(tiles, tiles), dim3(256), 0, s, (const unsigned long long *)fk,   \
+                           (long long)n_done, rank);                                                            \
+        break;
+                JAM_RANK_SORT(1)
+                JAM_RANK_SORT(2)
+                JAM_RANK_SORT(3)
+                JAM_RANK_SORT(4)
+#undef JAM_RANK_SORT
+            }
+            hipLaunchKernelGGL(jam::k_rank_scatter, dim3(gs), dim3(256), 0, s, (const int *)rank, (long long)n_done,
+                               c->s_perm[0].as<int>());
+            HIP_TRY(hipGetLastError());
+        } else if (n_done > 0) {
             const int per = std::max(1, 64 / bits);
             const int nt = stride;
             hipLaunchKernelGGL(jam::k_iota, dim3(gs), dim3(256), 0, s, c->s_perm[0].as<int>(), (long long)n_done);
@@ -1062,8 +1083,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             }
             hipLaunchKernelGGL(jam::k_pairkey, dim3(gs), dim3(256), 0, s, jn, T, c->s_perm[cur].as<const int>(),
                                (long long)n_done, c->s_pkey[0].as<unsigned int>());
-            int pbits = 1;
-            while ((1ll << pbits) < (long long)n_pairs) pbits++;
             HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->s_tmp.ptr, tb, c->s_pkey[0].as<unsigned int>(),
                                                        c->s_pkey[1].as<unsigned int>(), c->s_perm[cur].as<int>(),
                                                        c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, pbits, s));
@@ -1076,14 +1095,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         DevEvents ev;
         ev.cap = c->event_cap;
         ev.stride = stride;
-        ev.pair = c->e_pair.as<int>();
-        ev.rank = c->e_rank.as<long long>();
-        ev.flag = c->e_flag.as<int>();
-        ev.rmsd = c->e_rmsd.as<double>();
-        ev.rot = c->e_rot.as<double>();
-        ev.c0 = c->e_c0.as<double>();
-        ev.c1 = c->e_c1.as<double>();
-        ev.atoms = c->e_atoms.as<int>();
+        ev.base = c->e_all.as<unsigned char>();
         // the event count, then the complete matches of every pair: read back
         // in one copy
         unsigned long long *pcm = (unsigned long long *)c->p_cm.ptr;
@@ -1202,20 +1214,24 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
         for (auto &ev : c->stage_ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
     // the fields, cut into pieces of at most one staging half
+    // (in the order of the packed layout, so that neighbouring pieces are
+    // contiguous on the device)
     std::vector<CopyPiece> pieces;
-    auto add = [&](void *dst, const DBuf &src, size_t bytes) {
+    const EventLayout L = event_layout((int64_t)n, r->stride);
+    const char *base = c->e_all.as<const char>();
+    auto add = [&](void *dst, size_t at, size_t bytes) {
         if (!dst) return;
         for (size_t o = 0; o < bytes; o += JAM_STAGE_BYTES)
-            pieces.push_back({(char *)dst + o, (const char *)src.ptr + o, std::min(JAM_STAGE_BYTES, bytes - o)});
+            pieces.push_back({(char *)dst + o, base + at + o, std::min(JAM_STAGE_BYTES, bytes - o)});
     };
-    add(pair, c->e_pair, sizeof(int32_t) * n);
-    add(rank, c->e_rank, sizeof(int64_t) * n);
-    add(flag, c->e_flag, sizeof(int32_t) * n);
-    add(rmsd, c->e_rmsd, sizeof(double) * n);
-    add(rot, c->e_rot, sizeof(double) * 9 * n);
-    add(c0, c->e_c0, sizeof(double) * 3 * n);
-    add(c1, c->e_c1, sizeof(double) * 3 * n);
-    add(atoms, c->e_atoms, sizeof(int32_t) * (size_t)r->stride * n);
+    add(rank, L.rank, sizeof(int64_t) * n);
+    add(rmsd, L.rmsd, sizeof(double) * n);
+    add(rot, L.rot, sizeof(double) * 9 * n);
+    add(c0, L.c0, sizeof(double) * 3 * n);
+    add(c1, L.c1, sizeof(double) * 3 * n);
+    add(pair, L.pair, sizeof(int32_t) * n);
+    add(flag, L.flag, sizeof(int32_t) * n);
+    add(atoms, L.atoms, sizeof(int32_t) * (size_t)r->stride * n);
     // batches of consecutive pieces filling one half each; the copy of batch
     // k + 1 runs while batch k is unpacked from the other half
     struct Batch {
@@ -1231,10 +1247,14 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
     auto issue = [&](const Batch &b) -> hipError_t {
         char *h = (char *)c->hstage + b.half * JAM_STAGE_BYTES;
         size_t o = 0;
-        for (size_t i = b.first; i < b.last; i++) {
-            hipError_t e = hipMemcpyAsync(h + o, pieces[i].src, pieces[i].bytes, hipMemcpyDeviceToHost, s);
+        for (size_t i = b.first; i < b.last;) {
+            // one transfer for every run of pieces contiguous on the device
+            size_t j = i + 1, bytes = pieces[i].bytes;
+            while (j < b.last && pieces[j].src == pieces[j - 1].src + pieces[j - 1].bytes) bytes += pieces[j++].bytes;
+            hipError_t e = hipMemcpyAsync(h + o, pieces[i].src, bytes, hipMemcpyDeviceToHost, s);
             if (e != hipSuccess) return e;
-            o += pieces[i].bytes;
+            o += bytes;
+            i = j;
         }
         return hipEventRecord(c->stage_ev[b.half], s);
     };

@@ -121,14 +121,40 @@ struct DevJoin {
     unsigned int *overflow;             // set when a row did not fit
 };
 
+// The events of one query, packed field after field for exactly n events
+// (the 8-byte fields first), so that copying them out is one transfer.
+struct EventLayout {
+    size_t rank, rmsd, rot, c0, c1, pair, flag, atoms, bytes;
+};
+
+__host__ __device__ inline EventLayout event_layout(int64_t n, int stride)
+{
+    EventLayout L;
+    size_t o = 0;
+    L.rank = o;
+    o += sizeof(int64_t) * n;
+    L.rmsd = o;
+    o += sizeof(double) * n;
+    L.rot = o;
+    o += sizeof(double) * 9 * n;
+    L.c0 = o;
+    o += sizeof(double) * 3 * n;
+    L.c1 = o;
+    o += sizeof(double) * 3 * n;
+    L.pair = o;
+    o += sizeof(int32_t) * n;
+    L.flag = o;
+    o += sizeof(int32_t) * n;
+    L.atoms = o;
+    o += sizeof(int32_t) * (size_t)stride * n;
+    L.bytes = o;
+    return L;
+}
+
 struct DevEvents {
     int64_t cap;
     int stride;
-    int *pair;
-    long long *rank;
-    int *flag;
-    double *rmsd, *rot, *c0, *c1;
-    int *atoms;
+    unsigned char *base;        // event_layout(count, stride)
     unsigned long long *count;  // device scalar
 };
 

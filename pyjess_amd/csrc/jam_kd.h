@@ -376,8 +376,17 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
 
 // ---------------------------------------------------------------------------
 // exact path, sets of at most NT entries (NT = 64 ... KD_BIG): one position
-// per thread; the arrangement, prefix counts, pushes and QuickSelect windows
-// live in LDS, the prefix counts are ballots plus one LDS word per wave
+// per thread. The arrangement, prefix counts and pushes live in LDS; every
+// thread keeps its own element, its range and the QuickSelect window of that
+// range in registers, and computes the window updates itself, so one
+// partition step costs four barriers plus the pointer-jumping rounds.
+//
+// One partition step of a window [lw, rw] (pivot moved to rw) in the
+// reference's order: the "<=" elements keep their order at the front; every
+// position from the first ">" element f on pushes once onto a queue, a ">"
+// element itself, a "<=" element the element it pops from the queue's front
+// (the ">" block rotating by one); the pivot follows the "<=" elements and the
+// queue's final contents fill the rest, its front at rw.
 // ---------------------------------------------------------------------------
 
 template <int NT>
@@ -385,13 +394,12 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                                                 const unsigned int *count)
 {
     constexpr int NW = NT / JAM_WAVE;
-    __shared__ double cc[3][NT];      // coordinates by element
-    __shared__ double pk[NT];         // pivot key of the range starting at a position
-    __shared__ int xa[NT];            // arrangement: element at each position
-    __shared__ int xp1[NT], xp2[NT];  // exclusive prefix counts by position
-    __shared__ int src[NT];           // pushes of the FIFO queue
-    __shared__ int wl[NT], wr[NT];    // QuickSelect window of the range starting here; wr < 0: done
-    __shared__ int wsum[NW];
+    __shared__ double cc[3][NT];  // coordinates by element
+    __shared__ int xa[NT];        // arrangement: element at each position
+    __shared__ int xp1[NT];       // exclusive prefix counts of the "<=" flags
+    __shared__ int src[NT];       // queue pushes of the window starting at lw, from lw on
+    __shared__ int wf[NT];        // first ">" position of the window of the range starting here
+    __shared__ int wsum[NW], wact[NW];
     const int t = threadIdx.x;
     const int lane = lane_id();
     const int wv = t / JAM_WAVE;
@@ -408,138 +416,126 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
             cc[0][t] = ms.xyz[3 * g + 0];
             cc[1][t] = ms.xyz[3 * g + 1];
             cc[2][t] = ms.xyz[3 * g + 2];
+            xa[t] = t;
         }
-        int a = t;              // element at position t
-        int R = 0, H = n;       // range of position t
-        for (int d = 0;; d++) {
+        __syncthreads();
+        int a = t;                          // element at position t
+        int R = 0, H = n;                   // range of position t
+        int lw = 0, rw = n >= 2 ? n - 1 : -1;  // its window; rw < 0: none
+        int hmax = n;                       // largest range at this depth
+        for (int d = 0; hmax >= 2;) {
             const double *c = cc[d % 3];
-            xa[t] = a;
-            bool opened = false;
-            if (live && t == R) {
-                if (H - R >= 2) {
-                    wl[t] = R;
-                    wr[t] = H - 1;
-                    opened = true;
-                } else {
-                    wr[t] = -1;
-                }
+            const bool active = live && rw >= 0;
+            double pv = 0.0;
+            if (active) {
+                // pivot: median of three, swapped with the window's right end
+                const int mid = (lw + rw + 1) / 2;
+                const int el = xa[lw], em = xa[mid], er = xa[rw];
+                const double kl = c[el], km = c[em], kr = c[er];
+                int piv;
+                if ((kl > km) != (kl > kr)) piv = lw;
+                else if ((km < kl) != (km < kr)) piv = mid;
+                else piv = rw;
+                const int ep = piv == lw ? el : (piv == mid ? em : er);
+                pv = c[ep];
+                if (t == piv) a = er;
+                if (t == rw) a = ep;
+                if (t == R) wf[R] = rw;
             }
-            if (!__syncthreads_or(opened)) break;
+            const bool inwin = active && t >= lw && t < rw;
+            const bool le = inwin && c[a] <= pv;
+            const bool gt = inwin && !le;
+            const unsigned long long bal = ballot(le);
+            const unsigned long long gball = ballot(gt);
+            const unsigned long long aball = ballot(active);
+            if (lane == 0) {
+                wsum[wv] = __popcll(bal);
+                wact[wv] = aball != 0ull;
+            }
+            __syncthreads();
+            bool any = false;
+            for (int w = 0; w < NW; w++) any |= wact[w] != 0;
+            if (!any) {
+                // every selection of this depth is done: split the ranges at
+                // their medians and open the windows of the next depth
+                hmax -= hmax / 2;
+                if (live && H - R >= 2) {
+                    const int m = R + (H - R) / 2;
+                    if (t < m) H = m;
+                    else R = m;
+                }
+                lw = R;
+                rw = (live && H - R >= 2) ? H - 1 : -1;
+                d++;
+                __syncthreads();
+                continue;
+            }
+            int p1 = rank_below(bal);
+            for (int w = 0; w < wv; w++) p1 += wsum[w];
+            xp1[t] = p1;
+            if (gt) {
+                // the first ">" of its window among this wave's lanes
+                const int lo = max(lw - (t - lane), 0);
+                const unsigned long long below = gball & ((1ull << lane) - 1ull) & ~((1ull << lo) - 1ull);
+                if (below == 0ull) atomicMin(&wf[R], t);
+            }
+            __syncthreads();
+            int p1l = 0, p1r = 0, f = 0;
+            if (active) {
+                p1l = xp1[lw];
+                p1r = xp1[rw];
+                f = wf[R];
+            }
+            const int ones = p1 - p1l;        // "<=" elements before t in the window
+            const int k1 = p1r - p1l;         // "<=" elements of the window
+            const int m = (rw - lw) - k1;     // ">" elements
+            const int nrot = k1 - (f - lw);   // pops: "<=" elements after f
+            const int pushes = rw - f;
+            if (inwin && t >= f) {
+                // push t - f; a "<=" element pushes what pop number
+                // ones - (f - lw) takes, which is push number ones - (f - lw)
+                if (!le) src[lw + (t - f)] = a;
+                else src[lw + (t - f)] = -(lw + ones - (f - lw)) - 1;
+            }
+            __syncthreads();
+            // pointer jumping until every push names an element; each pop
+            // names a lower push, so the chains end
             for (;;) {
-                // pivot: median of three, moved to the right end of the window
-                if (live && t == R && wr[t] >= 0) {
-                    const int l = wl[t], r = wr[t], mid = (l + r + 1) / 2;
-                    const int el = xa[l], em = xa[mid], er = xa[r];
-                    const double kl = c[el], km = c[em], kr = c[er];
-                    int piv;
-                    if ((kl > km) != (kl > kr)) piv = l;
-                    else if ((km < kl) != (km < kr)) piv = mid;
-                    else piv = r;
-                    const int ep = xa[piv];
-                    xa[piv] = er;
-                    xa[r] = ep;
-                    pk[t] = c[ep];
-                }
-                __syncthreads();
-                int lw = 0, rw = -1;
-                double pv = 0.0;
-                if (live) {
-                    a = xa[t];
-                    lw = wl[R];
-                    rw = wr[R];
-                    pv = pk[R];
-                }
-                const bool inwin = rw >= 0 && t >= lw && t < rw;
-                const bool le = inwin && c[a] <= pv;
-                // "<= pivot" flags: exclusive prefix over all positions
-                unsigned long long bal = ballot(le);
-                if (lane == 0) wsum[wv] = __popcll(bal);
-                __syncthreads();
-                int p1 = rank_below(bal);
-                for (int w = 0; w < wv; w++) p1 += wsum[w];
-                xp1[t] = p1;
-                __syncthreads();
-                int p1l = 0, p1r = 0;
-                if (rw >= 0) {
-                    p1l = xp1[lw];
-                    p1r = xp1[rw];
-                }
-                const int ones = p1 - p1l;
-                const int zeros = (t - lw) - ones;
-                // re-push flags: a "<=" element after the first ">" one
-                const bool rot = le && zeros > 0;
-                bal = ballot(rot);
-                if (lane == 0) wsum[wv] = __popcll(bal);
-                __syncthreads();
-                int p2 = rank_below(bal);
-                for (int w = 0; w < wv; w++) p2 += wsum[w];
-                xp2[t] = p2;
-                __syncthreads();
-                int p2l = 0, p2r = 0;
-                if (rw >= 0) {
-                    p2l = xp2[lw];
-                    p2r = xp2[rw];
-                }
-                const int rots = p2 - p2l;
-                const int k1 = p1r - p1l;             // "<=" elements of the window
-                const int m = (rw - lw) - k1;         // ">" elements
-                const int nrot = p2r - p2l;
-                const int pushes = m + nrot;
-                // pushes: ">" elements are new pushes, re-pushes point to push q
-                if (inwin) {
-                    const int slot = lw + zeros + rots;
-                    if (!le) src[slot] = a;
-                    else if (zeros > 0) src[slot] = -(lw + rots) - 1;
-                }
-                __syncthreads();
-                // pointer jumping until every push names an element; each
-                // re-push points to a lower push, so the chains end
-                for (;;) {
-                    bool again = false;
-                    if (inwin && t - lw < pushes) {
-                        const int v = src[t];
-                        if (v < 0) {
-                            const int u = src[-v - 1];
-                            src[t] = u;
-                            again = u < 0;
-                        }
-                    }
-                    if (!__syncthreads_or(again)) break;
-                }
-                // scatter: "<=" elements in order, the pivot after them, the
-                // queue from its second element on, its front at the right end
-                if (rw >= 0) {
-                    if (t == rw) {
-                        xa[lw + k1] = a;
-                    } else if (inwin) {
-                        if (le) xa[lw + ones] = a;
-                        const int j = t - lw;
-                        if (j < m) xa[j == 0 ? rw : lw + k1 + j] = src[lw + nrot + j];
+                bool again = false;
+                if (inwin && t - lw < pushes) {
+                    const int v = src[t];
+                    if (v < 0) {
+                        const int u = src[-v - 1];
+                        src[t] = u;
+                        again = u < 0;
                     }
                 }
-                __syncthreads();
-                if (live) a = xa[t];
-                // next window of each QuickSelect
-                bool more = false;
-                if (live && t == R && rw >= 0) {
-                    const int pos = lw + k1;
-                    const int k = R + (H - R) / 2;
-                    int nl = lw, nr = rw;
-                    if (pos == k) nr = -1;
-                    else if (k < pos) nr = pos - 1;
-                    else nl = pos + 1;
-                    if (nr >= 0 && nl == nr) nr = -1;
-                    wl[t] = nl;
-                    wr[t] = nr;
-                    more = nr >= 0;
-                }
-                if (!__syncthreads_or(more)) break;
+                if (!__syncthreads_or(again)) break;
             }
-            // split every range at its median position
-            if (live && H - R >= 2) {
-                const int m = R + (H - R) / 2;
-                if (t < m) H = m;
-                else R = m;
+            // scatter: "<=" elements in order, the pivot after them, the
+            // queue from its second element on, its front at the right end
+            if (active) {
+                if (t == rw) {
+                    xa[lw + k1] = a;
+                } else if (inwin) {
+                    if (le) xa[lw + ones] = a;
+                    const int j = t - lw;
+                    if (j < m) xa[j == 0 ? rw : lw + k1 + j] = src[lw + nrot + j];
+                }
+            }
+            __syncthreads();
+            if (live) a = xa[t];
+            if (active) {
+                // next window of the range's QuickSelect
+                const int pos = lw + k1;
+                const int k = R + (H - R) / 2;
+                int nl = lw, nr = rw;
+                if (pos == k) nr = -1;
+                else if (k < pos) nr = pos - 1;
+                else nl = pos + 1;
+                if (nr >= 0 && nl == nr) nr = -1;
+                lw = nl;
+                rw = nr;
             }
         }
         // DFS order: position p holds element arrangement[n-1-p]

@@ -16,8 +16,9 @@
 //                  join box, each lane testing one candidate against every
 //                  annulus and check(); children appended in any order.
 //   k_sortkey      complete matches sorted by (molecule, template in scan
-//                  order, level positions): the reference's depth-first
-//                  order and each match's rank.
+//   k_rank         order, level positions): the reference's depth-first
+//                  order and each match's rank (radix passes, or rank
+//                  counting for a few thousand matches).
 //   k_super        thread per complete match: FP64 superposition and RMSD
 //                  threshold, then (after a scan of the marks) the events in
 //                  that same order, each with the match's rank.
@@ -564,16 +565,13 @@ __global__ void __launch_bounds__(256) k_iota(int *v, long long n)
     if (i < n) v[i] = (int)i;
 }
 
-// Sort key of sorted position s for levels [l0, l0 + per): the positions of
+// Sort key of complete match i for levels [l0, l0 + per): the positions of
 // the match's atoms in their sets (list position at level 0, kd position
 // above), `bits` bits each, the earliest level most significant; levels past
 // the template's size count as 0 (all matches of a pair have the same size).
-__global__ void __launch_bounds__(256) k_sortkey(DevSets st, DevTpls tp, DevPairs pr, DevJoin jn, const int *perm,
-                                                 long long n, int l0, int per, int bits, unsigned long long *key)
+__device__ static inline unsigned long long level_key(const DevSets &st, const DevTpls &tp, const DevPairs &pr,
+                                                      const DevJoin &jn, int i, int l0, int per, int bits)
 {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int i = perm[s];
     const int p = jn.done_pair[i];
     const int t = p % tp.n_templates;
     const int nt = tp.atom_off[t + 1] - tp.atom_off[t];
@@ -586,21 +584,105 @@ __global__ void __launch_bounds__(256) k_sortkey(DevSets st, DevTpls tp, DevPair
         else if (l < nt) pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
         k = (k << bits) | pos;
     }
-    key[s] = k;
+    return k;
 }
 
 // Last sort key: the molecule, then the template in scan order (Jess scans
 // templates in reverse insertion order, patches/Jess.c.patch:63-72), so the
 // complete matches end up grouped by pair in the order the caller reports
 // them.
+__device__ static inline unsigned int pair_key(const DevJoin &jn, int n_templates, int i)
+{
+    const int p = jn.done_pair[i];
+    const int t = p % n_templates;
+    return (unsigned int)(p - t + (n_templates - 1 - t));
+}
+
+__global__ void __launch_bounds__(256) k_sortkey(DevSets st, DevTpls tp, DevPairs pr, DevJoin jn, const int *perm,
+                                                 long long n, int l0, int per, int bits, unsigned long long *key)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    key[s] = level_key(st, tp, pr, jn, perm[s], l0, per, bits);
+}
+
 __global__ void __launch_bounds__(256) k_pairkey(DevJoin jn, int n_templates, const int *perm, long long n,
                                                  unsigned int *key)
 {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const int p = jn.done_pair[perm[s]];
-    const int t = p % n_templates;
-    key[s] = (unsigned int)(p - t + (n_templates - 1 - t));
+    key[s] = pair_key(jn, n_templates, perm[s]);
+}
+
+// Small sorts (at most SORT_RANK_MAX complete matches whose full key fits
+// SORT_RANK_WORDS words) go by rank counting instead of radix passes: a few
+// launches whatever the number of levels.
+#ifndef SORT_RANK_MAX
+#define SORT_RANK_MAX 8192
+#endif
+#define SORT_RANK_WORDS 4
+
+// Full sort key of complete match i as a W-word big-endian bit string: the
+// pair key (pbits bits), then the level positions of k_sortkey (bits bits
+// each, level 0 first). Clears the match's rank.
+template <int W>
+__global__ void __launch_bounds__(256) k_fullkey(DevSets st, DevTpls tp, DevPairs pr, DevJoin jn, long long n,
+                                                 int n_levels, int bits, int pbits, unsigned long long *key,
+                                                 int *rank)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long k[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) k[w] = 0;
+    auto push = [&](unsigned long long v, int nb) {
+#pragma unroll
+        for (int w = 0; w < W - 1; w++) k[w] = (k[w] << nb) | (k[w + 1] >> (64 - nb));
+        k[W - 1] = (k[W - 1] << nb) | v;
+    };
+    push(pair_key(jn, tp.n_templates, (int)i), pbits);
+    for (int l = 0; l < n_levels; l++) push(level_key(st, tp, pr, jn, (int)i, l, 1, bits), bits);
+#pragma unroll
+    for (int w = 0; w < W; w++) key[w * n + i] = k[w];
+    rank[i] = 0;
+}
+
+// rank[i] += the matches of a 256-wide tile of j that sort before match i
+// (smaller key, or equal key and j < i: the order of a stable sort)
+template <int W>
+__global__ void __launch_bounds__(256) k_rank(const unsigned long long *key, long long n, int *rank)
+{
+    __shared__ unsigned long long tile[W][256];
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long j0 = (long long)blockIdx.y * 256;
+    const long long jt = j0 + threadIdx.x;
+#pragma unroll
+    for (int w = 0; w < W; w++) tile[w][threadIdx.x] = jt < n ? key[w * n + jt] : 0ull;
+    __syncthreads();
+    if (i >= n) return;
+    unsigned long long ki[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) ki[w] = key[w * n + i];
+    const int m = (int)min(256LL, n - j0);
+    int cnt = 0;
+    for (int jj = 0; jj < m; jj++) {
+        int cmp = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (cmp == 0) {
+                const unsigned long long kj = tile[w][jj];
+                cmp = kj < ki[w] ? -1 : (kj > ki[w] ? 1 : 0);
+            }
+        }
+        cnt += (cmp < 0 || (cmp == 0 && j0 + jj < i)) ? 1 : 0;
+    }
+    if (cnt) atomicAdd(&rank[i], cnt);
+}
+
+__global__ void __launch_bounds__(256) k_rank_scatter(const int *rank, long long n, int *perm)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[rank[i]] = (int)i;
 }
 
 // first sorted position of every pair with complete matches
@@ -679,18 +761,23 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets s
     for (int k = 0; k < 9; k++) nan |= isnan(rot[k]);
     const long long e = pos[s];
     if (e >= ev.cap) return;
-    ev.pair[e] = pair;
-    ev.rank[e] = s - first[pair];
-    ev.flag[e] = nan ? 2 : 1;
-    ev.rmsd[e] = rmsd;
+    const EventLayout L = event_layout((int64_t)*ev.count, ev.stride);
+    ((int *)(ev.base + L.pair))[e] = pair;
+    ((long long *)(ev.base + L.rank))[e] = s - first[pair];
+    ((int *)(ev.base + L.flag))[e] = nan ? 2 : 1;
+    ((double *)(ev.base + L.rmsd))[e] = rmsd;
+    double *erot = (double *)(ev.base + L.rot) + 9 * e;
+    double *ec0 = (double *)(ev.base + L.c0) + 3 * e;
+    double *ec1 = (double *)(ev.base + L.c1) + 3 * e;
 #pragma unroll
-    for (int k = 0; k < 9; k++) ev.rot[9 * e + k] = rot[k];
+    for (int k = 0; k < 9; k++) erot[k] = rot[k];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        ev.c0[3 * e + d] = c0[d];
-        ev.c1[3 * e + d] = c1[d];
+        ec0[d] = c0[d];
+        ec1[d] = c1[d];
     }
-    for (int k = 0; k < ev.stride; k++) ev.atoms[e * (size_t)ev.stride + k] = k < n ? atoms[k] : -1;
+    int *eat = (int *)(ev.base + L.atoms) + e * (size_t)ev.stride;
+    for (int k = 0; k < ev.stride; k++) eat[k] = k < n ? atoms[k] : -1;
 }
 
 // number of events: the scan's last position plus the last mark
