@@ -919,7 +919,13 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             // not a set has ties.
             const bool direct = (M == 1);
             // largest tier first: its exact pass, the longest, starts earliest
-            for (int k = KD_TIERS - 1; k >= 0; k--) {
+            // (with one molecule the direct tiers first: sets above 1,024
+            // entries are rare there)
+            static const int by_size[KD_TIERS] = {5, 4, 3, 2, 1, 0};
+            static const int one_mol[KD_TIERS] = {4, 3, 5, 2, 1, 0};
+            static_assert(KD_TIERS == 6, "tier orders list six tiers");
+            for (int ki = 0; ki < KD_TIERS; ki++) {
+                const int k = direct ? one_mol[ki] : by_size[ki];
                 const bool all = direct && (k == 3 || k == 4);
                 if (all) {
                     HIP_TRY(hipEventRecord(c->kd_fork[k], s));
@@ -933,6 +939,10 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 }
                 HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
             }
+            // the join's counters and level-0 flags do not depend on the kd
+            // order: set them up while the exact passes still run
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnts, (long long)CNT_TOTAL);
+            hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
             for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
             HIP_TRY(hipGetLastError());
         }
@@ -946,7 +956,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         unsigned long long *fcount = cnts;   // rows per level
         unsigned long long *n_done_dev = cnts + CNT_DONE;
         unsigned int *overflow = (unsigned int *)(cnts + CNT_OVF);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, fcount, (long long)CNT_TOTAL);
         DevJoin jn{};
         jn.stride = c->front_stride;
         jn.n_done = n_done_dev;
@@ -960,7 +969,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             jn.done_ent = c->d_ent.as<int>();
         }
         {
-            hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
             jn.level = 0;
             jn.n_out = fcount;
             jn.out_cap = c->front_cap;
@@ -1047,13 +1055,13 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             HIP_TRY(c->s_key[0].reserve(sizeof(unsigned long long) * (size_t)n_done * words));
             unsigned long long *fk = c->s_key[0].as<unsigned long long>();
             int *rank = c->s_perm[1].as<int>();
-            const unsigned tiles = (unsigned)((n_done + 255) / 256);
+            const dim3 tiles((unsigned)((n_done + 255) / 256), (unsigned)((n_done + SORT_RANK_TJ - 1) / SORT_RANK_TJ));
             switch (words) {
 #define JAM_RANK_SORT(W)                                                                                        \
     case W:                                                                                                     \
         hipLaunchKernelGGL(jam::k_fullkey<W>, dim3(gs), dim3(256), 0, s, st, dt, pr, jn, (long long)n_done, stride, \
                            bits, pbits, fk, rank);                                                              \
-        hipLaunchKernelGGL(jam::k_rank<W>, dim3(tiles, tiles), dim3(256), 0, s, (const unsigned long long *)fk,   \
+        hipLaunchKernelGGL(jam::k_rank<W>, tiles, dim3(256), 0, s, (const unsigned long long *)fk,                \
                            (long long)n_done, rank);                                                            \
         break;
                 JAM_RANK_SORT(1)

@@ -647,34 +647,39 @@ __global__ void __launch_bounds__(256) k_fullkey(DevSets st, DevTpls tp, DevPair
     rank[i] = 0;
 }
 
-// rank[i] += the matches of a 256-wide tile of j that sort before match i
-// (smaller key, or equal key and j < i: the order of a stable sort)
+// rank[i] += the matches of a SORT_RANK_TJ-wide tile of j that sort before
+// match i (smaller key, or equal key and j < i: the order of a stable sort).
+// Narrow j tiles give many workgroups; the comparisons are branch-free so
+// that the tile reads of several j are in flight at once.
+#define SORT_RANK_TJ 64
 template <int W>
 __global__ void __launch_bounds__(256) k_rank(const unsigned long long *key, long long n, int *rank)
 {
-    __shared__ unsigned long long tile[W][256];
+    __shared__ unsigned long long tile[W][SORT_RANK_TJ];
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long j0 = (long long)blockIdx.y * 256;
-    const long long jt = j0 + threadIdx.x;
+    const long long j0 = (long long)blockIdx.y * SORT_RANK_TJ;
+    if (threadIdx.x < SORT_RANK_TJ) {
+        const long long jt = j0 + threadIdx.x;
 #pragma unroll
-    for (int w = 0; w < W; w++) tile[w][threadIdx.x] = jt < n ? key[w * n + jt] : 0ull;
+        for (int w = 0; w < W; w++) tile[w][threadIdx.x] = jt < n ? key[w * n + jt] : ~0ull;
+    }
     __syncthreads();
     if (i >= n) return;
     unsigned long long ki[W];
 #pragma unroll
     for (int w = 0; w < W; w++) ki[w] = key[w * n + i];
-    const int m = (int)min(256LL, n - j0);
+    // slots past n hold all-ones keys and j >= n: never before i
+    const long long lim = n - j0;
     int cnt = 0;
-    for (int jj = 0; jj < m; jj++) {
-        int cmp = 0;
+#pragma unroll 8
+    for (int jj = 0; jj < SORT_RANK_TJ; jj++) {
+        bool lt = jj < lim && j0 + jj < i;   // key equal so far: the index decides
 #pragma unroll
-        for (int w = 0; w < W; w++) {
-            if (cmp == 0) {
-                const unsigned long long kj = tile[w][jj];
-                cmp = kj < ki[w] ? -1 : (kj > ki[w] ? 1 : 0);
-            }
+        for (int w = W - 1; w >= 0; w--) {
+            const unsigned long long kj = tile[w][jj];
+            lt = (kj < ki[w]) || (kj == ki[w] && lt);
         }
-        cnt += (cmp < 0 || (cmp == 0 && j0 + jj < i)) ? 1 : 0;
+        cnt += lt ? 1 : 0;
     }
     if (cnt) atomicAdd(&rank[i], cnt);
 }
