@@ -7,6 +7,7 @@
 // query; a kernel that would overflow one skips its writes (the counters keep
 // counting) and the host grows the buffer to the exact size and re-runs the
 // chain, so results never depend on capacities.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -100,7 +101,7 @@ struct jam_context {
     // start/stop events around every k_join launch (one per join level), for
     // the join kernels' own duration
     hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
-    bool profile = false;       // record xev (each record costs stream time)
+    bool profile = false;       // join launches carry start/stop events (xev)
     // per-query workspace
     DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, kscr, kd_bucket, kd_tied, kd_count;
     DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair;
@@ -699,15 +700,17 @@ int join_workgroups(jam_context *c, int stride)
 }
 
 // one join level; the grid is the number of workgroups that stay resident
+// e0, e1 (profiling, else null): events the runtime records at the
+// dispatch's own start and end
 void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, const DevTpls &dt, const DevSets &st,
-                 const DevPairs &pr, const DevJoin &jn, const QueryParams &qp)
+                 const DevPairs &pr, const DevJoin &jn, const QueryParams &qp, hipEvent_t e0, hipEvent_t e1)
 {
     const int v = stride <= 8 ? 0 : stride <= 16 ? 1 : stride <= 24 ? 2 : 3;
     const dim3 g((unsigned)join_workgroups(c, stride)), b(JOIN_WAVES * JAM_WAVE);
-    if (v == 0) hipLaunchKernelGGL(JOIN_KERNEL(8), g, b, 0, s, dm, dt, st, pr, jn, qp);
-    else if (v == 1) hipLaunchKernelGGL(JOIN_KERNEL(16), g, b, 0, s, dm, dt, st, pr, jn, qp);
-    else if (v == 2) hipLaunchKernelGGL(JOIN_KERNEL(24), g, b, 0, s, dm, dt, st, pr, jn, qp);
-    else hipLaunchKernelGGL(JOIN_KERNEL(32), g, b, 0, s, dm, dt, st, pr, jn, qp);
+    if (v == 0) hipExtLaunchKernelGGL(JOIN_KERNEL(8), g, b, 0, s, e0, e1, 0, dm, dt, st, pr, jn, qp);
+    else if (v == 1) hipExtLaunchKernelGGL(JOIN_KERNEL(16), g, b, 0, s, e0, e1, 0, dm, dt, st, pr, jn, qp);
+    else if (v == 2) hipExtLaunchKernelGGL(JOIN_KERNEL(24), g, b, 0, s, e0, e1, 0, dm, dt, st, pr, jn, qp);
+    else hipExtLaunchKernelGGL(JOIN_KERNEL(32), g, b, 0, s, e0, e1, 0, dm, dt, st, pr, jn, qp);
 }
 
 }  // namespace
@@ -988,10 +991,9 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 jn.out_cap = c->front_cap;
                 jn.out_pair = c->j_pair[bo].as<int>();
                 jn.out_ent = c->j_ent[bo].as<int>();
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[2 * (L - 1) + 0], s));
-                launch_join(c, stride, s, dm, dt, st, pr, jn, qp);
+                launch_join(c, stride, s, dm, dt, st, pr, jn, qp, c->profile ? c->xev[2 * (L - 1) + 0] : nullptr,
+                            c->profile ? c->xev[2 * (L - 1) + 1] : nullptr);
                 HIP_TRY(hipGetLastError());
-                if (c->profile) HIP_TRY(hipEventRecord(c->xev[2 * (L - 1) + 1], s));
             }
         }
         unsigned long long *jc = c->hsmall;
