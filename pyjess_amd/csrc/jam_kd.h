@@ -389,6 +389,11 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
 // queue's final contents fill the rest, its front at rw.
 // ---------------------------------------------------------------------------
 
+// links of a pop chain followed per thread before the pointer jumping
+#ifndef KD_CHASE
+#define KD_CHASE 8
+#endif
+
 template <int NT>
 __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets st, const int *list,
                                                 const unsigned int *count)
@@ -427,12 +432,13 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
             const double *c = cc[d % 3];
             const bool active = live && rw >= 0;
             double pv = 0.0;
+            int piv = -1, er = 0;   // pivot position, and the element it takes in the swap
             if (active) {
                 // pivot: median of three, swapped with the window's right end
                 const int mid = (lw + rw + 1) / 2;
-                const int el = xa[lw], em = xa[mid], er = xa[rw];
+                const int el = xa[lw], em = xa[mid];
+                er = xa[rw];
                 const double kl = c[el], km = c[em], kr = c[er];
-                int piv;
                 if ((kl > km) != (kl > kr)) piv = lw;
                 else if ((km < kl) != (km < kr)) piv = mid;
                 else piv = rw;
@@ -491,17 +497,36 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
             const int m = (rw - lw) - k1;     // ">" elements
             const int nrot = k1 - (f - lw);   // pops: "<=" elements after f
             const int pushes = rw - f;
+            bool again = false;
             if (inwin && t >= f) {
                 // push t - f; a "<=" element pushes what pop number
-                // ones - (f - lw) takes, which is push number ones - (f - lw)
-                if (!le) src[lw + (t - f)] = a;
-                else src[lw + (t - f)] = -(lw + ones - (f - lw)) - 1;
+                // r = ones - (f - lw) takes, which is push number r, made at
+                // position f + r: that position's element if it is a ">" one
+                // (its "<=" flag, read off the prefix counts, is 0), else a
+                // link to push r (the first pointer-jumping step)
+                if (!le) {
+                    src[lw + (t - f)] = a;
+                } else {
+                    // follow the chain a few links on the prefix counts alone
+                    int r = ones - (f - lw), q = f + r;
+                    bool found = xp1[q + 1] == xp1[q];
+                    for (int hop = 1; hop < KD_CHASE && !found; hop++) {
+                        r = (xp1[q] - p1l) - (f - lw);
+                        q = f + r;
+                        found = xp1[q + 1] == xp1[q];
+                    }
+                    if (found) {
+                        src[lw + (t - f)] = q == piv ? er : xa[q];
+                    } else {
+                        src[lw + (t - f)] = -(lw + r) - 1;
+                        again = true;
+                    }
+                }
             }
-            __syncthreads();
             // pointer jumping until every push names an element; each pop
             // names a lower push, so the chains end
-            for (;;) {
-                bool again = false;
+            while (__syncthreads_or(again)) {
+                again = false;
                 if (inwin && t - lw < pushes) {
                     const int v = src[t];
                     if (v < 0) {
@@ -510,7 +535,6 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                         again = u < 0;
                     }
                 }
-                if (!__syncthreads_or(again)) break;
             }
             // scatter: "<=" elements in order, the pivot after them, the
             // queue from its second element on, its front at the right end
