@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
 // per thread. The arrangement, prefix counts and pushes live in LDS; every
 // thread keeps its own element, its range and the QuickSelect window of that
 // range in registers, and computes the window updates itself, so one
-// partition step costs four barriers plus the pointer-jumping rounds.
+// partition step costs three barriers plus the pointer-jumping rounds.
 //
 // One partition step of a window [lw, rw] (pivot moved to rw) in the
 // reference's order: the "<=" elements keep their order at the front; every
@@ -502,12 +502,12 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                 // push t - f; a "<=" element pushes what pop number
                 // r = ones - (f - lw) takes, which is push number r, made at
                 // position f + r: that position's element if it is a ">" one
-                // (its "<=" flag, read off the prefix counts, is 0), else a
-                // link to push r (the first pointer-jumping step)
+                // (its "<=" flag, read off the prefix counts, is 0), else
+                // what that position's own pop took, up to KD_CHASE links;
+                // a chain still open then is a link for the pointer jumping
                 if (!le) {
                     src[lw + (t - f)] = a;
                 } else {
-                    // follow the chain a few links on the prefix counts alone
                     int r = ones - (f - lw), q = f + r;
                     bool found = xp1[q + 1] == xp1[q];
                     for (int hop = 1; hop < KD_CHASE && !found; hop++) {
