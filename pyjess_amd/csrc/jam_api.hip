@@ -121,6 +121,7 @@ struct jam_context {
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
     int64_t sort_cap = 0;
     DBuf e_all;  // the events of the last query, event_layout(count, stride)
+    DBuf keep_d, keep_i;  // superpositions kept between the two k_super passes (few matches)
     DBuf scan_tmp, scan_tmp2;
     int64_t set_cap = 0, blk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
@@ -1117,16 +1118,27 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                            (long long)n_pairs + 1);
         if (n_done > 0) {
             const long long *first = c->p_first.as<long long>();
+            // few matches: the mark pass keeps its superpositions for the emit
+            // pass (each is a chain of dependent FP64 work, long next to
+            // reading it back)
+            double *keep_d = nullptr;
+            int *keep_i = nullptr;
+            if (n_done <= SORT_RANK_MAX) {
+                HIP_TRY(c->keep_d.reserve(sizeof(double) * 16 * (size_t)n_done));
+                HIP_TRY(c->keep_i.reserve(sizeof(int) * (size_t)(stride + 2) * (size_t)n_done));
+                keep_d = c->keep_d.as<double>();
+                keep_i = c->keep_i.as<int>();
+            }
             hipLaunchKernelGGL(jam::k_first, dim3(gs), dim3(256), 0, s, jn, perm, (long long)n_done,
                                c->p_first.as<long long>());
             hipLaunchKernelGGL(jam::k_super<false>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
-                               (long long)n_done, first, sel, (const int *)pos);
+                               (long long)n_done, first, sel, (const int *)pos, keep_d, keep_i);
             size_t tb = c->s_tmp.bytes;
             HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->s_tmp.ptr, tb, (const int *)sel, pos, (int)n_done, s));
             hipLaunchKernelGGL(jam::k_event_total, dim3(1), dim3(64), 0, s, (const int *)sel, (const int *)pos,
                                (long long)n_done, ev.count);
             hipLaunchKernelGGL(jam::k_super<true>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
-                               (long long)n_done, first, sel, (const int *)pos);
+                               (long long)n_done, first, sel, (const int *)pos, keep_d, keep_i);
             HIP_TRY(hipGetLastError());
         }
         HIP_TRY(hipEventRecord(c->ev[4], s));

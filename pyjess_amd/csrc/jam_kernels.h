@@ -745,20 +745,53 @@ __device__ __attribute__((always_inline)) static inline int super_of(DevMols ms,
 // superposition of the marked ones and writes their events at pos[s], so the
 // events keep the order of the sorted matches without a sort of their own.
 // Each event carries the match's rank within its pair.
+// With keep_d / keep_i (few matches), pass 1 also keeps each superposition
+// (16 doubles, then pair, size and atoms as stride + 2 ints per match) and
+// pass 2 reads it back instead of computing it again.
 template <bool EMIT>
 __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets st, DevPairs pr, DevJoin jn,
                                                DevEvents ev, QueryParams qp, const int *perm, long long n_matches,
-                                               const long long *first, int *sel, const int *pos)
+                                               const long long *first, int *sel, const int *pos, double *keep_d,
+                                               int *keep_i)
 {
     const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_matches) return;
     if (EMIT && !sel[s]) return;
     int atoms[JAM_MAX_LEVELS];
-    int n;
+    int n, pair;
     double rmsd, rot[9], c0[3], c1[3];
-    const int pair = super_of(ms, tp, st, pr, jn, perm, s, atoms, &n, &rmsd, rot, c0, c1);
+    double *kd = keep_d ? keep_d + 16 * s : nullptr;
+    int *ki = keep_i ? keep_i + (size_t)(ev.stride + 2) * s : nullptr;
+    if (EMIT && kd) {
+        rmsd = kd[0];
+#pragma unroll
+        for (int k = 0; k < 9; k++) rot[k] = kd[1 + k];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            c0[d] = kd[10 + d];
+            c1[d] = kd[13 + d];
+        }
+        pair = ki[0];
+        n = ki[1];
+        for (int k = 0; k < n; k++) atoms[k] = ki[2 + k];
+    } else {
+        pair = super_of(ms, tp, st, pr, jn, perm, s, atoms, &n, &rmsd, rot, c0, c1);
+    }
     if (!EMIT) {
         sel[s] = (rmsd <= qp.rmsd_threshold) ? 1 : 0;
+        if (kd) {
+            kd[0] = rmsd;
+#pragma unroll
+            for (int k = 0; k < 9; k++) kd[1 + k] = rot[k];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                kd[10 + d] = c0[d];
+                kd[13 + d] = c1[d];
+            }
+            ki[0] = pair;
+            ki[1] = n;
+            for (int k = 0; k < n; k++) ki[2 + k] = atoms[k];
+        }
         return;
     }
     bool nan = false;
