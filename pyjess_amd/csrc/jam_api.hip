@@ -637,16 +637,6 @@ __global__ void k_zero_u32(unsigned int *v, int n)
     if ((int)threadIdx.x < n) v[threadIdx.x] = 0u;
 }
 
-__global__ void k_total(const int *cnt, const int64_t *off, int n, int64_t *total)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
-}
-
-__global__ void k_total_items(const int *cnt, const long long *off, long long n, long long *total)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? off[n - 1] + cnt[n - 1] : 0;
-}
-
 __global__ void k_items_any(DevPairs pr)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -808,22 +798,21 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         const int wpb = 4;  // waves per block of the per-set candidate kernels
         const int set_blocks = (n_sets + wpb - 1) / wpb;
         HIP_TRY(hipEventRecord(c->ev[0], s));
-        hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st, pr,
+                           (long long *)nullptr);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_sets, st.cnt, st.off, n_sets, s));
         HIP_TRY(hipcub::DeviceReduce::Max(c->scan_tmp.ptr, scan_max, st.cnt, st.maxcnt, n_sets, s));
-        hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, s, st.cnt, st.off, n_sets, st.total);
         hipLaunchKernelGGL(jam::k_pairs, dim3((unsigned)((n_pairs + 7) / 8)), dim3(256), 0, s, dt, st, pr, qp);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_pairs, pr.nchunk, pr.item_base, n_pairs,
                                                  s));
-        hipLaunchKernelGGL(k_total_items, dim3(1), dim3(1), 0, s, pr.nchunk, pr.item_base, (long long)n_pairs,
-                           (long long *)(cnts + CNT_ITEMS));
         // no read-back here: the buffers keep the sizes of earlier queries,
         // every kernel below checks the sizes on the device, and the host
         // reads them with the join's counters, growing and running the query
         // again when something did not fit
-        hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st);
+        hipLaunchKernelGGL(jam::k_candidates<true>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st, pr,
+                           (long long *)(cnts + CNT_ITEMS));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
         {

@@ -79,8 +79,14 @@ __device__ static inline int find_bucket(const DevMols &ms, int b0, int b1, uint
 // ---------------------------------------------------------------------------
 
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevSets st)
+__global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
+                                                    long long *n_items)
 {
+    if (FILL && blockIdx.x == 0 && threadIdx.x == 0) {
+        // level-0 items of all pairs: the end of the scan of the pair chunks
+        const int np = pr.n_pairs;
+        *n_items = np > 0 ? pr.item_base[np - 1] + pr.nchunk[np - 1] : 0;
+    }
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) / JAM_WAVE);
     const int lane = lane_id();
     if (wave >= st.n_sets) return;
@@ -133,6 +139,11 @@ __global__ void __launch_bounds__(256) k_candidates(DevMols ms, DevTpls tp, DevS
 // reordering, k otherwise.
 __global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs pr, QueryParams qp)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // candidate-set entries of the query: the end of the scan of the set sizes
+        const int ns = st.n_sets;
+        *st.total = ns > 0 ? st.off[ns - 1] + st.cnt[ns - 1] : 0;
+    }
     const int lane = lane_id();
     const int team = lane >> 5, k = lane & 31;
     const int p = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / JAM_WAVE) * 2 + team;
