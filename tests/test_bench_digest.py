@@ -35,3 +35,17 @@ def test_vectorised_digest_matches_oracle_and_scalar_definition():
         m, t = divmod(int(p), T)
         want[p] = (want[p] + bench.hit_digest(t, m, r[:len(tpls[t])])) & bench._M64
     assert [int(x) for x in digests] == want
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per run: --gpus above 1 exits with status 2 and says why,
+    # before anything touches a device
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300, cwd=root)
+    assert r.returncode == 2
+    assert "--gpus 2 is not supported" in r.stderr
+    assert r.stdout.strip() == ""
