@@ -172,28 +172,47 @@ int32_t jam_superpose_batch(jam_context *ctx, const double *a, const double *b, 
                             int64_t n_sets, double *rmsd, double *rotation9, double *c0_3,
                             double *c1_3);
 
+/* ---- standalone kd-tree leaf order ----------------------------------- */
+
+/* Replaces KdTree_create / KdTreeNode_create (patches/KdTree.c.patch:130-185,
+ * 348-420, with qselect_r, patches/qselect.h.patch:50-116) for a batch of
+ * point sets: the leaf order, left subtree first, of the kd-tree the
+ * reference builds over each set (median split at n/2 on axis depth % 3 by
+ * median-of-three QuickSelect). xyz: 3 doubles per point, host memory; set s
+ * owns points [off[s], off[s+1]) (off[0] == 0, never decreasing);
+ * order[off[s] + i] receives the set-local index of the point at leaf i. The
+ * same device passes order every candidate set of jam_query. */
+int32_t jam_kd_order(jam_context *ctx, const double *xyz, const int64_t *off, int32_t n_sets, int32_t *order);
+
 /* ---- differentiable superposition RMSD --------------------------------- */
 
 /* RMSD of the least-squares superposition of a onto b for a batch of point
- * sets of any sizes, all arrays in DEVICE memory: set s owns points
- * [off[s], off[s+1]) of a and b (3 doubles per point, x y z); rmsd receives
- * n_sets values, rot9 (may be NULL) the 9 row-major entries of each optimal
- * proper rotation. The quantity of Superposition_rmsd/rotation
+ * sets of any sizes, all arrays in DEVICE memory: a and b hold n_points
+ * points (3 doubles each, x y z); set s owns points [off[s], off[s+1]), with
+ * off[0] == 0, off never decreasing and off[n_sets] == n_points. rmsd
+ * receives n_sets values, rot9 (may be NULL) the 9 row-major entries of each
+ * optimal proper rotation. The quantity of Superposition_rmsd/rotation
  * (include/jess/super.pxd:1-14), computed for differentiation: rotation from
  * a one-sided Jacobi SVD of the correlation, RMSD from the residuals, so that
  * near-degenerate sets keep full precision (a set of rank < 2 gets one of its
  * optimal rotations instead of NaN). Asynchronous on `stream` (a hipStream_t;
- * NULL for the default stream). */
-int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+ * NULL for the default stream): the offsets are not read on the host, so the
+ * call cannot reject malformed ones; the kernels clamp every set's range into
+ * [0, n_points), so malformed offsets give meaningless values but never an
+ * access outside a and b. */
+int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, int64_t n_sets, int64_t n_points,
                          double *rmsd, double *rot9, void *stream);
 /* Gradient of sum_s grad_rmsd[s] * rmsd[s] with respect to a and b (device
  * arrays laid out like a and b; either may be NULL), given the rmsd of
  * jam_rmsd_forward and, optionally, its rotations (rot9; NULL recomputes
- * them). No counterpart in the reference, which has no gradients:
+ * them). Every point of a set with at least one point is written, so with
+ * valid offsets every row of grad_a and grad_b is. No counterpart in the
+ * reference, which has no gradients:
  * d rmsd / d a_i = (A_i - R^T B_i) / (n rmsd) and
  * d rmsd / d b_i = (B_i - R A_i) / (n rmsd), A and B the centred points, R
- * the optimal proper rotation; zero where rmsd is 0. Asynchronous on `stream`. */
-int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
+ * the optimal proper rotation; zero where rmsd is 0. Asynchronous on
+ * `stream`, offsets clamped as in jam_rmsd_forward. */
+int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets, int64_t n_points,
                           const double *rmsd, const double *rot9, const double *grad_rmsd, double *grad_a,
                           double *grad_b, void *stream);
 

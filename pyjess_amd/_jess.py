@@ -229,19 +229,22 @@ class Jess:
         return hits
 
 
-_last_batch = [None, None, None]   # (context, molecules, device copy) of the latest batch
+# (context, molecules, their data versions, device copy) of the latest batch
+_last_batch = [None, None, None, None]
 
 
 def _batch_cache(ctx, molecules: Sequence[Molecule]) -> "_native.MoleculeSet":
     """Device copy of a batch of molecules; the latest one is kept, so a batch
     queried again (against other templates, or repeatedly) is not re-uploaded.
-    The cache holds the molecule objects, which keeps their identities valid."""
-    ctx0, mols0, mset0 = _last_batch
+    The cache holds the molecule objects, which keeps their identities valid,
+    and the version of each one's data (bumped when an array is replaced)."""
+    ctx0, mols0, vers0, mset0 = _last_batch
+    vers = [m.__dict__.get("_version", 0) for m in molecules]
     if ctx0 is ctx and mols0 is not None and len(mols0) == len(molecules) and \
-            all(a is b for a, b in zip(mols0, molecules)):
+            all(a is b for a, b in zip(mols0, molecules)) and vers == vers0:
         return mset0
     mset = _native.MoleculeSet(ctx, molecules)
-    _last_batch[:] = [ctx, list(molecules), mset]
+    _last_batch[:] = [ctx, list(molecules), vers, mset]
     return mset
 
 

@@ -404,13 +404,40 @@ class Atom:
 # Molecule
 # --------------------------------------------------------------------------
 
+_MOL_ARRAYS = frozenset(("xyz", "names", "resnames", "chains", "resseq", "serial", "altloc", "icode",
+                         "occupancy", "tempfactor", "segid", "element", "charge"))
+
+
+def _frozen(a: np.ndarray) -> np.ndarray:
+    """A read-only array that no one else can write: `a` itself when it owns
+    its data and is already read-only, else a read-only copy."""
+    if not (isinstance(a, np.ndarray) and a.flags.owndata and not a.flags.writeable):
+        a = np.array(a, copy=True)
+        a.setflags(write=False)
+    return a
+
+
 class Molecule:
     """A molecule, stored as structure-of-arrays (``_jess.pyx:243-907``).
 
     The residue-name index of the reference (``ResIndex_create``, built in
     ``Molecule.__init__`` at ``_jess.pyx:682``) is built by the native library
     when the molecule is first uploaded to a device.
+
+    Like the reference's, a molecule does not change once built: its arrays
+    are read-only, so the device copies made for queries stay valid. Assigning
+    a new array to an attribute replaces that data (with a private read-only
+    copy) and drops the device copies.
     """
+
+    def __setattr__(self, name, value):
+        if name in _MOL_ARRAYS:
+            value = _frozen(value)
+            dev = self.__dict__.get("_device")
+            if dev:
+                dev.clear()
+            object.__setattr__(self, "_version", self.__dict__.get("_version", 0) + 1)
+        object.__setattr__(self, name, value)
 
     def __init__(self, atoms: Iterable[Atom] = (), id: Optional[str] = None,
                  date: Optional[datetime.date] = None, name: Optional[str] = None):
@@ -421,36 +448,42 @@ class Molecule:
         self._id = id
         self._date = date
         self._name = name
-        self.xyz = np.empty((n, 3), dtype=np.float64)
-        self.names = np.empty((n, 4), dtype=np.uint8)
-        self.resnames = np.empty((n, 4), dtype=np.uint8)
-        self.chains = np.empty((n, 2), dtype=np.uint8)
-        self.resseq = np.empty(n, dtype=np.int32)
-        self.serial = np.empty(n, dtype=np.int32)
-        self.altloc = np.empty(n, dtype=np.uint8)
-        self.icode = np.empty(n, dtype=np.uint8)
-        self.occupancy = np.empty(n, dtype=np.float32)
-        self.tempfactor = np.empty(n, dtype=np.float32)
-        self.segid = np.empty((n, 4), dtype=np.uint8)
-        self.element = np.empty((n, 2), dtype=np.uint8)
-        self.charge = np.empty(n, dtype=np.int32)
+        self._device = {}
+        xyz = np.empty((n, 3), dtype=np.float64)
+        names = np.empty((n, 4), dtype=np.uint8)
+        resnames = np.empty((n, 4), dtype=np.uint8)
+        chains = np.empty((n, 2), dtype=np.uint8)
+        resseq = np.empty(n, dtype=np.int32)
+        serial = np.empty(n, dtype=np.int32)
+        altloc = np.empty(n, dtype=np.uint8)
+        icode = np.empty(n, dtype=np.uint8)
+        occupancy = np.empty(n, dtype=np.float32)
+        tempfactor = np.empty(n, dtype=np.float32)
+        segid = np.empty((n, 4), dtype=np.uint8)
+        element = np.empty((n, 2), dtype=np.uint8)
+        charge = np.empty(n, dtype=np.int32)
         for i, a in enumerate(atoms):
             if not isinstance(a, Atom):
                 raise TypeError(f"expected Atom, found {type(a).__name__}")
-            self.xyz[i] = (a.x, a.y, a.z)
-            self.names[i] = np.frombuffer(a._name, dtype=np.uint8)
-            self.resnames[i] = np.frombuffer(a._resname, dtype=np.uint8)
-            self.chains[i] = (a._chain1, a._chain2)
-            self.resseq[i] = a.residue_number
-            self.serial[i] = a.serial
-            self.altloc[i] = a._altloc
-            self.icode[i] = a._icode
-            self.occupancy[i] = a._occupancy
-            self.tempfactor[i] = a._tempfactor
-            self.segid[i] = np.frombuffer(a._segid, dtype=np.uint8)
-            self.element[i] = np.frombuffer(a._element, dtype=np.uint8)
-            self.charge[i] = a.charge
-        self._device = {}
+            xyz[i] = (a.x, a.y, a.z)
+            names[i] = np.frombuffer(a._name, dtype=np.uint8)
+            resnames[i] = np.frombuffer(a._resname, dtype=np.uint8)
+            chains[i] = (a._chain1, a._chain2)
+            resseq[i] = a.residue_number
+            serial[i] = a.serial
+            altloc[i] = a._altloc
+            icode[i] = a._icode
+            occupancy[i] = a._occupancy
+            tempfactor[i] = a._tempfactor
+            segid[i] = np.frombuffer(a._segid, dtype=np.uint8)
+            element[i] = np.frombuffer(a._element, dtype=np.uint8)
+            charge[i] = a.charge
+        for attr, arr in (("xyz", xyz), ("names", names), ("resnames", resnames), ("chains", chains),
+                          ("resseq", resseq), ("serial", serial), ("altloc", altloc), ("icode", icode),
+                          ("occupancy", occupancy), ("tempfactor", tempfactor), ("segid", segid),
+                          ("element", element), ("charge", charge)):
+            arr.setflags(write=False)
+            setattr(self, attr, arr)
 
     @classmethod
     def from_arrays(cls, xyz, names, resnames, chains, resseq, serial=None, *,
@@ -463,6 +496,7 @@ class Molecule:
         mol._id = id
         mol._date = None
         mol._name = None
+        mol._device = {}
         mol.xyz = xyz
         mol.names = np.ascontiguousarray(names, dtype=np.uint8).reshape(n, 4)
         mol.resnames = np.ascontiguousarray(resnames, dtype=np.uint8).reshape(n, 4)
@@ -480,7 +514,6 @@ class Molecule:
         mol.element = (np.full((n, 2), ord("_"), dtype=np.uint8) if element is None
                        else np.ascontiguousarray(element, dtype=np.uint8).reshape(n, 2))
         mol.charge = np.zeros(n, dtype=np.int32)
-        mol._device = {}
         return mol
 
     # -- parsing ---------------------------------------------------------
@@ -624,6 +657,7 @@ class Molecule:
         out = np.empty_like(self.xyz)
         for i in range(len(self)):
             out[i] = _apply(mat, *self.xyz[i].tolist())
+        out.setflags(write=False)
         new.xyz = out
         return new
 

@@ -31,7 +31,7 @@ _SYMBOLS = (
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
     "jam_result_timings", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_rmsd_forward",
-    "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch", "jam_last_error",
+    "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch", "jam_kd_order", "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -149,9 +149,9 @@ def lib():
         L.jam_result_join_kernels.restype = i32
         L.jam_result_level_partials.argtypes = [vp, vp]
         L.jam_result_level_partials.restype = i32
-        L.jam_rmsd_forward.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+        L.jam_rmsd_forward.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp]
         L.jam_rmsd_forward.restype = i32
-        L.jam_rmsd_backward.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, vp, vp]
+        L.jam_rmsd_backward.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp]
         L.jam_rmsd_backward.restype = i32
         L.jam_context_set_profiling.argtypes = [vp, i32]
         L.jam_context_set_profiling.restype = i32
@@ -159,6 +159,8 @@ def lib():
         L.jam_result_free.restype = None
         L.jam_superpose_batch.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, vp]
         L.jam_superpose_batch.restype = i32
+        L.jam_kd_order.argtypes = [vp, vp, vp, i32, vp]
+        L.jam_kd_order.restype = i32
         L.jam_last_error.argtypes = []
         L.jam_last_error.restype = ctypes.c_char_p
         L.jam_version.argtypes = []
@@ -376,6 +378,21 @@ def superpose_batch(a: np.ndarray, b: np.ndarray, device: int = 0):
         _check(lib().jam_superpose_batch(ctx.handle, _ptr(a), _ptr(b), int(k), int(n), _ptr(rmsd),
                                          _ptr(rot), _ptr(c0), _ptr(c1)))
     return rmsd[:n], rot[:n].reshape(n, 3, 3), c0[:n], c1[:n]
+
+
+def kd_order(xyz: np.ndarray, off: np.ndarray, device: int = 0) -> np.ndarray:
+    """Leaf orders (left subtree first) of the kd-trees the reference builds
+    over point sets: set s is xyz[off[s]:off[s+1]]; element off[s] + i of the
+    result is the set-local index of the point at leaf i."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    if off.ndim != 1 or off.shape[0] < 1 or off[0] != 0 or off[-1] != xyz.shape[0]:
+        raise ValueError("off must start at 0 and end at the number of points")
+    out = np.zeros(max(xyz.shape[0], 1), dtype=np.int32)
+    ctx = context(device)
+    with ctx.lock:
+        _check(lib().jam_kd_order(ctx.handle, _ptr(xyz), _ptr(off), int(off.shape[0] - 1), _ptr(out)))
+    return out[:xyz.shape[0]]
 
 
 def synchronize(device: int = 0) -> None:

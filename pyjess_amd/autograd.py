@@ -1,10 +1,13 @@
 """Differentiable superposition RMSD on the GPU.
 
 ``superposition_rmsd(a, b)`` returns, for every pair of point sets, the RMSD
-of the least-squares superposition of ``a`` onto ``b`` -- the same FP64
-computation the search uses to decide hits (``Superposition_rmsd``,
-``include/jess/super.pxd:1-14``) -- and ``backward`` runs a hand-written FP64
-gradient kernel for both coordinate sets. PyTorch supplies only the device
+of the least-squares superposition of ``a`` onto ``b`` -- the quantity of
+``Superposition_rmsd`` (``include/jess/super.pxd:1-14``), computed in FP64
+from a one-sided Jacobi SVD of the correlation and the residuals of the
+optimal rotation (the search decides hits with the reference's eigenvalue
+formula instead; the two agree to rounding on well-conditioned sets) -- and
+``backward`` runs a hand-written FP64 gradient kernel for both coordinate
+sets. PyTorch supplies only the device
 tensors, the stream and the autograd plumbing; the arithmetic is in the HIP
 library (``jam_rmsd_forward`` / ``jam_rmsd_backward``).
 """
@@ -37,8 +40,20 @@ def _check_inputs(a: torch.Tensor, b: torch.Tensor, offsets: torch.Tensor) -> No
         raise ValueError(f"`a` and `b` differ in shape: {tuple(a.shape)} and {tuple(b.shape)}")
     if a.device != b.device or offsets.device != a.device:
         raise ValueError("`a`, `b` and `offsets` must be on the same device")
+    _check_offsets(offsets, a.shape[0])
+
+
+def _check_offsets(offsets: torch.Tensor, n_points: int) -> None:
+    """Every row belongs to exactly one set: the offsets start at 0, never
+    decrease and end at the number of points (one read of a flag from the
+    device), so that the kernels write every gradient row."""
     if offsets.dtype != torch.int64 or offsets.dim() != 1 or offsets.numel() < 1:
         raise ValueError("`offsets` must be a 1-D int64 tensor of n_sets + 1 entries")
+    ok = (offsets[0] == 0) & (offsets[-1] == n_points)
+    if offsets.numel() > 1:
+        ok = ok & torch.all(offsets[1:] >= offsets[:-1])
+    if not bool(ok):
+        raise ValueError("`offsets` must start at 0, never decrease and end at the number of points")
 
 
 class SuperpositionRMSD(torch.autograd.Function):
@@ -58,8 +73,8 @@ class SuperpositionRMSD(torch.autograd.Function):
         L = _native.lib()
         with torch.cuda.device(a.device):
             stream = ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream)
-            _native._check(L.jam_rmsd_forward(_ptr(a), _ptr(b), _ptr(offsets), n_sets, _ptr(rmsd), _ptr(rot),
-                                              stream))
+            _native._check(L.jam_rmsd_forward(_ptr(a), _ptr(b), _ptr(offsets), n_sets, a.shape[0], _ptr(rmsd),
+                                              _ptr(rot), stream))
         ctx.save_for_backward(a, b, offsets, rmsd, rot)
         return rmsd
 
@@ -70,13 +85,15 @@ class SuperpositionRMSD(torch.autograd.Function):
         if not (need_a or need_b):
             return None, None, None
         g = grad_rmsd.to(torch.float64).contiguous()
+        # the offsets were checked in forward: every row belongs to a set of at
+        # least one point, and the kernel writes every such row
         ga = torch.empty_like(a) if need_a else None
         gb = torch.empty_like(b) if need_b else None
         L = _native.lib()
         with torch.cuda.device(a.device):
             stream = ctypes.c_void_p(torch.cuda.current_stream(a.device).cuda_stream)
-            _native._check(L.jam_rmsd_backward(_ptr(a), _ptr(b), _ptr(offsets), offsets.numel() - 1, _ptr(rmsd),
-                                               _ptr(rot), _ptr(g), _ptr(ga), _ptr(gb), stream))
+            _native._check(L.jam_rmsd_backward(_ptr(a), _ptr(b), _ptr(offsets), offsets.numel() - 1, a.shape[0],
+                                               _ptr(rmsd), _ptr(rot), _ptr(g), _ptr(ga), _ptr(gb), stream))
         return ga, gb, None
 
 

@@ -103,7 +103,7 @@ struct jam_context {
     hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
     bool profile = false;       // join launches carry start/stop events (xev)
     // per-query workspace
-    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, kscr, kd_bucket, kd_tied, kd_count;
+    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, kscr, kd_bucket, kd_tied, kd_count, ktied, kxpos;
     DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair;
     // join: two frontiers (pair + stride entries per row), complete matches,
     // counters (rows per level, complete matches, overflow flag)
@@ -112,11 +112,13 @@ struct jam_context {
     int front_stride = 0, done_stride = 0;
     int join_grid[4] = {0, 0, 0, 0};   // resident workgroups of the join kernel <8/16/24/32>
     int kd_grid[12] = {};           // resident workgroups of the kd kernels
-    // the exact kd pass of each size tier runs on a stream of its own, from
-    // the moment the tier's fast pass has listed its tied sets, beside the
-    // fast passes of the other tiers (all streams at the default priority)
+    // each kd size tier runs on a stream of its own: its fast pass, then the
+    // exact pass of its tied sets (which only the ranking of the complete
+    // matches needs, so it runs beside the join); all streams at the default
+    // priority
     hipStream_t kd_side[KD_TIERS] = {};
-    hipEvent_t kd_fork[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
+    hipEvent_t kd_fork = nullptr, kd_fast[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
+    bool kd_pending = false;    // side-stream work of the last kd launch may still run
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
     int64_t sort_cap = 0;
@@ -139,6 +141,8 @@ struct jam_context {
     int64_t hsmall_cap = 0;
     // standalone superposition buffers
     DBuf sa, sb, srmsd, srot, sc0, sc1;
+    // standalone kd order: points, per-set offsets, zero chain/resSeq words
+    DBuf o_xyz, o_aoff, o_zero;
 };
 
 struct jam_templates {
@@ -212,28 +216,30 @@ struct jam_result {
 // ---------------------------------------------------------------------------
 
 extern "C" int32_t jam_rmsd_forward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
-                                    double *rmsd, double *rot9, void *stream)
+                                    int64_t n_points, double *rmsd, double *rot9, void *stream)
 {
-    if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets < 0 || n_points < 0) return fail(JAM_EINVAL, "invalid sizes");
     if (n_sets == 0) return JAM_OK;
-    if (!a || !b || !off || !rmsd) return fail(JAM_EINVAL, "NULL argument");
+    if (!off || !rmsd || (n_points > 0 && (!a || !b))) return fail(JAM_EINVAL, "NULL argument");
     const int tb = RG_SETS;
     hipLaunchKernelGGL(jam::k_rmsd_fwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
-                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, rot9);
+                       (hipStream_t)stream, a, b, off, (long long)n_sets, (long long)n_points, rmsd, rot9);
     HIP_TRY(hipGetLastError());
     return JAM_OK;
 }
 
 extern "C" int32_t jam_rmsd_backward(const double *a, const double *b, const int64_t *off, int64_t n_sets,
-                                     const double *rmsd, const double *rot9, const double *grad_rmsd, double *grad_a,
-                                     double *grad_b, void *stream)
+                                     int64_t n_points, const double *rmsd, const double *rot9,
+                                     const double *grad_rmsd, double *grad_a, double *grad_b, void *stream)
 {
-    if (n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets < 0 || n_points < 0) return fail(JAM_EINVAL, "invalid sizes");
     if (n_sets == 0) return JAM_OK;
-    if (!a || !b || !off || !rmsd || !grad_rmsd || (!grad_a && !grad_b)) return fail(JAM_EINVAL, "NULL argument");
+    if (!off || !rmsd || !grad_rmsd || (!grad_a && !grad_b) || (n_points > 0 && (!a || !b)))
+        return fail(JAM_EINVAL, "NULL argument");
     const int tb = RG_SETS;
     hipLaunchKernelGGL(jam::k_rmsd_bwd, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0,
-                       (hipStream_t)stream, a, b, off, (long long)n_sets, rmsd, rot9, grad_rmsd, grad_a, grad_b);
+                       (hipStream_t)stream, a, b, off, (long long)n_sets, (long long)n_points, rmsd, rot9,
+                       grad_rmsd, grad_a, grad_b);
     HIP_TRY(hipGetLastError());
     return JAM_OK;
 }
@@ -281,9 +287,10 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
     for (auto &ev : c->xev) HIP_TRY(hipEventCreate(&ev));
+    HIP_TRY(hipEventCreateWithFlags(&c->kd_fork, hipEventDisableTiming));
     for (int k = 0; k < KD_TIERS; k++) {
         HIP_TRY(hipStreamCreateWithFlags(&c->kd_side[k], hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&c->kd_fork[k], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&c->kd_fast[k], hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->kd_joined[k], hipEventDisableTiming));
     }
     *out = c.release();
@@ -313,10 +320,11 @@ extern "C" void jam_context_free(jam_context *ctx)
         if (ev) (void)hipEventDestroy(ev);
     for (int k = 0; k < KD_TIERS; k++) {
         if (ctx->kd_side[k]) (void)hipStreamSynchronize(ctx->kd_side[k]);
-        if (ctx->kd_fork[k]) (void)hipEventDestroy(ctx->kd_fork[k]);
+        if (ctx->kd_fast[k]) (void)hipEventDestroy(ctx->kd_fast[k]);
         if (ctx->kd_joined[k]) (void)hipEventDestroy(ctx->kd_joined[k]);
         if (ctx->kd_side[k]) (void)hipStreamDestroy(ctx->kd_side[k]);
     }
+    if (ctx->kd_fork) (void)hipEventDestroy(ctx->kd_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -514,6 +522,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
         HIP_TRY(c->kd_bucket.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
         HIP_TRY(c->kd_tied.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
+        HIP_TRY(c->ktied.reserve((size_t)n_sets));
         c->nsets_cap = n_sets;
     }
     const int64_t blk = cap / JAM_WAVE + n_sets + 1;
@@ -525,6 +534,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->kent.reserve(sizeof(SetEnt) * cap));
         HIP_TRY(c->katom.reserve(sizeof(int) * cap));
         HIP_TRY(c->kscr.reserve((size_t)KD_SCRATCH_BYTES_PER_ENTRY * cap));
+        HIP_TRY(c->kxpos.reserve(sizeof(int) * cap));
         c->set_cap = cap;
     }
     const int64_t blk_need = c->set_cap / JAM_WAVE + n_sets + 1;
@@ -532,8 +542,18 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         c->blk_cap = std::max(blk, blk_need);
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
     }
-    HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * 2 * KD_TIERS));
+    // per tier: sets, tied sets; then an error word for jam_kd_order
+    HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * (2 * KD_TIERS + 1)));
     return JAM_OK;
+}
+
+// Waits for the side-stream work of the last kd launch (before buffers it
+// uses are grown, or when a query stops early).
+void drain_kd(jam_context *c)
+{
+    if (!c->kd_pending) return;
+    for (int k = 0; k < KD_TIERS; k++) (void)hipStreamSynchronize(c->kd_side[k]);
+    c->kd_pending = false;
 }
 
 int32_t grow_pairs(jam_context *c, int n_pairs)
@@ -543,7 +563,9 @@ int32_t grow_pairs(jam_context *c, int n_pairs)
     HIP_TRY(c->p_lvl.reserve(sizeof(PairLevel) * (size_t)JAM_MAX_LEVELS * n_pairs));
     HIP_TRY(c->p_nchunk.reserve(sizeof(int) * (size_t)n_pairs));
     HIP_TRY(c->p_base.reserve(sizeof(long long) * (size_t)n_pairs));
-    HIP_TRY(c->p_cm.reserve(sizeof(long long) * ((size_t)n_pairs + 1)));   // [0]: event count
+    // [0]: event count, [1, n_pairs]: complete matches per pair, then the kd
+    // passes' error word
+    HIP_TRY(c->p_cm.reserve(sizeof(long long) * ((size_t)n_pairs + 2)));
     HIP_TRY(c->p_first.reserve(sizeof(long long) * (size_t)n_pairs));
     HIP_TRY(c->p_n1.reserve(sizeof(int) * (size_t)n_pairs));
     c->npairs_cap = n_pairs;
@@ -632,9 +654,11 @@ int32_t grow_events(jam_context *c, int64_t cap, int stride)
     return JAM_OK;
 }
 
-__global__ void k_zero_u32(unsigned int *v, int n)
+// clears the kd passes' per-tier counts and their error word
+__global__ void k_kd_setup(unsigned int *counts, int n, unsigned int *err)
 {
-    if ((int)threadIdx.x < n) v[threadIdx.x] = 0u;
+    if ((int)threadIdx.x < n) counts[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *err = 0u;
 }
 
 __global__ void k_items_any(DevPairs pr)
@@ -704,6 +728,111 @@ void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, c
     else hipExtLaunchKernelGGL(JOIN_KERNEL(32), g, b, 0, s, e0, e1, 0, dm, dt, st, pr, jn, qp);
 }
 
+
+// Launches the kd passes over the candidate sets of `st` on the side streams:
+// the sets are bucketed by size tier on `s`, then tier k runs on
+// c->kd_side[k] its fast pass (every set: <= 64 entries in one wave's
+// registers, up to KD_BIG with one position per thread, larger in global
+// scratch; sets with a tie at a split point are listed) and then the exact
+// QuickSelect pass over the listed sets. kd_fast[k] is recorded after the
+// fast pass, kd_joined[k] after the exact pass. Grids are upper bounds (the
+// counts stay on the device); every kernel loops over its own list.
+int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const DevSets &st, hipStream_t s)
+{
+    const int n_sets = st.n_sets;
+    unsigned char *scr = c->kscr.as<unsigned char>();
+    int *bucket = c->kd_bucket.as<int>();
+    int *tied = c->kd_tied.as<int>();
+    unsigned int *nb = c->kd_count.as<unsigned int>();
+    unsigned int *nt = nb + KD_TIERS;
+    const size_t ns = (size_t)n_sets;
+    hipLaunchKernelGGL(k_kd_setup, dim3(1), dim3(64), 0, s, nb, 2 * KD_TIERS, st.kd_err);
+    hipLaunchKernelGGL(jam::k_kdbucket, dim3(std::min((n_sets + 255) / 256, 1024)), dim3(256), 0, s, st, bucket,
+                       nb);
+    HIP_TRY(hipGetLastError());
+    if (c->kd_grid[0] == 0) {
+        const int d = c->device;
+        c->kd_grid[0] = resident_grid(jam::k_kdwave, JAM_WAVE, d);
+        c->kd_grid[1] = resident_grid(jam::k_kdblock<128>, 128, d);
+        c->kd_grid[2] = resident_grid(jam::k_kdblock<256>, 256, d);
+        c->kd_grid[3] = resident_grid(jam::k_kdblock<512>, 512, d);
+        c->kd_grid[4] = resident_grid(jam::k_kdblock<1024>, 1024, d);
+        c->kd_grid[5] = resident_grid(jam::k_kdbig, KD_THREADS, d);
+        c->kd_grid[6] = resident_grid(jam::k_kdexact<64>, 64, d);
+        c->kd_grid[7] = resident_grid(jam::k_kdexact<128>, 128, d);
+        c->kd_grid[8] = resident_grid(jam::k_kdexact<256>, 256, d);
+        c->kd_grid[9] = resident_grid(jam::k_kdexact<512>, 512, d);
+        c->kd_grid[10] = resident_grid(jam::k_kdexact<1024>, 1024, d);
+        c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_EXACT_THREADS, d);
+    }
+    auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
+    auto launch_fast = [&](int k, hipStream_t x) {
+        const size_t o = (size_t)k * ns;
+        switch (k) {
+        case 0:
+            hipLaunchKernelGGL(jam::k_kdwave, grid(0), dim3(JAM_WAVE), 0, x, dm, dt, st, bucket, nb, tied, nt);
+            break;
+        case 1:
+            hipLaunchKernelGGL((jam::k_kdblock<128>), grid(1), dim3(128), 0, x, dm, dt, st, bucket + o, nb + 1,
+                               tied + o, nt + 1);
+            break;
+        case 2:
+            hipLaunchKernelGGL((jam::k_kdblock<256>), grid(2), dim3(256), 0, x, dm, dt, st, bucket + o, nb + 2,
+                               tied + o, nt + 2);
+            break;
+        case 3:
+            hipLaunchKernelGGL((jam::k_kdblock<512>), grid(3), dim3(512), 0, x, dm, dt, st, bucket + o, nb + 3,
+                               tied + o, nt + 3);
+            break;
+        case 4:
+            hipLaunchKernelGGL((jam::k_kdblock<1024>), grid(4), dim3(1024), 0, x, dm, dt, st, bucket + o, nb + 4,
+                               tied + o, nt + 4);
+            break;
+        default:
+            hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, x, dm, dt, st, scr, bucket + o, nb + 5,
+                               tied + o, nt + 5);
+            break;
+        }
+    };
+    auto launch_exact = [&](int k, hipStream_t x) {
+        const int *lst = tied + (size_t)k * ns;
+        const unsigned int *lc = nt + k;
+        switch (k) {
+        case 0:
+            hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, lst, lc);
+            break;
+        case 1:
+            hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, x, dm, dt, st, lst, lc);
+            break;
+        case 2:
+            hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, x, dm, dt, st, lst, lc);
+            break;
+        case 3:
+            hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, x, dm, dt, st, lst, lc);
+            break;
+        case 4:
+            hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, lst, lc);
+            break;
+        default:
+            hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr, lst,
+                               lc);
+            break;
+        }
+    };
+    HIP_TRY(hipEventRecord(c->kd_fork, s));
+    c->kd_pending = true;
+    // largest tier first: its passes are the longest
+    for (int k = KD_TIERS - 1; k >= 0; k--) {
+        hipStream_t x = c->kd_side[k];
+        HIP_TRY(hipStreamWaitEvent(x, c->kd_fork, 0));
+        launch_fast(k, x);
+        HIP_TRY(hipEventRecord(c->kd_fast[k], x));
+        launch_exact(k, x);
+        HIP_TRY(hipEventRecord(c->kd_joined[k], x));
+    }
+    HIP_TRY(hipGetLastError());
+    return JAM_OK;
+}
 }  // namespace
 
 // One device array of counters, read back in one copy after the join: rows
@@ -724,6 +853,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     if (p->ignore_chain < 0 || p->ignore_chain > 2) return fail(JAM_EINVAL, "invalid ignore_chain");
     *out = nullptr;
     HIP_TRY(hipSetDevice(c->device));
+    drain_kd(c);
     std::unique_ptr<jam_result> R(new jam_result());
     R->ctx = c;
     R->gen = ++c->query_gen;
@@ -755,7 +885,7 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     if ((rc = grow_events(c, std::max<int64_t>(c->event_cap, 1 << 12), stride)) != JAM_OK) return rc;
     if ((rc = grow_pairs(c, n_pairs)) != JAM_OK) return rc;
     if ((rc = grow_items(c, std::max<int64_t>(c->items_cap, 4096))) != JAM_OK) return rc;
-    if ((rc = grow_hsmall(c, std::max<int64_t>(CNT_WORDS, (int64_t)n_pairs + 1))) != JAM_OK) return rc;
+    if ((rc = grow_hsmall(c, std::max<int64_t>(CNT_WORDS, (int64_t)n_pairs + 2))) != JAM_OK) return rc;
     HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * CNT_WORDS));
     DevMols dm = ms->dev();
     DevTpls dt = tp->dev();
@@ -784,6 +914,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         unsigned long long *cnts = c->j_count.as<unsigned long long>();
         st.total = (int64_t *)(cnts + CNT_TOTAL);
         st.maxcnt = (int *)(cnts + CNT_MAXCNT);
+        st.tied = c->ktied.as<unsigned char>();
+        st.xpos = c->kxpos.as<int>();
+        // the kd error word sits after the per-pair counts, which are read
+        // back at the end of the query (the exact passes run until then)
+        unsigned long long *pcm = (unsigned long long *)c->p_cm.ptr;
+        st.kd_err = (unsigned int *)(pcm + (size_t)n_pairs + 1);
         DevPairs pr;
         pr.n_pairs = n_pairs;
         pr.n_items = (const long long *)(cnts + CNT_ITEMS);
@@ -815,130 +951,19 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                            (long long *)(cnts + CNT_ITEMS));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
-        {
-            // sets bucketed by size; fast path for every set: <= 64 entries
-            // in one wave's registers, up to KD_BIG with one position per
-            // thread, larger in global scratch; sets with a tie across a split
-            // point are queued per tier for the exact QuickSelect path. Grids
-            // are upper bounds (the counts stay on the device); every kernel
-            // loops over its own list.
-            unsigned char *scr = c->kscr.as<unsigned char>();
-            int *bucket = c->kd_bucket.as<int>();
-            int *tied = c->kd_tied.as<int>();
-            unsigned int *nb = c->kd_count.as<unsigned int>();
-            unsigned int *nt = nb + KD_TIERS;
-            const size_t ns = (size_t)n_sets;
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, nb, 2 * KD_TIERS);
-            hipLaunchKernelGGL(jam::k_kdbucket, dim3(std::min((n_sets + 255) / 256, 1024)), dim3(256), 0, s, st,
-                               bucket, nb);
-            if (c->kd_grid[0] == 0) {
-                const int d = c->device;
-                c->kd_grid[0] = resident_grid(jam::k_kdwave, JAM_WAVE, d);
-                c->kd_grid[1] = resident_grid(jam::k_kdblock<128>, 128, d);
-                c->kd_grid[2] = resident_grid(jam::k_kdblock<256>, 256, d);
-                c->kd_grid[3] = resident_grid(jam::k_kdblock<512>, 512, d);
-                c->kd_grid[4] = resident_grid(jam::k_kdblock<1024>, 1024, d);
-                c->kd_grid[5] = resident_grid(jam::k_kdbig, KD_THREADS, d);
-                c->kd_grid[6] = resident_grid(jam::k_kdexact<64>, 64, d);
-                c->kd_grid[7] = resident_grid(jam::k_kdexact<128>, 128, d);
-                c->kd_grid[8] = resident_grid(jam::k_kdexact<256>, 256, d);
-                c->kd_grid[9] = resident_grid(jam::k_kdexact<512>, 512, d);
-                c->kd_grid[10] = resident_grid(jam::k_kdexact<1024>, 1024, d);
-                c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_EXACT_THREADS, d);
-            }
-            auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
-            auto launch_fast = [&](int k) {
-                const size_t o = (size_t)k * ns;
-                switch (k) {
-                case 0:
-                    hipLaunchKernelGGL(jam::k_kdwave, grid(0), dim3(JAM_WAVE), 0, s, dm, dt, st, bucket, nb, tied,
-                                       nt);
-                    break;
-                case 1:
-                    hipLaunchKernelGGL((jam::k_kdblock<128>), grid(1), dim3(128), 0, s, dm, dt, st, bucket + o,
-                                       nb + 1, tied + o, nt + 1);
-                    break;
-                case 2:
-                    hipLaunchKernelGGL((jam::k_kdblock<256>), grid(2), dim3(256), 0, s, dm, dt, st, bucket + o,
-                                       nb + 2, tied + o, nt + 2);
-                    break;
-                case 3:
-                    hipLaunchKernelGGL((jam::k_kdblock<512>), grid(3), dim3(512), 0, s, dm, dt, st, bucket + o,
-                                       nb + 3, tied + o, nt + 3);
-                    break;
-                case 4:
-                    hipLaunchKernelGGL((jam::k_kdblock<1024>), grid(4), dim3(1024), 0, s, dm, dt, st, bucket + o,
-                                       nb + 4, tied + o, nt + 4);
-                    break;
-                default:
-                    hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, s, dm, dt, st, scr, bucket + o,
-                                       nb + 5, tied + o, nt + 5);
-                    break;
-                }
-            };
-            // the exact pass of tier k over the sets its fast pass found tied,
-            // or (`all`) over every set of the tier
-            auto launch_exact = [&](int k, hipStream_t x, bool all) {
-                const size_t o = (size_t)k * ns;
-                const int *lst = all ? bucket + o : tied + o;
-                const unsigned int *lc = all ? nb + k : nt + k;
-                switch (k) {
-                case 0:
-                    hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, lst, lc);
-                    break;
-                case 1:
-                    hipLaunchKernelGGL((jam::k_kdexact<128>), grid(7), dim3(128), 0, x, dm, dt, st, lst, lc);
-                    break;
-                case 2:
-                    hipLaunchKernelGGL((jam::k_kdexact<256>), grid(8), dim3(256), 0, x, dm, dt, st, lst, lc);
-                    break;
-                case 3:
-                    hipLaunchKernelGGL((jam::k_kdexact<512>), grid(9), dim3(512), 0, x, dm, dt, st, lst, lc);
-                    break;
-                case 4:
-                    hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, lst, lc);
-                    break;
-                default:
-                    hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr,
-                                       lst, lc);
-                    break;
-                }
-            };
-            // One molecule has few sets in the 512 and 1024 tiers, and their
-            // exact pass (hundreds of microseconds per set, all sets side by
-            // side) is the kd phase's longest chain: it runs on every set of
-            // those tiers at once, without waiting for a fast pass to list the
-            // tied ones. The exact pass gives the reference's order whether or
-            // not a set has ties.
-            const bool direct = (M == 1);
-            // largest tier first: its exact pass, the longest, starts earliest
-            // (with one molecule the direct tiers first: sets above 1,024
-            // entries are rare there)
-            static const int by_size[KD_TIERS] = {5, 4, 3, 2, 1, 0};
-            static const int one_mol[KD_TIERS] = {4, 3, 5, 2, 1, 0};
-            static_assert(KD_TIERS == 6, "tier orders list six tiers");
-            for (int ki = 0; ki < KD_TIERS; ki++) {
-                const int k = direct ? one_mol[ki] : by_size[ki];
-                const bool all = direct && (k == 3 || k == 4);
-                if (all) {
-                    HIP_TRY(hipEventRecord(c->kd_fork[k], s));
-                    HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
-                    launch_exact(k, c->kd_side[k], true);
-                } else {
-                    launch_fast(k);
-                    HIP_TRY(hipEventRecord(c->kd_fork[k], s));
-                    HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fork[k], 0));
-                    launch_exact(k, c->kd_side[k], false);
-                }
-                HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
-            }
-            // the join's counters and level-0 flags do not depend on the kd
-            // order: set them up while the exact passes still run
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnts, (long long)CNT_TOTAL);
-            hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
-            for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
-            HIP_TRY(hipGetLastError());
+        // the kd passes: each tier's fast pass, then the exact pass of its
+        // tied sets, on the tier's own stream; the join needs only the fast
+        // passes, the ranking after it the exact ones
+        if ((rc = launch_kd(c, dm, dt, st, s)) != JAM_OK) {
+            drain_kd(c);
+            return rc;
         }
+        // the join's counters and level-0 flags do not depend on the kd
+        // order: set them up meanwhile
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnts, (long long)CNT_TOTAL);
+        hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
+        for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_fast[k], 0));
+        HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[2], s));
 
         // ---- the join: one launch per level, back to back; the counters
@@ -994,22 +1019,20 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         int maxcnt = 0;
         std::memcpy(&maxcnt, jc + CNT_MAXCNT, sizeof(int));
         if (total >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 candidate-set entries in one query");
-        bool regrow = false;
-        if (total > c->set_cap) {
-            if ((rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
-            regrow = true;
-        }
-        if (n0 > c->items_cap) {
-            if ((rc = grow_items(c, n0)) != JAM_OK) return rc;
-            regrow = true;
-        }
-        if (n0 > c->front_cap) {
-            if ((rc = grow_front(c, std::max<int64_t>(2 * c->front_cap, n0), stride)) != JAM_OK) return rc;
-            regrow = true;
-        }
-        if (regrow) continue;
         const unsigned int ovf = (unsigned int)(jc[CNT_OVF] & 0xFFFFFFFFull);
         const int64_t n_done = (int64_t)jc[CNT_DONE];
+        const bool regrow = total > c->set_cap || n0 > c->items_cap || n0 > c->front_cap;
+        if (regrow || ovf || n_done > c->done_cap) {
+            // the exact kd passes may still read the buffers about to grow
+            drain_kd(c);
+        }
+        if (regrow) {
+            if (total > c->set_cap && (rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
+            if (n0 > c->items_cap && (rc = grow_items(c, n0)) != JAM_OK) return rc;
+            if (n0 > c->front_cap && (rc = grow_front(c, std::max<int64_t>(2 * c->front_cap, n0), stride)) != JAM_OK)
+                return rc;
+            continue;
+        }
         unsigned long long rows[JAM_MAX_LEVELS];   // the read-back area is reused below
         std::memcpy(rows, jc, sizeof(rows));
         if (ovf || n_done > c->done_cap) {
@@ -1023,6 +1046,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             continue;
         }
         HIP_TRY(hipEventRecord(c->ev[3], s));
+        // the ranking needs every set's position in the reference's kd order
+        for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
 
         // ---- reference order of the complete matches: the order of the key
         // (pair, level 0 position, level 1 position, ...), by rank counting
@@ -1096,9 +1121,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         ev.cap = c->event_cap;
         ev.stride = stride;
         ev.base = c->e_all.as<unsigned char>();
-        // the event count, then the complete matches of every pair: read back
-        // in one copy
-        unsigned long long *pcm = (unsigned long long *)c->p_cm.ptr;
+        // the event count, then the complete matches of every pair and the kd
+        // error word: read back in one copy
         ev.count = pcm;
         // marks and their scan reuse the (now free) 64-bit sort keys
         int *sel = c->s_key[0].as<int>();
@@ -1136,9 +1160,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                                (const long long *)c->p_first.as<long long>(), (long long *)(pcm + 1));
             HIP_TRY(hipGetLastError());
         }
-        HIP_TRY(hipMemcpyAsync(c->hsmall, pcm, sizeof(unsigned long long) * ((size_t)n_pairs + 1),
+        HIP_TRY(hipMemcpyAsync(c->hsmall, pcm, sizeof(unsigned long long) * ((size_t)n_pairs + 2),
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        c->kd_pending = false;   // s waited for every side stream
+        if (c->hsmall[(size_t)n_pairs + 1] != 0ull)
+            return fail(JAM_EDEVICE, "a kd pass reached its round bound (broken QuickSelect invariant)");
         const unsigned long long ne_dev = c->hsmall[0];
         std::memcpy(R->pair_cm.data(), c->hsmall + 1, sizeof(long long) * (size_t)n_pairs);
         if ((int64_t)ne_dev > c->event_cap || (int64_t)ne_dev > n_done)
@@ -1353,5 +1380,100 @@ extern "C" int32_t jam_superpose_batch(jam_context *c, const double *a, const do
     HIP_TRY(hipMemcpyAsync(c0, c->sc0.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(c1, c->sc1.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// standalone kd-tree leaf order
+// ---------------------------------------------------------------------------
+
+extern "C" int32_t jam_kd_order(jam_context *c, const double *xyz, const int64_t *off, int32_t n_sets,
+                                int32_t *order)
+{
+    if (!c || !off || (n_sets > 0 && (!xyz || !order))) return fail(JAM_EINVAL, "NULL argument");
+    if (n_sets < 0) return fail(JAM_EINVAL, "negative set count");
+    if (off[0] != 0) return fail(JAM_EINVAL, "off[0] must be 0");
+    for (int32_t k = 0; k < n_sets; k++)
+        if (off[k + 1] < off[k]) return fail(JAM_EINVAL, "set offsets decrease");
+    const int64_t total = off[n_sets];
+    if (total >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 points");
+    if (n_sets == 0 || total == 0) return JAM_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    drain_kd(c);
+    hipStream_t s = c->stream;
+    // every set is a molecule of its own whose candidate list is its points
+    // in input order; chains and residue numbers are zero
+    std::vector<int> aoff(n_sets + 1), cnt(n_sets), list((size_t)total);
+    for (int32_t k = 0; k <= n_sets; k++) aoff[k] = (int)off[k];
+    for (int32_t k = 0; k < n_sets; k++) {
+        cnt[k] = (int)(off[k + 1] - off[k]);
+        for (int i = 0; i < cnt[k]; i++) list[(size_t)off[k] + i] = i;
+    }
+    int32_t rc;
+    if ((rc = grow_sets(c, std::max<int64_t>(total, c->set_cap), n_sets)) != JAM_OK) return rc;
+    HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * CNT_WORDS));
+    HIP_TRY(c->o_xyz.reserve(sizeof(double) * 3 * (size_t)total));
+    HIP_TRY(c->o_zero.reserve(sizeof(int) * (size_t)total));
+    HIP_TRY(upload(c->o_aoff, aoff, s));
+    HIP_TRY(upload(c->cnt, cnt, s));
+    HIP_TRY(upload(c->list, list, s));
+    HIP_TRY(hipMemcpyAsync(c->off.ptr, off, sizeof(int64_t) * (size_t)n_sets, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->o_xyz.ptr, xyz, sizeof(double) * 3 * (size_t)total, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(c->o_zero.ptr, 0, sizeof(int) * (size_t)total, s));
+    unsigned long long *cnts = c->j_count.as<unsigned long long>();
+    const unsigned long long tot = (unsigned long long)total;
+    HIP_TRY(hipMemcpyAsync(cnts + CNT_TOTAL, &tot, sizeof(tot), hipMemcpyHostToDevice, s));
+    DevMols dm{};
+    dm.n_mols = n_sets;
+    dm.atom_off = c->o_aoff.as<int>();
+    dm.xyz = c->o_xyz.as<double>();
+    dm.chain = c->o_zero.as<int>();
+    dm.resseq = c->o_zero.as<int>();
+    DevTpls dt{};
+    dt.n_types = 1;
+    DevSets st{};
+    st.n_sets = n_sets;
+    st.cap = c->set_cap;
+    st.blk_cap = c->blk_cap;
+    st.cnt = c->cnt.as<int>();
+    st.off = c->off.as<int64_t>();
+    st.list = c->list.as<int>();
+    st.kidx = c->kidx.as<int>();
+    st.l2k = c->l2k.as<int>();
+    st.kent = c->kent.as<SetEnt>();
+    st.katom = c->katom.as<int>();
+    st.bbox = c->bbox.as<double>();
+    st.total = (int64_t *)(cnts + CNT_TOTAL);
+    st.maxcnt = (int *)(cnts + CNT_MAXCNT);
+    st.tied = c->ktied.as<unsigned char>();
+    st.xpos = c->kxpos.as<int>();
+    st.kd_err = c->kd_count.as<unsigned int>() + 2 * KD_TIERS;
+    if ((rc = launch_kd(c, dm, dt, st, s)) != JAM_OK) {
+        drain_kd(c);
+        return rc;
+    }
+    for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
+    std::vector<int> kidx((size_t)total), xpos((size_t)total);
+    std::vector<unsigned char> tied((size_t)n_sets);
+    unsigned int err = 0;
+    HIP_TRY(hipMemcpyAsync(kidx.data(), st.kidx, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(xpos.data(), st.xpos, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tied.data(), st.tied, (size_t)n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&err, st.kd_err, sizeof(err), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->kd_pending = false;
+    if (err) return fail(JAM_EDEVICE, "a kd pass reached its round bound (broken QuickSelect invariant)");
+    // DFS position p (right child first) is leaf n-1-p; kd position q holds
+    // list element kidx[q], which sits at DFS position xpos[q] of the
+    // reference's order when the set is tied, else at q
+    for (int32_t k = 0; k < n_sets; k++) {
+        const int64_t o = off[k];
+        const int n = cnt[k];
+        for (int q = 0; q < n; q++) {
+            const int p = tied[k] ? xpos[(size_t)o + q] : q;
+            if (p < 0 || p >= n) return fail(JAM_EDEVICE, "kd positions out of range");
+            order[(size_t)o + (n - 1 - p)] = kidx[(size_t)o + q];
+        }
+    }
     return JAM_OK;
 }

@@ -15,7 +15,8 @@
 //   sets        reference order (molecule-local atom index), and the set in
 //               kd-tree DFS leaf order (a 32-byte entry: x, y, z, chain,
 //               resSeq; and the atom index) with one bounding box per 64
-//               consecutive entries.
+//               consecutive entries; for sets with ties at a split point,
+//               each position's place in the reference's own order.
 //   join        frontier rows (pair, one entry index per placed level),
 //               double buffered, and the complete matches.
 #pragma once
@@ -79,6 +80,14 @@ struct DevSets {
     double *bbox;             // 6 per block: lo xyz, hi xyz
     int64_t *total;           // device scalar: sum of cnt
     int *maxcnt;              // device scalar: largest cnt
+    // Sets with equal coordinates on both sides of a split point: the kd
+    // arrays above then hold a valid spatial order (the join's pruning needs
+    // nothing more), and xpos maps each of its positions to the position of
+    // the same entry in the reference's own order (QuickSelect's swap
+    // sequence), which ranks the complete matches.
+    unsigned char *tied;      // per set: 1 when xpos applies
+    int *xpos;                // per entry: set-local position in the reference's DFS order
+    unsigned int *kd_err;     // device word, 0 unless a kd pass hit its round bound
 };
 
 #define JAM_MAX_LEVELS 32

@@ -261,16 +261,33 @@ __device__ __attribute__((always_inline)) static inline void rmsd_grad_one(int n
 #define RG_SETS 64
 #define RG_CAP 768   // points per tile held in LDS (12 per set on average)
 
+// Range of set s, clamped into [lo, hi]: with valid offsets (off[0] = 0,
+// never decreasing, off[n_sets] = n_points) this is [off[s], off[s+1]);
+// malformed offsets give meaningless values, never an access outside a and b.
+__device__ static inline void set_range(const int64_t *off, long long s, int64_t lo, int64_t hi, int64_t &p0,
+                                        int64_t &p1)
+{
+    int64_t x = off[s], y = off[s + 1];
+    x = x < lo ? lo : (x > hi ? hi : x);
+    y = y < x ? x : (y > hi ? hi : y);
+    p0 = x;
+    p1 = y;
+}
+
 // Set s owns points [off[s], off[s+1]) of a and b, 3 doubles per point.
 __global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const double *b, const int64_t *off,
-                                                      long long n_sets, double *rmsd, double *rot)
+                                                      long long n_sets, long long n_points, double *rmsd,
+                                                      double *rot)
 {
     __shared__ double la[3 * RG_CAP], lb[3 * RG_CAP];
     const int lane = threadIdx.x;
     const long long s0 = (long long)blockIdx.x * RG_SETS;
     if (s0 >= n_sets) return;
     const long long s1 = s0 + RG_SETS < n_sets ? s0 + RG_SETS : n_sets;
-    const int64_t P0 = off[s0], P1 = off[s1];
+    // the tile's points: from its first set's start to its last set's end
+    int64_t P0, P1, q0, q1;
+    set_range(off, s0, 0, n_points, P0, q0);
+    set_range(off, s1 - 1, P0, n_points, q1, P1);
     const bool staged = P1 - P0 <= RG_CAP;
     if (staged) {
         const int m = (int)(3 * (P1 - P0));
@@ -282,8 +299,9 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const dou
     }
     const long long s = s0 + lane;
     if (s >= s1) return;
-    const int64_t p0 = off[s];
-    const int n = (int)(off[s + 1] - p0);
+    int64_t p0, p1;
+    set_range(off, s, P0, P1, p0, p1);
+    const int n = (int)(p1 - p0);
     double *rt = rot ? rot + 9 * s : nullptr;
     if (n <= 0) {
         rmsd[s] = 0.0;
@@ -298,15 +316,18 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const dou
 }
 
 __global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const double *b, const int64_t *off,
-                                                      long long n_sets, const double *rmsd, const double *rot,
-                                                      const double *grad_rmsd, double *grad_a, double *grad_b)
+                                                      long long n_sets, long long n_points, const double *rmsd,
+                                                      const double *rot, const double *grad_rmsd, double *grad_a,
+                                                      double *grad_b)
 {
     __shared__ double la[3 * RG_CAP], lb[3 * RG_CAP];
     const int lane = threadIdx.x;
     const long long s0 = (long long)blockIdx.x * RG_SETS;
     if (s0 >= n_sets) return;
     const long long s1 = s0 + RG_SETS < n_sets ? s0 + RG_SETS : n_sets;
-    const int64_t P0 = off[s0], P1 = off[s1];
+    int64_t P0, P1, q0, q1;
+    set_range(off, s0, 0, n_points, P0, q0);
+    set_range(off, s1 - 1, P0, n_points, q1, P1);
     const bool staged = P1 - P0 <= RG_CAP;
     const int m = (int)(3 * (P1 - P0));
     if (staged) {
@@ -318,8 +339,9 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const dou
     }
     const long long s = s0 + lane;
     if (s < s1) {
-        const int64_t p0 = off[s];
-        const int n = (int)(off[s + 1] - p0);
+        int64_t p0, p1;
+        set_range(off, s, P0, P1, p0, p1);
+        const int n = (int)(p1 - p0);
         if (n > 0) {
             const double *rt = rot ? rot + 9 * s : nullptr;
             if (staged) {

@@ -13,14 +13,16 @@
 // the atom index) with one bounding box per 64 consecutive entries, which the
 // join uses for pruning.
 //
-// Two algorithms, both exact:
-//   fast   when no range of the tree has equal coordinates on both sides of
-//          its split point, the left half of every range is its n/2 smallest
-//          elements whatever the arrangement, and the leaf order follows from
-//          memberships alone: sort every axis once, then at level d cut each
-//          range of axis d%3 in half and stably partition the two other axes
-//          by side. Detects a tie across a split point and then gives up.
-//   exact  for the sets the fast path gave up on: all ranges of a tree level
+// Two algorithms:
+//   fast   the left half of every range is taken as its n/2 smallest
+//          elements by (coordinate, element): sort every axis once, then at
+//          level d cut each range of axis d%3 in half and stably partition
+//          the two other axes by side. This is the reference's order whenever
+//          no range has equal coordinates on both sides of its split point;
+//          otherwise it is still a valid spatial order (every left half lies
+//          below its right half on the split axis), which is all the join's
+//          pruning needs, and the set is flagged tied.
+//   exact  for the tied sets, beside the join: all ranges of a tree level
 //          run their QuickSelect together, one partition round at a time,
 //          and each Lomuto partition is computed without its serial scan:
 //            * the "<= pivot" elements keep their order: a stable compaction;
@@ -31,10 +33,16 @@
 //              every push to its element. The final queue fills
 //              [l+k+1, r-1] from its second element on, its front goes to r,
 //              the pivot to l+k.
+//          It writes, for every position of the fast order, the position of
+//          the same entry in the reference's order (xpos), which ranks the
+//          complete matches before they are reported.
 //
 // Sets are bucketed by size; sets of up to 64 entries run in one wave's
 // registers, up to KD_BIG entries with one position per thread (shuffles,
-// ballots and LDS exchanges), larger ones in global scratch.
+// ballots and LDS exchanges), larger ones in global scratch. Every loop of the
+// exact pass whose length depends on the data has a bound that a correct
+// chain never reaches; a pass that reaches it sets st.kd_err and leaves the
+// set's positions unwritten, and the query reports an error.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -152,7 +160,8 @@ __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st,
 // everything in registers
 // ---------------------------------------------------------------------------
 
-// Returns false (nothing written) when a tie straddles a split point.
+// Returns true when a tie straddles a split point (the order written is then
+// the fast one, not the reference's).
 __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, int n, int64_t off, int abase)
 {
     const int lane = lane_id();
@@ -186,6 +195,7 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
         perm[a] = id;
     }
     int sb = 0, se = n;
+    bool tie = false;
     for (int d = 0;; d++) {
         const int a = d % 3;
         const bool act = lane < n && se - sb >= 2;
@@ -197,7 +207,7 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
         const double ca = (a == 0) ? c[0] : (a == 1) ? c[1] : c[2];
         const double here = __shfl(ca, pa & 63, JAM_WAVE);
         const double prev = __shfl_up(here, 1, JAM_WAVE);
-        if (ballot(split && prev == here) != 0ull) return false;
+        if (ballot(split && prev == here) != 0ull) tie = true;
         // side of every element, delivered to the lane of the element id
         const int left_pos = act && lane < m ? 1 : 0;
         const int side_by_elem = push_int(left_pos, pa & 63);
@@ -226,7 +236,7 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
     const int src = n - 1 - lane;
     const int e = shfl_int(perm[0], src < 0 ? 0 : src);
     kd_emit_lane(ms, st, s, n, off, abase, lane, e);
-    return true;
+    return tie;
 }
 
 __global__ void __launch_bounds__(JAM_WAVE) k_kdwave(DevMols ms, DevTpls tp, DevSets st, const int *list,
@@ -239,7 +249,11 @@ __global__ void __launch_bounds__(JAM_WAVE) k_kdwave(DevMols ms, DevTpls tp, Dev
         const int n = st.cnt[s];
         const int64_t off = st.off[s];
         const int abase = ms.atom_off[s / tp.n_types];
-        if (!kd_wave64(ms, st, s, n, off, abase) && lane_id() == 0) tied[atomicAdd(n_tied, 1u)] = s;
+        const bool tie = kd_wave64(ms, st, s, n, off, abase);
+        if (lane_id() == 0) {
+            st.tied[s] = tie ? 1 : 0;
+            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
+        }
     }
 }
 
@@ -327,10 +341,7 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
             side[ia] = (act && t < m) ? 1 : 0;
             __syncthreads();
             const bool my_tie = act && t == m && xkey[t - 1] == ka;
-            if (__syncthreads_or(my_tie)) {
-                tie = true;
-                break;
-            }
+            if (__syncthreads_or(my_tie)) tie = true;
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 if (b == a) continue;
@@ -360,10 +371,9 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
                 else sb = m;
             }
         }
-        if (tie) {
-            if (t == 0) tied[atomicAdd(n_tied, 1u)] = s;
-            __syncthreads();
-            continue;
+        if (t == 0) {
+            st.tied[s] = tie ? 1 : 0;
+            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
         }
         // DFS order: position p holds element arrangement[n-1-p]
         xid[t] = id[0];
@@ -394,11 +404,37 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
 #define KD_CHASE 8
 #endif
 
+// ceil(log2(n)) for n >= 1
+__host__ __device__ constexpr int kd_log2_ceil(int n) { return n <= 1 ? 0 : 1 + kd_log2_ceil((n + 1) / 2); }
+
+// Bounds of the exact pass's data-dependent loops. A window of w elements
+// needs at most w partition rounds, a tree of n elements fewer than
+// n * (ceil(log2 n) + 1) in all; a pop chain has at most n links, so pointer
+// jumping (each round halves every open chain) ends within ceil(log2 n) + 1
+// rounds. Reaching a bound means a broken invariant: the pass stops and
+// reports it instead of looping.
+#define KD_JUMP_BOUND(n) (kd_log2_ceil(n) + 2)
+#define KD_ROUND_BOUND(n) ((long long)(n) * (kd_log2_ceil(n) + 2) + 64)
+
+// Position of every entry of set s in the reference's DFS order: the exact
+// arrangement `arr_at(i)` (element at leaf i, left subtree first) visited
+// right child first, so DFS position p holds element arr_at(n-1-p). Written
+// through the fast order's list -> position map, at the fast position.
+template <class F>
+__device__ static inline void kd_write_xpos(const DevSets &st, int n, int64_t off, int p, F arr_at)
+{
+    if (p < n) {
+        const int e = arr_at(n - 1 - p);
+        st.xpos[off + st.l2k[off + e]] = p;
+    }
+}
+
 template <int NT>
 __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets st, const int *list,
                                                 const unsigned int *count)
 {
     constexpr int NW = NT / JAM_WAVE;
+    constexpr int JUMPS = KD_JUMP_BOUND(NT);
     __shared__ double cc[3][NT];  // coordinates by element
     __shared__ int xa[NT];        // arrangement: element at each position
     __shared__ int xp1[NT];       // exclusive prefix counts of the "<=" flags
@@ -428,7 +464,14 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
         int R = 0, H = n;                   // range of position t
         int lw = 0, rw = n >= 2 ? n - 1 : -1;  // its window; rw < 0: none
         int hmax = n;                       // largest range at this depth
+        bool broken = false;                // a loop bound was reached (block-uniform)
+        const long long round_bound = KD_ROUND_BOUND(n);
+        long long rounds = 0;
         for (int d = 0; hmax >= 2;) {
+            if (++rounds > round_bound) {
+                broken = true;
+                break;
+            }
             const double *c = cc[d % 3];
             const bool active = live && rw >= 0;
             double pv = 0.0;
@@ -524,8 +567,12 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                 }
             }
             // pointer jumping until every push names an element; each pop
-            // names a lower push, so the chains end
-            while (__syncthreads_or(again)) {
+            // names a lower push, so the chains end within JUMPS rounds
+            for (int jumps = 0; __syncthreads_or(again); jumps++) {
+                if (jumps >= JUMPS) {
+                    broken = true;
+                    break;
+                }
                 again = false;
                 if (inwin && t - lw < pushes) {
                     const int v = src[t];
@@ -536,6 +583,7 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                     }
                 }
             }
+            if (broken) break;
             // scatter: "<=" elements in order, the pivot after them, the
             // queue from its second element on, its front at the right end
             if (active) {
@@ -562,9 +610,12 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
                 rw = nr;
             }
         }
-        // DFS order: position p holds element arrangement[n-1-p]
-        const int e = live ? xa[n - 1 - t] : 0;
-        kd_emit_lane(ms, st, s, n, off, abase, t, e);
+        __syncthreads();
+        if (broken) {
+            if (t == 0) atomicOr(st.kd_err, 1u);
+        } else {
+            kd_write_xpos(st, n, off, t, [&](int i) { return xa[i]; });
+        }
         __syncthreads();
     }
 }
@@ -663,7 +714,7 @@ __device__ static void kdf_sort(const double *c, int *perm, int n, int n2)
     }
 }
 
-// fast path; returns false on a split tie
+// fast path; returns true when a tie straddles a split point
 template <int NT>
 __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, int *flag)
 {
@@ -677,6 +728,7 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
         w.se[p] = n;
     }
     int cur[3] = {0, 0, 0};
+    bool any_tie = false;
     __syncthreads();
     for (int d = 0;; d++) {
         const int a = d % 3;
@@ -700,7 +752,7 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
         __syncthreads();
         const int splits = flag[0], tie = flag[1];
         __syncthreads();
-        if (tie) return false;
+        if (tie) any_tie = true;
         if (!splits) break;
         for (int b = 0; b < 3; b++) {
             if (b == a) continue;
@@ -734,7 +786,7 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
     const int *fin = w.perm + cur[0] * S;
     for (int p = tid; p < n; p += NT) out[p] = fin[p];
     __syncthreads();
-    return true;
+    return any_tie;
 }
 
 struct KdWork {
@@ -753,11 +805,15 @@ __device__ static inline int kd_cmp3(const double *c, int x, int y)
     return u < v ? -1 : (u > v ? 1 : 0);
 }
 
-// exact path: arranges elements 0..n-1 into the reference's leaf order
+// exact path: arranges elements 0..n-1 into the reference's leaf order;
+// false when a loop bound was reached (out is then not written)
 template <int NT>
-__device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
+__device__ static bool kd_exact(const KdWork &w, int n, int *out, int *partial, int *flag)
 {
     const int tid = threadIdx.x;
+    const int jump_bound = KD_JUMP_BOUND(n);
+    const long long round_bound = KD_ROUND_BOUND(n);
+    long long rounds = 0;
     int cur = 0;
     for (int p = tid; p < n; p += NT) {
         w.arr[p] = p;
@@ -783,6 +839,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
         __syncthreads();
         if (!flag[0]) break;
         for (;;) {
+            if (++rounds > round_bound) return false;
             int *a = w.arr + cur * w.stride;
             int *an = w.arr + (cur ^ 1) * w.stride;
             // pivot: median of three, moved to the right end of the window
@@ -835,7 +892,8 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
             }
             __syncthreads();
             // pointer jumping until every push names an element
-            for (;;) {
+            for (int jumps = 0;; jumps++) {
+                if (jumps > jump_bound) return false;
                 if (tid == 0) flag[1] = 0;
                 __syncthreads();
                 for (int i = tid; i < n; i += NT) {
@@ -914,6 +972,7 @@ __device__ static void kd_exact(const KdWork &w, int n, int *out, int *partial, 
     const int *fin = w.arr + cur * w.stride;
     for (int p = tid; p < n; p += NT) out[p] = fin[p];
     __syncthreads();
+    return true;
 }
 
 __device__ static void kd_load_global(const DevMols &ms, const DevSets &st, int n, int64_t off, int abase,
@@ -928,7 +987,7 @@ __device__ static void kd_load_global(const DevMols &ms, const DevSets &st, int 
     __syncthreads();
 }
 
-// fast path of the sets of `list`; sets with a split tie go to `tied`
+// fast path of the sets of `list`; sets with a split tie also go to `tied`
 __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
                                                       const int *list, const unsigned int *count, int *tied,
                                                       unsigned int *n_tied)
@@ -957,13 +1016,17 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
         kd_load_global(ms, st, n, off, abase, w.c, (size_t)n2);
         // the arrangement after the work arrays: 61 * n2 + 4 * n <= 128 * n bytes
         int *order = (int *)((unsigned char *)(ib + 9 * n2) + n2);
-        if (kd_fast<KD_THREADS>(w, n, order, partial, flag)) kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
-        else if (threadIdx.x == 0) tied[atomicAdd(n_tied, 1u)] = s;
+        const bool tie = kd_fast<KD_THREADS>(w, n, order, partial, flag);
+        kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
+        if (threadIdx.x == 0) {
+            st.tied[s] = tie ? 1 : 0;
+            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
+        }
         __syncthreads();
     }
 }
 
-// exact path of the sets of `list`
+// exact path of the sets of `list`: their positions in the reference's order
 __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, DevTpls tp, DevSets st,
                                                                   unsigned char *scr, const int *list,
                                                                   const unsigned int *count)
@@ -993,8 +1056,12 @@ __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, De
         w.stride = (int)stride;
         kd_load_global(ms, st, n, off, abase, w.c, stride);
         int *order = ib + 9 * stride;   // the arrangement after the work arrays (64 * n bytes)
-        kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag);
-        kd_emit_order<KD_EXACT_THREADS>(ms, st, s, n, off, abase, order);
+        if (kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag)) {
+            for (int p = threadIdx.x; p < n; p += KD_EXACT_THREADS)
+                kd_write_xpos(st, n, off, p, [&](int i) { return order[i]; });
+        } else if (threadIdx.x == 0) {
+            atomicOr(st.kd_err, 1u);
+        }
         __syncthreads();
     }
 }

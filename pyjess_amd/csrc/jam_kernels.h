@@ -4,8 +4,10 @@
 //                  reference order (TessTemplate_candidates,
 //                  patches/TessTemplate.c.patch:173-235), two passes (count,
 //                  fill) with ballot compaction.
-//   k_kd*          (jam_kd.h) the leaf order of the reference's QuickSelect
-//                  kd-tree of every candidate set, the set stored as SoA in
+//   k_kd*          (jam_kd.h) a kd leaf order of every candidate set (the
+//                  reference's QuickSelect order, or for sets with ties at a
+//                  split point a valid spatial order plus each entry's
+//                  position in the reference's), the set stored as SoA in
 //                  that order plus one bounding box per 64 entries.
 //   k_pairs        thread per (template, molecule) pair: level order and the
 //                  number of level-0 partial matches.
@@ -577,8 +579,8 @@ __global__ void __launch_bounds__(256) k_iota(int *v, long long n)
 }
 
 // Sort key of complete match i for levels [l0, l0 + per): the positions of
-// the match's atoms in their sets (list position at level 0, kd position
-// above), `bits` bits each, the earliest level most significant; levels past
+// the match's atoms in their sets (list position at level 0, position in the
+// reference's kd DFS order above), `bits` bits each, the earliest level most significant; levels past
 // the template's size count as 0 (all matches of a pair have the same size).
 __device__ static inline unsigned long long level_key(const DevSets &st, const DevTpls &tp, const DevPairs &pr,
                                                       const DevJoin &jn, int i, int l0, int per, int bits)
@@ -591,8 +593,14 @@ __device__ static inline unsigned long long level_key(const DevSets &st, const D
     for (int q = 0; q < per; q++) {
         const int l = l0 + q;
         unsigned long long pos = 0;
-        if (l == 0) pos = (unsigned long long)st.kidx[row[0]];
-        else if (l < nt) pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
+        if (l == 0) {
+            pos = (unsigned long long)st.kidx[row[0]];
+        } else if (l < nt) {
+            // the position in the reference's kd order: the join's own
+            // position unless the set has ties at a split point
+            const PairLevel h = pr.lvl[(size_t)p * JAM_MAX_LEVELS + l];
+            pos = (unsigned long long)(st.tied[h.set] ? st.xpos[row[l]] : row[l] - h.off);
+        }
         k = (k << bits) | pos;
     }
     return k;

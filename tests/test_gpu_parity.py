@@ -15,24 +15,27 @@ from conftest import DOCTEST_DUMP, DOCTEST_RMSD, golden, parse_hit_file
 
 pytestmark = pytest.mark.gpu
 
-RTOL = 1e-9   # RMSD and rotation entries: within 1e-9 of the oracle
-
-
 def key(mol, h):
     return tuple(int(a.serial) for a in h.atoms(transform=False))
 
 
+def same_bits(x, y) -> bool:
+    """Equal FP64 values (NaN equal to NaN): both sides compute with the same
+    operations in the same order, without contraction."""
+    return np.array_equal(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64), equal_nan=True)
+
+
 def assert_same_hits(mol, tpls, got, want):
-    """Device hits equal oracle hits: same templates, atoms and order; RMSD and
-    rotation within RTOL."""
+    """Device hits equal oracle hits: same templates, atoms and order; RMSD,
+    rotation and centroids bit for bit."""
     assert len(got) == len(want)
     for g, w in zip(got, want):
         assert g.template() is tpls[w.template]
         assert key(mol, g) == tuple(int(mol.serial[a]) for a in w.atoms)
-        assert abs(g.rmsd - w.rmsd) <= RTOL * max(1.0, abs(w.rmsd))
-        assert np.allclose(g._rotation, w.rotation, rtol=0, atol=RTOL)
-        assert np.allclose(g._centre[0], w.c0, rtol=0, atol=RTOL)
-        assert np.allclose(g._centre[1], w.c1, rtol=0, atol=RTOL)
+        assert same_bits(g.rmsd, w.rmsd)
+        assert same_bits(np.ravel(g._rotation), w.rotation)
+        assert same_bits(g._centre[0], w.c0)
+        assert same_bits(g._centre[1], w.c1)
 
 
 def test_device_is_gfx950():
@@ -246,10 +249,36 @@ def test_kd_order_large_sets_and_ties(decimals, reorder):
     rng = np.random.Generator(np.random.PCG64(21))
     mol = synthetic.protein(rng, 2600)
     if decimals == 0:
-        mol.xyz[:] = np.round(mol.xyz)
+        mol.xyz = np.round(mol.xyz)
     tpls = [trp_template(mol, rng, f"W{i}") for i in range(6)]
     tpls += synthetic.template_set(rng, [mol], 20, 0.5, (2, 4), (2, 3), jitter=0.3, p_flexible=0.8)
     got = list(Jess(tpls).query(mol, 2.0, 1.5, 1.5, reorder=reorder))
     want = oracle.hits(mol, tpls, 2.0, 1.5, 1.5, reorder=reorder)
     assert len(want) > 0
     assert_same_hits(mol, tpls, got, want)
+
+
+@pytest.mark.parametrize("reorder", [True, False])
+def test_kd_ties_in_batched_queries(reorder):
+    """Several molecules with integer coordinates (candidate sets of 257 to
+    1024 entries full of ties) in one batched query: the tie handling of the
+    batched kd passes, in order, against the oracle molecule by molecule."""
+    rng = np.random.Generator(np.random.PCG64(33))
+    mols = []
+    for n in (2600, 3100, 1800):
+        m = synthetic.protein(rng, n)
+        m.xyz = np.round(m.xyz)
+        mols.append(m)
+    tpls = [trp_template(mols[0], rng, f"W{i}") for i in range(4)]
+    tpls += [trp_template(mols[1], rng, f"V{i}") for i in range(4)]
+    tpls += synthetic.template_set(rng, mols, 24, 0.5, (2, 4), (2, 3), jitter=0.3, p_flexible=0.8)
+    print(f"batched ties: {[len(m) for m in mols]} atoms x {len(tpls)} templates, reorder={reorder}", flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", UserWarning)
+        batch = Jess(tpls).query_batch(mols, 2.0, 1.5, 1.5, reorder=reorder)
+    total = 0
+    for mol, got in zip(mols, batch):
+        want = oracle.hits(mol, tpls, 2.0, 1.5, 1.5, reorder=reorder)
+        assert_same_hits(mol, tpls, got, want)
+        total += len(want)
+    assert total > 0

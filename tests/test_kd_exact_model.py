@@ -141,6 +141,6 @@ def test_levelwise_exact_arrangement(n, decimals):
     rng = np.random.Generator(np.random.PCG64(n + 1000 * decimals))
     mol = synthetic.protein(rng, max(3, n // 4 + 3))
     if decimals == 0:
-        mol.xyz[:] = np.round(mol.xyz)
+        mol.xyz = np.round(mol.xyz)
     atoms = rng.choice(len(mol), size=min(n, len(mol)), replace=False)
     assert arrange_levelwise(mol.xyz[atoms]) == oracle.kd_order(mol, atoms)

@@ -67,7 +67,7 @@ def test_parallel_model_equals_quickselect(n, decimals):
     rng = np.random.Generator(np.random.PCG64(n * 10 + decimals))
     mol = synthetic.protein(rng, max(3, n // 4 + 3))
     if decimals == 0:
-        mol.xyz[:] = np.round(mol.xyz)       # many ties
+        mol.xyz = np.round(mol.xyz)       # many ties
     atoms = rng.choice(len(mol), size=min(n, len(mol)), replace=False)
     got = parallel_model(mol.xyz[atoms])
     want = oracle.kd_order(mol, atoms)

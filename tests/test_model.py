@@ -140,3 +140,34 @@ def test_mat4_matmul():
     rng = np.random.default_rng(0)
     a, b = rng.normal(size=(4, 4)), rng.normal(size=(4, 4))
     assert np.allclose(np.asarray(Mat4(a) @ b), a @ b, rtol=0, atol=1e-12)
+
+
+def test_molecule_arrays_are_read_only():
+    """A molecule does not change once built (as the reference's cannot), so
+    its device copies stay valid: in-place edits raise."""
+    mol = Molecule.load(golden("1AMY.pdb"))
+    for attr in ("xyz", "names", "resnames", "chains", "resseq", "serial"):
+        arr = getattr(mol, attr)
+        assert not arr.flags.writeable
+        with pytest.raises(ValueError):
+            arr[0] = arr[1]
+
+
+def test_molecule_from_arrays_copies_caller_arrays():
+    xyz = np.zeros((2, 3))
+    names = np.frombuffer(b"_CA__CB_", dtype=np.uint8).reshape(2, 4).copy()
+    resnames = np.frombuffer(b"ALA\0ALA\0", dtype=np.uint8).reshape(2, 4).copy()
+    chains = np.array([[32, 65], [32, 65]], dtype=np.uint8)
+    mol = Molecule.from_arrays(xyz, names, resnames, chains, np.array([1, 1], dtype=np.int32))
+    xyz[0, 0] = 5.0
+    assert mol.xyz[0, 0] == 0.0
+
+
+def test_replacing_an_array_drops_device_copies():
+    mol = Molecule.load(golden("1AMY.pdb"))
+    mol._device[123] = object()
+    v0 = mol.__dict__.get("_version", 0)
+    mol.xyz = np.round(mol.xyz)
+    assert mol._device == {}
+    assert mol.__dict__["_version"] == v0 + 1
+    assert not mol.xyz.flags.writeable

@@ -148,7 +148,7 @@ def test_kd_order_with_ties():
     """Equal coordinates (3-decimal data) still give a valid median split."""
     rng = np.random.Generator(np.random.PCG64(7))
     mol = synthetic.protein(rng, 40)
-    mol.xyz[:] = np.round(mol.xyz / 2.0) * 2.0       # many ties on every axis
+    mol.xyz = np.round(mol.xyz / 2.0) * 2.0       # many ties on every axis
     atoms = np.arange(len(mol))
     order = oracle.kd_order(mol, atoms)
     check_median_split(mol.xyz[atoms], order, 0, len(atoms), 0)

@@ -111,3 +111,22 @@ def test_gradient_is_translation_invariant_and_sums_to_zero():
     _, ga2, gb2 = kabsch_rmsd_grad(A + 5.0, B - 3.0)
     assert np.allclose(ga, ga2, atol=1e-12) and np.allclose(gb, gb2, atol=1e-12)
     assert np.allclose(ga.sum(0), 0.0, atol=1e-12) and np.allclose(gb.sum(0), 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("off,n", [([1, 3], 3), ([0, 5, 3], 5), ([0, 2], 3), ([0, 4], 3), ([-1, 3], 3)])
+def test_malformed_offsets_raise_before_any_launch(off, n):
+    """Offsets that do not start at 0, decrease, or do not end at the number of
+    points are rejected on the host, before the kernels could see them."""
+    from pyjess_amd import autograd
+
+    with pytest.raises(ValueError):
+        autograd._check_offsets(torch.tensor(off, dtype=torch.int64), n)
+
+
+def test_valid_offsets_pass_and_host_tensors_are_refused():
+    from pyjess_amd import autograd
+
+    autograd._check_offsets(torch.tensor([0, 3, 3, 7], dtype=torch.int64), 7)
+    with pytest.raises(TypeError):
+        autograd.superposition_rmsd(torch.zeros((2, 3, 3), dtype=torch.float64),
+                                    torch.zeros((2, 3, 3), dtype=torch.float64))
