@@ -266,7 +266,7 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
     return out
 
 
-def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
+def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu, parity_all=True, search_only=False):
     """Library-scale batched search: 64 assemblies of 100,000 atoms x 7,000
     templates, streamed through jam_query in chunks of ``chunk`` assemblies
     (one batched call each, every (template, molecule) pair of the chunk).
@@ -277,8 +277,9 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
     chunks from two host threads so that copies overlap searches;
     ``drop_in`` is ``Jess.query_batch``
     (Python hit objects) on the first chunk. The CPU baseline is the C
-    restatement on the first assembly x all templates, and the hit set of that
-    assembly is checked against it."""
+    restatement on the first assembly x all templates; the hit set of every
+    assembly (of the first chunk with ``parity_all`` false) is checked against
+    the restatement run over all of them."""
     import ctypes
 
     from pyjess_amd import Jess, synthetic, _native
@@ -298,10 +299,13 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
     msets = [_native.MoleculeSet(ctx, c) for c in chunks]
     params = _native.Params(*qargs, 0, 1)
 
+    part = np.zeros(32, dtype=np.int64)   # partial matches per join level, of the last pass
+
     def device_pass():
         n_ev = 0
         stage = np.zeros(5)
         jms, jl = 0.0, 0
+        part[:] = 0
         for ms in msets:
             h = ctypes.c_void_p()
             with ctx.lock:
@@ -316,6 +320,9 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
                     _native._check(L.jam_result_join_kernels(h, ctypes.byref(x), ctypes.byref(nl)))
                     jms += x.value
                     jl += nl.value
+                    lp = np.zeros(32, dtype=np.int64)
+                    _native._check(L.jam_result_level_partials(h, _native._ptr(lp)))
+                    np.add(part, lp, out=part)
                 finally:
                     L.jam_result_free(h)
         return n_ev, stage, jms, jl
@@ -339,53 +346,49 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
         f"device ms: candidates {stage[0]:.1f} kd {stage[1]:.1f} join {stage[2]:.1f} super {stage[3]:.1f} "
         f"total {stage[4]:.1f}")
 
-    # the same with every event copied to host memory: two contexts (each
-    # with its own stream and workspace) take alternate chunks from two host
-    # threads, so that one chunk's event copy overlaps the next one's search
-    ctx2 = _native.Context()
-    tset2 = _native.TemplateSet(ctx2, jess._templates)
-    lanes = [[(tset, msets[i]) for i in range(0, len(msets), 2)],
-             [(tset2, _native.MoleculeSet(ctx2, chunks[i])) for i in range(1, len(msets), 2)]]
-    if lanes[1]:
-        _native.query(lanes[1][0][0], lanes[1][0][1], *qargs, 0, True)   # sizes the second workspace
-    results = [[], []]
-    errors = []
+    copy_s = drop_s = None
+    n_hits0 = 0
+    partials = part.copy()
+    if not search_only:
+        # the same with every event copied to host memory: two contexts (each
+        # with its own stream and workspace) take alternate chunks from two host
+        # threads, so that one chunk's event copy overlaps the next one's search
+        ctx2 = _native.Context()
+        tset2 = _native.TemplateSet(ctx2, jess._templates)
+        lanes = [[(tset, msets[i]) for i in range(0, len(msets), 2)],
+                 [(tset2, _native.MoleculeSet(ctx2, chunks[i])) for i in range(1, len(msets), 2)]]
+        if lanes[1]:
+            _native.query(lanes[1][0][0], lanes[1][0][1], *qargs, 0, True)   # sizes the second workspace
+        errors = []
 
-    def copy_lane(k):
-        try:
-            part = np.zeros(32, dtype=np.int64)
-            for n, (ts, ms) in enumerate(lanes[k]):
-                r = _native.query(ts, ms, *qargs, 0, True)
-                part += r.level_partials
-                if n == 0 and k == 0:
-                    results[k].append(r)
-                del r
-            results[k].append(part)
-        except Exception as e:   # re-raised on the main thread
-            errors.append(e)
+        def copy_lane(k):
+            try:
+                for ts, ms in lanes[k]:
+                    r = _native.query(ts, ms, *qargs, 0, True)
+                    del r
+            except Exception as e:   # re-raised on the main thread
+                errors.append(e)
 
-    threads = [threading.Thread(target=copy_lane, args=(k,)) for k in range(2)]
-    t0 = time.perf_counter()
-    for th in threads:
-        th.start()
-    for th in threads:
-        th.join()
-    copy_s = time.perf_counter() - t0
-    if errors:
-        raise errors[0]
-    first = results[0][0]
-    partials = results[0][-1] + results[1][-1]
-    log(f"library: with the event copy {copy_s * 1000:.1f} ms, {n_pairs / copy_s:,.0f} queries/s "
-        f"(two contexts, alternate chunks)")
+        threads = [threading.Thread(target=copy_lane, args=(k,)) for k in range(2)]
+        t0 = time.perf_counter()
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        copy_s = time.perf_counter() - t0
+        if errors:
+            raise errors[0]
+        log(f"library: with the event copy {copy_s * 1000:.1f} ms, {n_pairs / copy_s:,.0f} queries/s "
+            f"(two contexts, alternate chunks)")
 
-    # the drop-in batched API on the first chunk (Python hit objects)
-    t0 = time.perf_counter()
-    hits = jess.query_batch(chunks[0], *qargs, max_candidates=kw["max_candidates"])
-    drop_s = time.perf_counter() - t0
-    n_hits0 = sum(len(h) for h in hits)
-    del hits
-    log(f"library: query_batch on {len(chunks[0])} assemblies {drop_s * 1000:.1f} ms "
-        f"({len(chunks[0]) * T / drop_s:,.0f} queries/s, {n_hits0} hits)")
+        # the drop-in batched API on the first chunk (Python hit objects)
+        t0 = time.perf_counter()
+        hits = jess.query_batch(chunks[0], *qargs, max_candidates=kw["max_candidates"])
+        drop_s = time.perf_counter() - t0
+        n_hits0 = sum(len(h) for h in hits)
+        del hits
+        log(f"library: query_batch on {len(chunks[0])} assemblies {drop_s * 1000:.1f} ms "
+            f"({len(chunks[0]) * T / drop_s:,.0f} queries/s, {n_hits0} hits)")
 
     out = {
         "value": value,
@@ -400,9 +403,10 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
         "join_level_partials": [int(x) for x in partials[1:] if x > 0],
         "device_ms": {"candidates": stage[0], "kd_order": stage[1], "join": stage[2], "superposition": stage[3],
                       "total": stage[4]},
-        "with_event_copy": {"value": n_pairs / copy_s, "unit": "queries/s", "seconds": copy_s, "contexts": 2},
-        "drop_in": {"value": len(chunks[0]) * T / drop_s, "unit": "queries/s", "molecules": len(chunks[0]),
-                    "hits": n_hits0, "seconds": drop_s},
+        "with_event_copy": None if copy_s is None else {"value": n_pairs / copy_s, "unit": "queries/s",
+                                                        "seconds": copy_s, "contexts": 2},
+        "drop_in": None if drop_s is None else {"value": len(chunks[0]) * T / drop_s, "unit": "queries/s",
+                                                "molecules": len(chunks[0]), "hits": n_hits0, "seconds": drop_s},
         "k_join_ms": join_ms,
         "k_join_launches": join_launches,
         "roofline": None,
@@ -452,14 +456,47 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu):
                 if line.startswith("traffic_bytes_per_launch:"):
                     out["roofline"]["traffic"] = float(line.split(":")[1])
                     out["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
-        sizes = [len(t) for t in tpls]
-        sel = (first.flag == 1) & (first.pair < T)
-        gpu_hc, gpu_dg = pair_digests(first.pair[sel], first.atoms[sel], sizes, T, T)
-        parity = bool(np.array_equal(gpu_hc, hc) and np.array_equal(gpu_dg, dg.astype(np.uint64)) and
-                      np.array_equal(first.pair_matches[:T], cm))
+        # hit-set parity on every assembly: per pair, complete matches, hits
+        # and an order-independent digest of the hits -- on the device from
+        # the events in HBM (jam_result_pair_digests), chunk by chunk, and by
+        # the oracle over all assemblies on the host's cores
+        n_par = len(mols) if parity_all else min(len(mols), chunk)
+        gpu_cm = np.zeros(n_par * T, dtype=np.int64)
+        gpu_hc = np.zeros(n_par * T, dtype=np.int64)
+        gpu_dg = np.zeros(n_par * T, dtype=np.uint64)
+        for ci, ms in enumerate(msets):
+            m0 = ci * chunk
+            if m0 >= n_par:
+                break
+            h = ctypes.c_void_p()
+            with ctx.lock:
+                _native._check(L.jam_query(ctx.handle, tset.handle, ms.handle, ctypes.byref(params),
+                                           ctypes.byref(h)))
+                try:
+                    npc = L.jam_result_n_pairs(h)
+                    cm_c = np.zeros(npc, dtype=np.int64)
+                    hc_c = np.zeros(npc, dtype=np.int64)
+                    dg_c = np.zeros(npc, dtype=np.uint64)
+                    _native._check(L.jam_result_pair_matches(h, _native._ptr(cm_c)))
+                    _native._check(L.jam_result_pair_digests(h, m0, T, _native._ptr(hc_c), _native._ptr(dg_c)))
+                finally:
+                    L.jam_result_free(h)
+            k = min(npc, (n_par - m0) * T)
+            gpu_cm[m0 * T:m0 * T + k] = cm_c[:k]
+            gpu_hc[m0 * T:m0 * T + k] = hc_c[:k]
+            gpu_dg[m0 * T:m0 * T + k] = dg_c[:k]
+        t0 = time.perf_counter()
+        cm_all, hc_all, dg_all = oracle.batch([oracle.MolView.of(m) for m in mols[:n_par]], tv, *qargs,
+                                              n_threads=threads)
+        full_s = time.perf_counter() - t0
+        parity = bool(np.array_equal(gpu_cm, cm_all) and np.array_equal(gpu_hc, hc_all) and
+                      np.array_equal(gpu_dg, dg_all.astype(np.uint64)))
         out["hit_set_parity"] = parity
-        log(f"library: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads; hit-set parity on the "
-            f"first assembly: {parity}")
+        out["hit_set_parity_scope"] = {"assemblies": n_par, "of": len(mols), "pairs": int(n_par * T),
+                                       "hits": int(hc_all.sum()), "complete_matches": int(cm_all.sum()),
+                                       "oracle_seconds": full_s, "oracle_threads": threads}
+        log(f"library: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads; hit-set parity on "
+            f"{n_par}/{len(mols)} assemblies ({int(hc_all.sum())} hits, oracle {full_s:.1f}s): {parity}")
     return out
 
 
@@ -609,6 +646,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of each CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--search-only", action="store_true",
+                    help="library-64: only the device-resident search (no event copy, no query_batch leg)")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the 4,096 molecules of the batched measurement (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -636,7 +675,7 @@ def main():
         args.steps = min(args.steps, 3)
         args.warmup = min(args.warmup, 1)
         d = measure_library(args.scale, args.seed, args.library_chunk, args.steps, args.warmup, args.cpu_seconds,
-                            not args.no_cpu)
+                            not args.no_cpu, search_only=args.search_only)
         out = {"metric": METRIC, "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": d["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",

@@ -142,6 +142,15 @@ int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32
                         int32_t *atoms_stride);
 /* Number of complete matches (before the RMSD threshold) of every pair. */
 int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts);
+/* Measurement (no reference counterpart): per pair, the number of hits
+ * (events with flag 1) and, summed mod 2^64 over them, a 64-bit digest of
+ * each hit -- h = mix64(template << 32 ^ (mol_base + molecule)), then
+ * h = mix64(h ^ atom) for each matched atom in template order, mix64 being
+ * SplitMix64's finaliser -- an order-independent digest of the hit set that
+ * a CPU run computes the same way. Computed on the device from the events in
+ * HBM (n_pairs values each); JAM_ESTALE once a later query has run. */
+int32_t jam_result_pair_digests(const jam_result *r, int32_t mol_base, int32_t n_templates, int64_t *hit_counts,
+                                uint64_t *digests);
 /* Kernel durations of the last query, milliseconds: [candidates, kd order,
  * join, superposition, total device time]. */
 int32_t jam_result_timings(const jam_result *r, double *ms5);

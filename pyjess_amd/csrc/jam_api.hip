@@ -103,7 +103,7 @@ struct jam_context {
     hipEvent_t xev[2 * JAM_MAX_LEVELS] = {};
     bool profile = false;       // join launches carry start/stop events (xev)
     // per-query workspace
-    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, kscr, kd_bucket, kd_tied, kd_count, ktied, kxpos;
+    DBuf cnt, off, list, kidx, l2k, kent, katom, bbox, sbox, kscr, kd_bucket, kd_tied, kd_count;
     DBuf p_ord, p_lvl, p_nchunk, p_n1, p_base, p_cm, p_first, i_pair;
     // join: two frontiers (pair + stride entries per row), complete matches,
     // counters (rows per level, complete matches, overflow flag)
@@ -112,12 +112,11 @@ struct jam_context {
     int front_stride = 0, done_stride = 0;
     int join_grid[4] = {0, 0, 0, 0};   // resident workgroups of the join kernel <8/16/24/32>
     int kd_grid[12] = {};           // resident workgroups of the kd kernels
-    // each kd size tier runs on a stream of its own: its fast pass, then the
-    // exact pass of its tied sets (which only the ranking of the complete
-    // matches needs, so it runs beside the join); all streams at the default
-    // priority
+    // the exact kd pass of each size tier runs on a stream of its own, from
+    // the moment the tier's fast pass has listed its tied sets, beside the
+    // fast passes of the other tiers (all streams at the default priority)
     hipStream_t kd_side[KD_TIERS] = {};
-    hipEvent_t kd_fork = nullptr, kd_fast[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
+    hipEvent_t kd_fast[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
     bool kd_pending = false;    // side-stream work of the last kd launch may still run
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
@@ -125,7 +124,7 @@ struct jam_context {
     DBuf e_all;  // the events of the last query, event_layout(count, stride)
     DBuf keep_d, keep_i;  // superpositions kept between the two k_super passes (few matches)
     DBuf scan_tmp, scan_tmp2;
-    int64_t set_cap = 0, blk_cap = 0, event_cap = 0;
+    int64_t set_cap = 0, blk_cap = 0, sblk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
     int64_t items_cap = 0;
     int event_stride = 0;
@@ -143,6 +142,8 @@ struct jam_context {
     DBuf sa, sb, srmsd, srot, sc0, sc1;
     // standalone kd order: points, per-set offsets, zero chain/resSeq words
     DBuf o_xyz, o_aoff, o_zero;
+    // per-pair hit counts and digests of jam_result_pair_digests
+    DBuf dig;
 };
 
 struct jam_templates {
@@ -287,7 +288,6 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) HIP_TRY(hipEventCreate(&ev));
     for (auto &ev : c->xev) HIP_TRY(hipEventCreate(&ev));
-    HIP_TRY(hipEventCreateWithFlags(&c->kd_fork, hipEventDisableTiming));
     for (int k = 0; k < KD_TIERS; k++) {
         HIP_TRY(hipStreamCreateWithFlags(&c->kd_side[k], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&c->kd_fast[k], hipEventDisableTiming));
@@ -324,7 +324,6 @@ extern "C" void jam_context_free(jam_context *ctx)
         if (ctx->kd_joined[k]) (void)hipEventDestroy(ctx->kd_joined[k]);
         if (ctx->kd_side[k]) (void)hipStreamDestroy(ctx->kd_side[k]);
     }
-    if (ctx->kd_fork) (void)hipEventDestroy(ctx->kd_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -522,7 +521,6 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
         HIP_TRY(c->kd_bucket.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
         HIP_TRY(c->kd_tied.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
-        HIP_TRY(c->ktied.reserve((size_t)n_sets));
         c->nsets_cap = n_sets;
     }
     const int64_t blk = cap / JAM_WAVE + n_sets + 1;
@@ -534,13 +532,18 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
         HIP_TRY(c->kent.reserve(sizeof(SetEnt) * cap));
         HIP_TRY(c->katom.reserve(sizeof(int) * cap));
         HIP_TRY(c->kscr.reserve((size_t)KD_SCRATCH_BYTES_PER_ENTRY * cap));
-        HIP_TRY(c->kxpos.reserve(sizeof(int) * cap));
         c->set_cap = cap;
     }
+    // block and sub-block boxes: set s's boxes start at off[s] / size + s
     const int64_t blk_need = c->set_cap / JAM_WAVE + n_sets + 1;
     if (blk_need > c->blk_cap || blk > c->blk_cap) {
         c->blk_cap = std::max(blk, blk_need);
         HIP_TRY(c->bbox.reserve(sizeof(double) * 6 * (size_t)c->blk_cap));
+    }
+    const int64_t sblk_need = c->set_cap / JAM_SUB + n_sets + 1;
+    if (sblk_need > c->sblk_cap) {
+        c->sblk_cap = sblk_need;
+        HIP_TRY(c->sbox.reserve(sizeof(double) * 6 * (size_t)c->sblk_cap));
     }
     // per tier: sets, tied sets; then an error word for jam_kd_order
     HIP_TRY(c->kd_count.reserve(sizeof(unsigned int) * (2 * KD_TIERS + 1)));
@@ -729,15 +732,22 @@ void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, c
 }
 
 
-// Launches the kd passes over the candidate sets of `st` on the side streams:
-// the sets are bucketed by size tier on `s`, then tier k runs on
-// c->kd_side[k] its fast pass (every set: <= 64 entries in one wave's
-// registers, up to KD_BIG with one position per thread, larger in global
-// scratch; sets with a tie at a split point are listed) and then the exact
-// QuickSelect pass over the listed sets. kd_fast[k] is recorded after the
-// fast pass, kd_joined[k] after the exact pass. Grids are upper bounds (the
-// counts stay on the device); every kernel loops over its own list.
-int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const DevSets &st, hipStream_t s)
+// Launches the kd passes over the candidate sets of `st`: the sets are
+// bucketed by size tier, then every tier gets its fast pass on `s` (every
+// set: <= 64 entries in one wave's registers, up to KD_BIG with one position
+// per thread, larger in global scratch; a set with a tie at a split point is
+// listed instead of written) and, on the tier's side stream from the end of
+// that fast pass, the exact QuickSelect pass over the listed sets. With one
+// molecule (few sets, latency-bound) the 513-1024 and 257-512 tiers skip the
+// fast pass and run the exact pass on all their sets from the start: for a
+// handful of big sets it is the kd phase's longest chain, and the fast pass
+// would mostly give up on them (with 0.001 A coordinates a big set nearly
+// always has equal values at one of its n-1 split points). kd_joined[k] is
+// recorded at the end of tier k's passes; every set is written once all are.
+// Grids are upper bounds (the counts stay on the device); every kernel loops
+// over its own list.
+int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const DevSets &st, hipStream_t s,
+                  bool one_molecule)
 {
     const int n_sets = st.n_sets;
     unsigned char *scr = c->kscr.as<unsigned char>();
@@ -794,9 +804,12 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
             break;
         }
     };
-    auto launch_exact = [&](int k, hipStream_t x) {
-        const int *lst = tied + (size_t)k * ns;
-        const unsigned int *lc = nt + k;
+    // the exact pass of tier k over the sets its fast pass listed, or
+    // (`all`) over every set of the tier
+    auto launch_exact = [&](int k, hipStream_t x, bool all) {
+        const size_t o = (size_t)k * ns;
+        const int *lst = all ? bucket + o : tied + o;
+        const unsigned int *lc = all ? nb + k : nt + k;
         switch (k) {
         case 0:
             hipLaunchKernelGGL((jam::k_kdexact<64>), grid(6), dim3(64), 0, x, dm, dt, st, lst, lc);
@@ -819,20 +832,26 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
             break;
         }
     };
-    HIP_TRY(hipEventRecord(c->kd_fork, s));
     c->kd_pending = true;
-    // largest tier first: its passes are the longest
-    for (int k = KD_TIERS - 1; k >= 0; k--) {
-        hipStream_t x = c->kd_side[k];
-        HIP_TRY(hipStreamWaitEvent(x, c->kd_fork, 0));
-        launch_fast(k, x);
-        HIP_TRY(hipEventRecord(c->kd_fast[k], x));
-        launch_exact(k, x);
-        HIP_TRY(hipEventRecord(c->kd_joined[k], x));
+    // largest tier first: its exact pass, the longest, starts earliest (with
+    // one molecule the exact-only tiers first: sets above 1,024 entries are
+    // rare there)
+    static const int by_size[KD_TIERS] = {5, 4, 3, 2, 1, 0};
+    static const int one_mol[KD_TIERS] = {4, 3, 5, 2, 1, 0};
+    static_assert(KD_TIERS == 6, "tier orders list six tiers");
+    for (int i = 0; i < KD_TIERS; i++) {
+        const int k = one_molecule ? one_mol[i] : by_size[i];
+        const bool all = one_molecule && (k == 3 || k == 4);
+        if (!all) launch_fast(k, s);
+        HIP_TRY(hipEventRecord(c->kd_fast[k], s));
+        HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fast[k], 0));
+        launch_exact(k, c->kd_side[k], all);
+        HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
     }
     HIP_TRY(hipGetLastError());
     return JAM_OK;
 }
+
 }  // namespace
 
 // One device array of counters, read back in one copy after the join: rows
@@ -911,13 +930,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         st.kent = c->kent.as<SetEnt>();
         st.katom = c->katom.as<int>();
         st.bbox = c->bbox.as<double>();
+        st.sbox = c->sbox.as<double>();
         unsigned long long *cnts = c->j_count.as<unsigned long long>();
         st.total = (int64_t *)(cnts + CNT_TOTAL);
         st.maxcnt = (int *)(cnts + CNT_MAXCNT);
-        st.tied = c->ktied.as<unsigned char>();
-        st.xpos = c->kxpos.as<int>();
         // the kd error word sits after the per-pair counts, which are read
-        // back at the end of the query (the exact passes run until then)
+        // back at the end of the query
         unsigned long long *pcm = (unsigned long long *)c->p_cm.ptr;
         st.kd_err = (unsigned int *)(pcm + (size_t)n_pairs + 1);
         DevPairs pr;
@@ -951,18 +969,16 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                            (long long *)(cnts + CNT_ITEMS));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[1], s));
-        // the kd passes: each tier's fast pass, then the exact pass of its
-        // tied sets, on the tier's own stream; the join needs only the fast
-        // passes, the ranking after it the exact ones
-        if ((rc = launch_kd(c, dm, dt, st, s)) != JAM_OK) {
+        if ((rc = launch_kd(c, dm, dt, st, s, M == 1)) != JAM_OK) {
             drain_kd(c);
             return rc;
         }
         // the join's counters and level-0 flags do not depend on the kd
-        // order: set them up meanwhile
+        // order: set them up while the exact passes still run
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnts, (long long)CNT_TOTAL);
         hipLaunchKernelGGL(k_items_any, dim3(pair_blocks), dim3(256), 0, s, pr);
-        for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_fast[k], 0));
+        for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
+        c->kd_pending = false;   // the rest of the query is ordered after every kd pass
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c->ev[2], s));
 
@@ -1022,10 +1038,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         const unsigned int ovf = (unsigned int)(jc[CNT_OVF] & 0xFFFFFFFFull);
         const int64_t n_done = (int64_t)jc[CNT_DONE];
         const bool regrow = total > c->set_cap || n0 > c->items_cap || n0 > c->front_cap;
-        if (regrow || ovf || n_done > c->done_cap) {
-            // the exact kd passes may still read the buffers about to grow
-            drain_kd(c);
-        }
         if (regrow) {
             if (total > c->set_cap && (rc = grow_sets(c, total, n_sets)) != JAM_OK) return rc;
             if (n0 > c->items_cap && (rc = grow_items(c, n0)) != JAM_OK) return rc;
@@ -1046,8 +1058,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             continue;
         }
         HIP_TRY(hipEventRecord(c->ev[3], s));
-        // the ranking needs every set's position in the reference's kd order
-        for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
 
         // ---- reference order of the complete matches: the order of the key
         // (pair, level 0 position, level 1 position, ...), by rank counting
@@ -1163,8 +1173,9 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         HIP_TRY(hipMemcpyAsync(c->hsmall, pcm, sizeof(unsigned long long) * ((size_t)n_pairs + 2),
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        c->kd_pending = false;   // s waited for every side stream
-        if (c->hsmall[(size_t)n_pairs + 1] != 0ull)
+        // the error word is the low 32 bits of that slot (only they are
+        // cleared and set on the device)
+        if ((c->hsmall[(size_t)n_pairs + 1] & 0xFFFFFFFFull) != 0ull)
             return fail(JAM_EDEVICE, "a kd pass reached its round bound (broken QuickSelect invariant)");
         const unsigned long long ne_dev = c->hsmall[0];
         std::memcpy(R->pair_cm.data(), c->hsmall + 1, sizeof(long long) * (size_t)n_pairs);
@@ -1318,6 +1329,33 @@ extern "C" int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts)
     return JAM_OK;
 }
 
+extern "C" int32_t jam_result_pair_digests(const jam_result *r, int32_t mol_base, int32_t n_templates,
+                                           int64_t *hit_counts, uint64_t *digests)
+{
+    if (!r || !hit_counts || !digests) return fail(JAM_EINVAL, "NULL argument");
+    if (n_templates < 1 || r->n_pairs % n_templates != 0) return fail(JAM_EINVAL, "invalid template count");
+    jam_context *c = r->ctx;
+    if (!c || c->query_gen != r->gen)
+        return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
+    const size_t np = (size_t)r->n_pairs;
+    if (np == 0) return JAM_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    HIP_TRY(c->dig.reserve(2 * sizeof(unsigned long long) * np));
+    unsigned long long *d = c->dig.as<unsigned long long>();
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)std::min<size_t>(1024, (2 * np + 255) / 256)), dim3(256), 0, s, d,
+                       (long long)(2 * np));
+    if (r->n_events > 0)
+        hipLaunchKernelGGL(jam::k_pair_digest, dim3((unsigned)((r->n_events + 255) / 256)), dim3(256), 0, s,
+                           c->e_all.as<const unsigned char>(), (long long)r->n_events, r->stride, n_templates,
+                           mol_base, d, d + np);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(hit_counts, d, sizeof(int64_t) * np, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(digests, d + np, sizeof(uint64_t) * np, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}
+
 extern "C" int32_t jam_result_timings(const jam_result *r, double *ms5)
 {
     if (!r || !ms5) return fail(JAM_EINVAL, "NULL argument");
@@ -1353,6 +1391,33 @@ extern "C" void jam_result_free(jam_result *r) { delete r; }
 // standalone superposition
 // ---------------------------------------------------------------------------
 
+namespace {
+
+// k_superpose_batch over device arrays, on stream s
+int32_t superpose_launch(hipStream_t s, const double *da, const double *db, int k, int64_t n_sets, double *drmsd,
+                         double *drot, double *dc0, double *dc1)
+{
+    const int tb = 256;
+    hipLaunchKernelGGL(jam::k_superpose_batch, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0, s, da, db, k,
+                       (long long)n_sets, drmsd, drot, dc0, dc1);
+    HIP_TRY(hipGetLastError());
+    return JAM_OK;
+}
+
+// copies the outputs back and waits for them
+int32_t superpose_fetch(hipStream_t s, int64_t n_sets, const void *drmsd, const void *drot, const void *dc0,
+                        const void *dc1, double *rmsd, double *rot, double *c0, double *c1)
+{
+    HIP_TRY(hipMemcpyAsync(rmsd, drmsd, sizeof(double) * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(rot, drot, sizeof(double) * 9 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c0, dc0, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(c1, dc1, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}
+
+}  // namespace
+
 extern "C" int32_t jam_superpose_batch(jam_context *c, const double *a, const double *b, int32_t k,
                                        int64_t n_sets, double *rmsd, double *rot, double *c0, double *c1)
 {
@@ -1370,17 +1435,10 @@ extern "C" int32_t jam_superpose_batch(jam_context *c, const double *a, const do
     HIP_TRY(c->sc1.reserve(sizeof(double) * 3 * n_sets));
     HIP_TRY(hipMemcpyAsync(c->sa.ptr, a, sizeof(double) * pts, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->sb.ptr, b, sizeof(double) * pts, hipMemcpyHostToDevice, s));
-    const int tb = 256;
-    hipLaunchKernelGGL(jam::k_superpose_batch, dim3((unsigned)((n_sets + tb - 1) / tb)), dim3(tb), 0, s,
-                       c->sa.as<double>(), c->sb.as<double>(), k, (long long)n_sets, c->srmsd.as<double>(),
-                       c->srot.as<double>(), c->sc0.as<double>(), c->sc1.as<double>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(rmsd, c->srmsd.ptr, sizeof(double) * n_sets, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(rot, c->srot.ptr, sizeof(double) * 9 * n_sets, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(c0, c->sc0.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(c1, c->sc1.ptr, sizeof(double) * 3 * n_sets, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    return JAM_OK;
+    int32_t rc = superpose_launch(s, c->sa.as<double>(), c->sb.as<double>(), k, n_sets, c->srmsd.as<double>(),
+                                  c->srot.as<double>(), c->sc0.as<double>(), c->sc1.as<double>());
+    if (rc != JAM_OK) return rc;
+    return superpose_fetch(s, n_sets, c->srmsd.ptr, c->srot.ptr, c->sc0.ptr, c->sc1.ptr, rmsd, rot, c0, c1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1443,37 +1501,28 @@ extern "C" int32_t jam_kd_order(jam_context *c, const double *xyz, const int64_t
     st.kent = c->kent.as<SetEnt>();
     st.katom = c->katom.as<int>();
     st.bbox = c->bbox.as<double>();
+    st.sbox = c->sbox.as<double>();
     st.total = (int64_t *)(cnts + CNT_TOTAL);
     st.maxcnt = (int *)(cnts + CNT_MAXCNT);
-    st.tied = c->ktied.as<unsigned char>();
-    st.xpos = c->kxpos.as<int>();
     st.kd_err = c->kd_count.as<unsigned int>() + 2 * KD_TIERS;
-    if ((rc = launch_kd(c, dm, dt, st, s)) != JAM_OK) {
+    if ((rc = launch_kd(c, dm, dt, st, s, n_sets == 1)) != JAM_OK) {
         drain_kd(c);
         return rc;
     }
     for (int k = 0; k < KD_TIERS; k++) HIP_TRY(hipStreamWaitEvent(s, c->kd_joined[k], 0));
-    std::vector<int> kidx((size_t)total), xpos((size_t)total);
-    std::vector<unsigned char> tied((size_t)n_sets);
+    std::vector<int> kidx((size_t)total);
     unsigned int err = 0;
     HIP_TRY(hipMemcpyAsync(kidx.data(), st.kidx, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(xpos.data(), st.xpos, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(tied.data(), st.tied, (size_t)n_sets, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&err, st.kd_err, sizeof(err), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     c->kd_pending = false;
     if (err) return fail(JAM_EDEVICE, "a kd pass reached its round bound (broken QuickSelect invariant)");
-    // DFS position p (right child first) is leaf n-1-p; kd position q holds
-    // list element kidx[q], which sits at DFS position xpos[q] of the
-    // reference's order when the set is tied, else at q
+    // DFS position p (right child first) holds list element kidx[p] and is
+    // leaf n-1-p
     for (int32_t k = 0; k < n_sets; k++) {
         const int64_t o = off[k];
         const int n = cnt[k];
-        for (int q = 0; q < n; q++) {
-            const int p = tied[k] ? xpos[(size_t)o + q] : q;
-            if (p < 0 || p >= n) return fail(JAM_EDEVICE, "kd positions out of range");
-            order[(size_t)o + (n - 1 - p)] = kidx[(size_t)o + q];
-        }
+        for (int p = 0; p < n; p++) order[(size_t)o + (n - 1 - p)] = kidx[(size_t)o + p];
     }
     return JAM_OK;
 }

@@ -15,8 +15,7 @@
 //   sets        reference order (molecule-local atom index), and the set in
 //               kd-tree DFS leaf order (a 32-byte entry: x, y, z, chain,
 //               resSeq; and the atom index) with one bounding box per 64
-//               consecutive entries; for sets with ties at a split point,
-//               each position's place in the reference's own order.
+//               consecutive entries and one per 16.
 //   join        frontier rows (pair, one entry index per placed level),
 //               double buffered, and the complete matches.
 #pragma once
@@ -24,6 +23,8 @@
 #include <stdint.h>
 
 #define JAM_WAVE 64
+// entries per sub-block: the join tests a sub-block's box before its entries
+#define JAM_SUB 16
 
 struct DevMols {
     int n_mols;
@@ -77,16 +78,10 @@ struct DevSets {
     int *l2k;                 // list position -> kd position (set-local), per entry
     SetEnt *kent;             // kd DFS order
     int *katom;               // kd DFS order: molecule-local atom
-    double *bbox;             // 6 per block: lo xyz, hi xyz
+    double *bbox;             // 6 per block of 64 entries: lo xyz, hi xyz
+    double *sbox;             // 6 per sub-block of JAM_SUB entries
     int64_t *total;           // device scalar: sum of cnt
     int *maxcnt;              // device scalar: largest cnt
-    // Sets with equal coordinates on both sides of a split point: the kd
-    // arrays above then hold a valid spatial order (the join's pruning needs
-    // nothing more), and xpos maps each of its positions to the position of
-    // the same entry in the reference's own order (QuickSelect's swap
-    // sequence), which ranks the complete matches.
-    unsigned char *tied;      // per set: 1 when xpos applies
-    int *xpos;                // per entry: set-local position in the reference's DFS order
     unsigned int *kd_err;     // device word, 0 unless a kd pass hit its round bound
 };
 

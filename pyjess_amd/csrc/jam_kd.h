@@ -13,16 +13,14 @@
 // the atom index) with one bounding box per 64 consecutive entries, which the
 // join uses for pruning.
 //
-// Two algorithms:
-//   fast   the left half of every range is taken as its n/2 smallest
-//          elements by (coordinate, element): sort every axis once, then at
-//          level d cut each range of axis d%3 in half and stably partition
-//          the two other axes by side. This is the reference's order whenever
-//          no range has equal coordinates on both sides of its split point;
-//          otherwise it is still a valid spatial order (every left half lies
-//          below its right half on the split axis), which is all the join's
-//          pruning needs, and the set is flagged tied.
-//   exact  for the tied sets, beside the join: all ranges of a tree level
+// Two algorithms, both exact:
+//   fast   when no range of the tree has equal coordinates on both sides of
+//          its split point, the left half of every range is its n/2 smallest
+//          elements whatever the arrangement, and the leaf order follows from
+//          memberships alone: sort every axis once, then at level d cut each
+//          range of axis d%3 in half and stably partition the two other axes
+//          by side. Detects a tie across a split point and then gives up.
+//   exact  for the sets the fast path gave up on: all ranges of a tree level
 //          run their QuickSelect together, one partition round at a time,
 //          and each Lomuto partition is computed without its serial scan:
 //            * the "<= pivot" elements keep their order: a stable compaction;
@@ -33,16 +31,13 @@
 //              every push to its element. The final queue fills
 //              [l+k+1, r-1] from its second element on, its front goes to r,
 //              the pivot to l+k.
-//          It writes, for every position of the fast order, the position of
-//          the same entry in the reference's order (xpos), which ranks the
-//          complete matches before they are reported.
 //
 // Sets are bucketed by size; sets of up to 64 entries run in one wave's
 // registers, up to KD_BIG entries with one position per thread (shuffles,
 // ballots and LDS exchanges), larger ones in global scratch. Every loop of the
 // exact pass whose length depends on the data has a bound that a correct
 // chain never reaches; a pass that reaches it sets st.kd_err and leaves the
-// set's positions unwritten, and the query reports an error.
+// set unwritten, and the query reports an error.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -111,7 +106,8 @@ __global__ void __launch_bounds__(256) k_kdbucket(DevSets st, int *bucket, unsig
 
 // Writes position p of set s (element e of its candidate list, p < n) and,
 // with the other lanes of the wave holding positions p & ~63 ... p | 63, the
-// bounding box of that 64-entry block. Every lane of the wave calls it.
+// bounding boxes of that 64-entry block and of its JAM_SUB-entry sub-blocks.
+// Every lane of the wave calls it.
 __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st, int s, int n, int64_t off,
                                            int abase, int p, int e)
 {
@@ -136,7 +132,24 @@ __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st,
     }
     double blo[3] = {ok ? x : INFINITY, ok ? y : INFINITY, ok ? z : INFINITY};
     double bhi[3] = {ok ? x : -INFINITY, ok ? y : -INFINITY, ok ? z : -INFINITY};
-    for (int sh = 32; sh >= 1; sh >>= 1) {
+    // the box of each JAM_SUB entries, then of the 64
+    for (int sh = JAM_SUB / 2; sh >= 1; sh >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            blo[q] = fmin(blo[q], __shfl_xor(blo[q], sh, JAM_WAVE));
+            bhi[q] = fmax(bhi[q], __shfl_xor(bhi[q], sh, JAM_WAVE));
+        }
+    }
+    if ((p & (JAM_SUB - 1)) == 0 && p < n) {
+        double *sb = st.sbox + 6 * (off / JAM_SUB + s + p / JAM_SUB);
+        sb[0] = blo[0];
+        sb[1] = blo[1];
+        sb[2] = blo[2];
+        sb[3] = bhi[0];
+        sb[4] = bhi[1];
+        sb[5] = bhi[2];
+    }
+    for (int sh = JAM_WAVE / 2; sh >= JAM_SUB; sh >>= 1) {
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             blo[q] = fmin(blo[q], __shfl_xor(blo[q], sh, JAM_WAVE));
@@ -160,8 +173,7 @@ __device__ static inline void kd_emit_lane(const DevMols &ms, const DevSets &st,
 // everything in registers
 // ---------------------------------------------------------------------------
 
-// Returns true when a tie straddles a split point (the order written is then
-// the fast one, not the reference's).
+// Returns true (nothing written) when a tie straddles a split point.
 __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, int n, int64_t off, int abase)
 {
     const int lane = lane_id();
@@ -195,7 +207,6 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
         perm[a] = id;
     }
     int sb = 0, se = n;
-    bool tie = false;
     for (int d = 0;; d++) {
         const int a = d % 3;
         const bool act = lane < n && se - sb >= 2;
@@ -207,7 +218,7 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
         const double ca = (a == 0) ? c[0] : (a == 1) ? c[1] : c[2];
         const double here = __shfl(ca, pa & 63, JAM_WAVE);
         const double prev = __shfl_up(here, 1, JAM_WAVE);
-        if (ballot(split && prev == here) != 0ull) tie = true;
+        if (ballot(split && prev == here) != 0ull) return true;
         // side of every element, delivered to the lane of the element id
         const int left_pos = act && lane < m ? 1 : 0;
         const int side_by_elem = push_int(left_pos, pa & 63);
@@ -236,7 +247,7 @@ __device__ static bool kd_wave64(const DevMols &ms, const DevSets &st, int s, in
     const int src = n - 1 - lane;
     const int e = shfl_int(perm[0], src < 0 ? 0 : src);
     kd_emit_lane(ms, st, s, n, off, abase, lane, e);
-    return tie;
+    return false;
 }
 
 __global__ void __launch_bounds__(JAM_WAVE) k_kdwave(DevMols ms, DevTpls tp, DevSets st, const int *list,
@@ -249,11 +260,7 @@ __global__ void __launch_bounds__(JAM_WAVE) k_kdwave(DevMols ms, DevTpls tp, Dev
         const int n = st.cnt[s];
         const int64_t off = st.off[s];
         const int abase = ms.atom_off[s / tp.n_types];
-        const bool tie = kd_wave64(ms, st, s, n, off, abase);
-        if (lane_id() == 0) {
-            st.tied[s] = tie ? 1 : 0;
-            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
-        }
+        if (kd_wave64(ms, st, s, n, off, abase) && lane_id() == 0) tied[atomicAdd(n_tied, 1u)] = s;
     }
 }
 
@@ -341,7 +348,10 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
             side[ia] = (act && t < m) ? 1 : 0;
             __syncthreads();
             const bool my_tie = act && t == m && xkey[t - 1] == ka;
-            if (__syncthreads_or(my_tie)) tie = true;
+            if (__syncthreads_or(my_tie)) {
+                tie = true;
+                break;
+            }
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 if (b == a) continue;
@@ -371,9 +381,10 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
                 else sb = m;
             }
         }
-        if (t == 0) {
-            st.tied[s] = tie ? 1 : 0;
-            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
+        if (tie) {
+            if (t == 0) tied[atomicAdd(n_tied, 1u)] = s;
+            __syncthreads();
+            continue;
         }
         // DFS order: position p holds element arrangement[n-1-p]
         xid[t] = id[0];
@@ -415,19 +426,6 @@ __host__ __device__ constexpr int kd_log2_ceil(int n) { return n <= 1 ? 0 : 1 + 
 // reports it instead of looping.
 #define KD_JUMP_BOUND(n) (kd_log2_ceil(n) + 2)
 #define KD_ROUND_BOUND(n) ((long long)(n) * (kd_log2_ceil(n) + 2) + 64)
-
-// Position of every entry of set s in the reference's DFS order: the exact
-// arrangement `arr_at(i)` (element at leaf i, left subtree first) visited
-// right child first, so DFS position p holds element arr_at(n-1-p). Written
-// through the fast order's list -> position map, at the fast position.
-template <class F>
-__device__ static inline void kd_write_xpos(const DevSets &st, int n, int64_t off, int p, F arr_at)
-{
-    if (p < n) {
-        const int e = arr_at(n - 1 - p);
-        st.xpos[off + st.l2k[off + e]] = p;
-    }
-}
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets st, const int *list,
@@ -614,7 +612,9 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
         if (broken) {
             if (t == 0) atomicOr(st.kd_err, 1u);
         } else {
-            kd_write_xpos(st, n, off, t, [&](int i) { return xa[i]; });
+            // DFS order: position p holds element arrangement[n-1-p]
+            const int e = live ? xa[n - 1 - t] : 0;
+            kd_emit_lane(ms, st, s, n, off, abase, t, e);
         }
         __syncthreads();
     }
@@ -714,7 +714,7 @@ __device__ static void kdf_sort(const double *c, int *perm, int n, int n2)
     }
 }
 
-// fast path; returns true when a tie straddles a split point
+// fast path; returns true (nothing written to out) on a split tie
 template <int NT>
 __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, int *flag)
 {
@@ -728,7 +728,6 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
         w.se[p] = n;
     }
     int cur[3] = {0, 0, 0};
-    bool any_tie = false;
     __syncthreads();
     for (int d = 0;; d++) {
         const int a = d % 3;
@@ -752,7 +751,7 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
         __syncthreads();
         const int splits = flag[0], tie = flag[1];
         __syncthreads();
-        if (tie) any_tie = true;
+        if (tie) return true;
         if (!splits) break;
         for (int b = 0; b < 3; b++) {
             if (b == a) continue;
@@ -786,7 +785,7 @@ __device__ static bool kd_fast(const KdFast &w, int n, int *out, int *partial, i
     const int *fin = w.perm + cur[0] * S;
     for (int p = tid; p < n; p += NT) out[p] = fin[p];
     __syncthreads();
-    return any_tie;
+    return false;
 }
 
 struct KdWork {
@@ -987,7 +986,7 @@ __device__ static void kd_load_global(const DevMols &ms, const DevSets &st, int 
     __syncthreads();
 }
 
-// fast path of the sets of `list`; sets with a split tie also go to `tied`
+// fast path of the sets of `list`; sets with a split tie go to `tied`
 __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
                                                       const int *list, const unsigned int *count, int *tied,
                                                       unsigned int *n_tied)
@@ -1016,17 +1015,13 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
         kd_load_global(ms, st, n, off, abase, w.c, (size_t)n2);
         // the arrangement after the work arrays: 61 * n2 + 4 * n <= 128 * n bytes
         int *order = (int *)((unsigned char *)(ib + 9 * n2) + n2);
-        const bool tie = kd_fast<KD_THREADS>(w, n, order, partial, flag);
-        kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
-        if (threadIdx.x == 0) {
-            st.tied[s] = tie ? 1 : 0;
-            if (tie) tied[atomicAdd(n_tied, 1u)] = s;
-        }
+        if (!kd_fast<KD_THREADS>(w, n, order, partial, flag)) kd_emit_order<KD_THREADS>(ms, st, s, n, off, abase, order);
+        else if (threadIdx.x == 0) tied[atomicAdd(n_tied, 1u)] = s;
         __syncthreads();
     }
 }
 
-// exact path of the sets of `list`: their positions in the reference's order
+// exact path of the sets of `list`
 __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, DevTpls tp, DevSets st,
                                                                   unsigned char *scr, const int *list,
                                                                   const unsigned int *count)
@@ -1056,12 +1051,10 @@ __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, De
         w.stride = (int)stride;
         kd_load_global(ms, st, n, off, abase, w.c, stride);
         int *order = ib + 9 * stride;   // the arrangement after the work arrays (64 * n bytes)
-        if (kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag)) {
-            for (int p = threadIdx.x; p < n; p += KD_EXACT_THREADS)
-                kd_write_xpos(st, n, off, p, [&](int i) { return order[i]; });
-        } else if (threadIdx.x == 0) {
+        if (kd_exact<KD_EXACT_THREADS>(w, n, order, partial, flag))
+            kd_emit_order<KD_EXACT_THREADS>(ms, st, s, n, off, abase, order);
+        else if (threadIdx.x == 0)
             atomicOr(st.kd_err, 1u);
-        }
         __syncthreads();
     }
 }

@@ -4,10 +4,8 @@
 //                  reference order (TessTemplate_candidates,
 //                  patches/TessTemplate.c.patch:173-235), two passes (count,
 //                  fill) with ballot compaction.
-//   k_kd*          (jam_kd.h) a kd leaf order of every candidate set (the
-//                  reference's QuickSelect order, or for sets with ties at a
-//                  split point a valid spatial order plus each entry's
-//                  position in the reference's), the set stored as SoA in
+//   k_kd*          (jam_kd.h) the leaf order of the reference's QuickSelect
+//                  kd-tree of every candidate set, the set stored as SoA in
 //                  that order plus one bounding box per 64 entries.
 //   k_pairs        thread per (template, molecule) pair: level order and the
 //                  number of level-0 partial matches.
@@ -214,7 +212,11 @@ __global__ void __launch_bounds__(256) k_pairs(DevTpls tp, DevSets st, DevPairs 
 // so sorting the complete matches by (pair, positions) recovers the
 // reference's enumeration order and each match's rank.
 
-// Level-0 rows: the candidates of every pair's level-0 list.
+// Level-0 rows: the candidates of every pair's level-0 list, enumerated in
+// the kd order of its set (consecutive rows then lie close in space, and the
+// join's candidate blocks stay in cache from row to row); the ranking of the
+// complete matches reads each row's list position back (kidx), so the order
+// rows are made in does not matter.
 // The buffers were sized before the sizes were known: when the candidate
 // sets, the items or the frontier do not fit, no row is written (the join then
 // has nothing to do) and the host grows the buffers and runs the query again.
@@ -227,12 +229,12 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         const int p = pr.item_pair[i];
-        const int pos0 = (int)(i - pr.item_base[p]) + pr.n1[p];
         const int off0 = pr.lvl[(size_t)p * JAM_MAX_LEVELS].off;
+        int k = (int)(i - pr.item_base[p]);
+        // a one-atom template steps past list entry 0: skip its kd position
+        if (pr.n1[p] && k >= st.l2k[off0]) k++;
         jn.out_pair[i] = p;
-        // the row holds kd entries at every level; level 0 walks its list in
-        // the reference order, so its entry is the kd position of list entry pos0
-        jn.out_ent[i * jn.stride] = off0 + st.l2k[off0 + pos0];
+        jn.out_ent[i * jn.stride] = off0 + k;
     }
 }
 
@@ -242,13 +244,15 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
 #ifndef JOIN_STAGE
 #define JOIN_STAGE 128
 #endif
-// lanes per team (one partial match per team) and the candidate loads a
-// team keeps in flight
+// waves per SIMD the join's register budget is set for (the LDS of
+// JOIN_WAVES-wave workgroups allows five)
+#ifndef JOIN_MIN_WAVES
+#define JOIN_MIN_WAVES 5
+#endif
+// lanes per team (one partial match per team; a team tests the JAM_SUB
+// candidates of one sub-block at a time)
 #ifndef JOIN_TW
 #define JOIN_TW 16
-#endif
-#ifndef JOIN_TEAM_LOADS
-#define JOIN_TEAM_LOADS 2
 #endif
 // most waves sharing one row's blocks when the frontier is small
 #ifndef JOIN_SPLIT
@@ -406,9 +410,68 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
     wave_sync();
     const int ignore = qp.ignore_chain;
     const long long bbase = (long long)offL / JAM_WAVE + setL;
+    const long long sbase = (long long)offL / JAM_SUB + setL;
     const bool last = (L == n - 1);
     const int nblk = live ? (cntL + JAM_WAVE - 1) / JAM_WAVE : 0;
     const int ngroups = (nblk + TW - 1) / TW;
+    // one candidate per lane: the annuli of every placed atom and check(),
+    // then the survivors staged (children, or complete matches at the last
+    // level)
+    auto test = [&](bool ok, int q) {
+        SetEnt c;
+        if (ok) c = st.kent[offL + q];
+        for (int j = 0; j < L; j++) {
+            // most candidates fail an early annulus: stop as soon as no lane
+            // of the wave is still in
+            if (ballot(ok) == 0ull) break;
+            if (ok) {
+                // _Annulus_po: squared distance summed x, y, z from 0.0
+                double tx = M.px[j] - c.x, ty = M.py[j] - c.y, tz = M.pz[j] - c.z;
+                double s2 = 0.0;
+                s2 += tx * tx;
+                s2 += ty * ty;
+                s2 += tz * tz;
+                if (s2 < M.r2lo[j] || s2 > M.r2hi[j]) ok = false;
+                // TessTemplate_check
+                const bool same_chain = M.pch[j] == c.ch;
+                const bool tsame_chain = M.tch[j] == tchL;
+                if (ignore == 0) {
+                    if (same_chain != tsame_chain) ok = false;
+                    if (!same_chain) continue;
+                }
+                const bool same_res = M.pres[j] == c.res;
+                const bool tsame_res = M.tres[j] == tresL;
+                if (same_res != tsame_res) ok = false;
+                if (ignore != 2 && same_res && !same_chain) ok = false;
+            }
+        }
+        const unsigned long long cm = ballot(ok && !last), dm = ballot(ok && last);
+        if (cm) {
+            const int cnt = __popcll(cm);
+            if (staged + cnt > JOIN_STAGE) join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap, jn.out_pair,
+                                                      jn.out_ent);
+            if (ok && !last) {
+                const int r = rank_below(cm);
+                S.sp[staged + r] = (int)i;
+                S.se[staged + r] = offL + q;
+            }
+            staged += cnt;
+        }
+        if (dm) {
+            const int cnt = __popcll(dm);
+            if (dstaged + cnt > JOIN_STAGE) join_write(jn, S.dp, S.de, dstaged, jn.n_done, jn.done_cap,
+                                                       jn.done_pair, jn.done_ent);
+            if (ok && last) {
+                const int r = rank_below(dm);
+                S.dp[dstaged + r] = (int)i;
+                S.de[dstaged + r] = offL + q;
+            }
+            dstaged += cnt;
+        }
+    };
+    static_assert(TW == JAM_SUB, "a team tests one sub-block of candidates at a time");
+    constexpr int SUBS = JAM_WAVE / JAM_SUB;   // sub-blocks per block
+    constexpr int BPR = TW / SUBS;             // blocks whose sub-blocks one round tests
     int seen = 0;   // in-box blocks met so far; this team takes every parts-th one
     for (int g = 0;; g++) {
         if (ballot(g < ngroups) == 0ull) break;
@@ -438,78 +501,31 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
             bm = keep;
         }
         for (;;) {
-            const bool has = bm != 0ull;
-            if (ballot(has) == 0ull) break;
-            int blk = 0;
-            if (has) {
-                blk = g * TW + first_bit(bm);
-                bm &= bm - 1;
+            if (ballot(bm != 0ull) == 0ull) break;
+            // lane tl tests the box of sub-block tl % SUBS of the
+            // (tl / SUBS)-th remaining in-box block
+            unsigned long long m = bm;
+            for (int k = 0; k < tl / SUBS; k++) m &= m - 1;
+            for (int k = 0; k < BPR; k++) bm &= bm - 1;
+            const int q0 = m ? (g * TW + first_bit(m)) * JAM_WAVE + (tl % SUBS) * JAM_SUB : 0;
+            bool sin = false;
+            if (m && q0 < cntL) {
+                const double *sb = st.sbox + 6 * (sbase + q0 / JAM_SUB);
+                sin = !(sb[0] > hi[0] || lo[0] > sb[3] || sb[1] > hi[1] || lo[1] > sb[4] || sb[2] > hi[2] ||
+                        lo[2] > sb[5]);
             }
-            // the block's 64 entries, TW at a time, JOIN_TEAM_LOADS loads in
-            // flight together
-            constexpr int NU = JAM_WAVE / TW;
-            constexpr int NL = JOIN_TEAM_LOADS < NU ? JOIN_TEAM_LOADS : NU;
-            SetEnt cand[NL];
-#pragma unroll
-            for (int u = 0; u < NU; u++) {
-                if (u % NL == 0) {
-#pragma unroll
-                    for (int v = 0; v < NL; v++) {
-                        const int q = blk * JAM_WAVE + (u + v) * TW + tl;
-                        if (has && u + v < NU && q < cntL) cand[v] = st.kent[offL + q];
-                    }
+            unsigned long long sm = (ballot(sin) >> (team * TW)) & TMASK;
+            // the in-box sub-blocks, one at a time: a candidate per lane
+            for (;;) {
+                const bool has = sm != 0ull;
+                if (ballot(has) == 0ull) break;
+                int x = 0;
+                if (has) {
+                    x = first_bit(sm);
+                    sm &= sm - 1;
                 }
-                const int q = blk * JAM_WAVE + u * TW + tl;
-                bool ok = has && q < cntL;
-                const SetEnt c = cand[u % NL];
-                for (int j = 0; j < L; j++) {
-                    // most in-box candidates fail an early annulus: stop as
-                    // soon as no lane of the wave is still in
-                    if (ballot(ok) == 0ull) break;
-                    if (ok) {
-                        // _Annulus_po: squared distance summed x, y, z from 0.0
-                        double tx = M.px[j] - c.x, ty = M.py[j] - c.y, tz = M.pz[j] - c.z;
-                        double s2 = 0.0;
-                        s2 += tx * tx;
-                        s2 += ty * ty;
-                        s2 += tz * tz;
-                        if (s2 < M.r2lo[j] || s2 > M.r2hi[j]) ok = false;
-                        // TessTemplate_check
-                        const bool same_chain = M.pch[j] == c.ch;
-                        const bool tsame_chain = M.tch[j] == tchL;
-                        if (ignore == 0) {
-                            if (same_chain != tsame_chain) ok = false;
-                            if (!same_chain) continue;
-                        }
-                        const bool same_res = M.pres[j] == c.res;
-                        const bool tsame_res = M.tres[j] == tresL;
-                        if (same_res != tsame_res) ok = false;
-                        if (ignore != 2 && same_res && !same_chain) ok = false;
-                    }
-                }
-                const unsigned long long cm = ballot(ok && !last), dm = ballot(ok && last);
-                if (cm) {
-                    const int cnt = __popcll(cm);
-                    if (staged + cnt > JOIN_STAGE) join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap,
-                                                              jn.out_pair, jn.out_ent);
-                    if (ok && !last) {
-                        const int r = rank_below(cm);
-                        S.sp[staged + r] = (int)i;
-                        S.se[staged + r] = offL + q;
-                    }
-                    staged += cnt;
-                }
-                if (dm) {
-                    const int cnt = __popcll(dm);
-                    if (dstaged + cnt > JOIN_STAGE) join_write(jn, S.dp, S.de, dstaged, jn.n_done, jn.done_cap,
-                                                               jn.done_pair, jn.done_ent);
-                    if (ok && last) {
-                        const int r = rank_below(dm);
-                        S.dp[dstaged + r] = (int)i;
-                        S.de[dstaged + r] = offL + q;
-                    }
-                    dstaged += cnt;
-                }
+                const int q = __shfl(q0, team * TW + x, JAM_WAVE) + tl;
+                test(has && q < cntL, q);
             }
         }
     }
@@ -520,7 +536,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
 // device); the output frontier's counter is also the next level's input
 // size, so levels run back to back without the host.
 template <int MAXN, int TW>
-__global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE) k_join(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
+__global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
                                                               DevJoin jn, QueryParams qp)
 {
     constexpr int NTEAM = JAM_WAVE / TW;
@@ -579,8 +595,8 @@ __global__ void __launch_bounds__(256) k_iota(int *v, long long n)
 }
 
 // Sort key of complete match i for levels [l0, l0 + per): the positions of
-// the match's atoms in their sets (list position at level 0, position in the
-// reference's kd DFS order above), `bits` bits each, the earliest level most significant; levels past
+// the match's atoms in their sets (list position at level 0, kd position
+// above), `bits` bits each, the earliest level most significant; levels past
 // the template's size count as 0 (all matches of a pair have the same size).
 __device__ static inline unsigned long long level_key(const DevSets &st, const DevTpls &tp, const DevPairs &pr,
                                                       const DevJoin &jn, int i, int l0, int per, int bits)
@@ -596,10 +612,7 @@ __device__ static inline unsigned long long level_key(const DevSets &st, const D
         if (l == 0) {
             pos = (unsigned long long)st.kidx[row[0]];
         } else if (l < nt) {
-            // the position in the reference's kd order: the join's own
-            // position unless the set has ties at a split point
-            const PairLevel h = pr.lvl[(size_t)p * JAM_MAX_LEVELS + l];
-            pos = (unsigned long long)(st.tied[h.set] ? st.xpos[row[l]] : row[l] - h.off);
+            pos = (unsigned long long)(row[l] - pr.lvl[(size_t)p * JAM_MAX_LEVELS + l].off);
         }
         k = (k << bits) | pos;
     }
@@ -841,6 +854,42 @@ __global__ void __launch_bounds__(256) k_super(DevMols ms, DevTpls tp, DevSets s
 __global__ void k_event_total(const int *sel, const int *pos, long long n, unsigned long long *count)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *count = n > 0 ? (unsigned long long)(pos[n - 1] + sel[n - 1]) : 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// hit-set digests (measurement)
+// ---------------------------------------------------------------------------
+
+// SplitMix64's finaliser (Steele, Lea & Flood, "Fast splittable pseudorandom
+// number generators", 2014): a 64-bit mix whose sum over a set of hits is an
+// order-independent digest of that set.
+__host__ __device__ static inline unsigned long long mix64(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Thread per event: hits (flag 1) add one to their pair's count and
+// mix64-chain(template << 32 ^ molecule, atoms...) to its digest; the
+// molecule is numbered from mol_base. Rows end at the first -1 (the
+// template's size).
+__global__ void __launch_bounds__(256) k_pair_digest(const unsigned char *base, long long n, int stride,
+                                                     int n_templates, int mol_base, unsigned long long *counts,
+                                                     unsigned long long *digests)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const EventLayout L = event_layout((int64_t)n, stride);
+    if (((const int *)(base + L.flag))[e] != 1) return;
+    const int p = ((const int *)(base + L.pair))[e];
+    const unsigned int t = (unsigned int)(p % n_templates), mi = (unsigned int)(p / n_templates + mol_base);
+    unsigned long long h = mix64(((unsigned long long)t << 32) ^ (unsigned long long)mi);
+    const int *at = (const int *)(base + L.atoms) + (size_t)e * stride;
+    for (int k = 0; k < stride && at[k] >= 0; k++) h = mix64(h ^ (unsigned long long)(unsigned int)at[k]);
+    atomicAdd(&counts[p], 1ull);
+    atomicAdd(&digests[p], h);
 }
 
 // Standalone batched superposition: sets of k points, a/b as [set][point][xyz].
