@@ -181,6 +181,24 @@ int32_t jam_superpose_batch(jam_context *ctx, const double *a, const double *b, 
                             int64_t n_sets, double *rmsd, double *rotation9, double *c0_3,
                             double *c1_3);
 
+/* Size of the per-device scratch array of jam_superpose_batch_noalloc. */
+#define JAM_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* jam_superpose_batch without any device allocation, for callers that cannot
+ * allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Same arguments, kernels and
+ * results as jam_superpose_batch, but inputs and outputs are staged in a
+ * statically sized device array (JAM_NOALLOC_SCRATCH_BYTES per device, held
+ * until the process ends) instead of context buffers: batches needing more
+ * than 8 * (6k + 16) bytes per set in all (about 32 MiB) return
+ * JAM_ETOOBIG. The array is shared by every context of a device and keeps
+ * the last call's contents (nothing in it needs to start at zero); calls on
+ * one device from one process run one at a time (a lock), calls from
+ * several processes on one device would race. */
+int32_t jam_superpose_batch_noalloc(jam_context *ctx, const double *a, const double *b, int32_t k,
+                                    int64_t n_sets, double *rmsd, double *rotation9, double *c0_3,
+                                    double *c1_3);
+
 /* ---- standalone kd-tree leaf order ----------------------------------- */
 
 /* Replaces KdTree_create / KdTreeNode_create (patches/KdTree.c.patch:130-185,

@@ -31,7 +31,8 @@ _SYMBOLS = (
     "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
     "jam_result_timings", "jam_result_pair_digests", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_rmsd_forward",
-    "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch", "jam_kd_order", "jam_last_error",
+    "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch",
+    "jam_superpose_batch_noalloc", "jam_kd_order", "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -159,6 +160,8 @@ def lib():
         L.jam_result_free.restype = None
         L.jam_superpose_batch.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, vp]
         L.jam_superpose_batch.restype = i32
+        L.jam_superpose_batch_noalloc.argtypes = [vp, vp, vp, i32, i64, vp, vp, vp, vp]
+        L.jam_superpose_batch_noalloc.restype = i32
         L.jam_result_pair_digests.argtypes = [vp, i32, i32, vp, vp]
         L.jam_result_pair_digests.restype = i32
         L.jam_kd_order.argtypes = [vp, vp, vp, i32, vp]

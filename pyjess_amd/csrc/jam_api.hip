@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <string>
@@ -88,6 +89,24 @@ uint32_t pack4(const char *p)
            ((uint32_t)(uint8_t)p[3] << 24);
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device scratch of the allocation-free entry point
+// ---------------------------------------------------------------------------
+
+// jam_superpose_batch_noalloc stages its inputs and outputs here instead of in
+// buffers the library allocates: one copy of the array per device, held
+// until the process ends. In host code `scratch` only names the symbol; its
+// address on each device comes from hipGetSymbolAddress at context creation
+// (g_scratch), and only that address is ever handed to the device.
+static __device__ __align__(256) unsigned char scratch[JAM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int JAM_MAX_DEVICES = 64;
+void *g_scratch[JAM_MAX_DEVICES] = {};
+std::mutex g_scratch_init;                    // guards g_scratch
+std::mutex g_scratch_busy[JAM_MAX_DEVICES];   // one call at a time uses a device's array
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -283,6 +302,15 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
     if (device < 0 || device >= n) return fail(JAM_EINVAL, "device ordinal out of range");
     if (!is_gfx950(device)) return fail(JAM_EDEVICE, "device is not a gfx950 (MI355X) GPU");
     HIP_TRY(hipSetDevice(device));
+    if (device < JAM_MAX_DEVICES) {
+        // the device address of this device's scratch array, once per device
+        std::lock_guard<std::mutex> g(g_scratch_init);
+        if (!g_scratch[device]) {
+            void *p = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_scratch[device] = p;
+        }
+    }
     std::unique_ptr<jam_context> c(new jam_context());
     c->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1439,6 +1467,46 @@ extern "C" int32_t jam_superpose_batch(jam_context *c, const double *a, const do
                                   c->srot.as<double>(), c->sc0.as<double>(), c->sc1.as<double>());
     if (rc != JAM_OK) return rc;
     return superpose_fetch(s, n_sets, c->srmsd.ptr, c->srot.ptr, c->sc0.ptr, c->sc1.ptr, rmsd, rot, c0, c1);
+}
+
+extern "C" int32_t jam_superpose_batch_noalloc(jam_context *c, const double *a, const double *b, int32_t k,
+                                               int64_t n_sets, double *rmsd, double *rot, double *c0, double *c1)
+{
+    if (!c || !a || !b || !rmsd || !rot || !c0 || !c1) return fail(JAM_EINVAL, "NULL argument");
+    if (k < 1 || n_sets < 0) return fail(JAM_EINVAL, "invalid sizes");
+    if (n_sets == 0) return JAM_OK;
+    // the array, cut into 256-byte aligned pieces: a, b, rmsd, rotations, centroids
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t per_set = sizeof(double) * (6 * (size_t)k + 16);
+    if ((size_t)n_sets > JAM_NOALLOC_SCRATCH_BYTES / per_set)
+        return fail(JAM_ETOOBIG, "batch larger than the scratch of jam_superpose_batch_noalloc");
+    const size_t pts = (size_t)n_sets * k * 3;
+    const size_t o_a = 0, o_b = o_a + up(sizeof(double) * pts), o_r = o_b + up(sizeof(double) * pts);
+    const size_t o_m = o_r + up(sizeof(double) * n_sets), o_c0 = o_m + up(sizeof(double) * 9 * n_sets);
+    const size_t o_c1 = o_c0 + up(sizeof(double) * 3 * n_sets), end = o_c1 + up(sizeof(double) * 3 * n_sets);
+    if (end > JAM_NOALLOC_SCRATCH_BYTES)
+        return fail(JAM_ETOOBIG, "batch larger than the scratch of jam_superpose_batch_noalloc");
+    if (c->device < 0 || c->device >= JAM_MAX_DEVICES) return fail(JAM_EDEVICE, "device ordinal has no scratch");
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_init);
+        base = g_scratch[c->device];
+    }
+    if (!base) return fail(JAM_EDEVICE, "the device's scratch address is unknown");
+    std::lock_guard<std::mutex> busy(g_scratch_busy[c->device]);
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    unsigned char *w = static_cast<unsigned char *>(base);
+    double *da = (double *)(w + o_a), *db = (double *)(w + o_b), *dr = (double *)(w + o_r);
+    double *dm = (double *)(w + o_m), *d0 = (double *)(w + o_c0), *d1 = (double *)(w + o_c1);
+    HIP_TRY(hipMemcpyAsync(da, a, sizeof(double) * pts, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(db, b, sizeof(double) * pts, hipMemcpyHostToDevice, s));
+    int32_t rc = superpose_launch(s, da, db, k, n_sets, dr, dm, d0, d1);
+    if (rc != JAM_OK) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    return superpose_fetch(s, n_sets, dr, dm, d0, d1, rmsd, rot, c0, c1);
 }
 
 // ---------------------------------------------------------------------------
