@@ -309,8 +309,10 @@ template <int MAXN>
 struct JoinTeamLds {
     double px[MAXN], py[MAXN], pz[MAXN];
     double r2lo[MAXN], r2hi[MAXN];
-    int pch[MAXN], pres[MAXN];
-    int tch[MAXN], tres[MAXN];
+    // per placed atom: chainID2 and resSeq of the molecule atom, and whether
+    // its template atom has the new template atom's chainID2 (bit 0) and
+    // resSeq (bit 1)
+    int4 pk[MAXN];
 };
 
 template <int MAXN, int TW>
@@ -351,13 +353,15 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         }
     }
     const bool live = active && L < n;
-    int kL = 0, setL = 0, cntL = 0, offL = 0;
+    int kL = 0, setL = 0, cntL = 0, offL = 0, tchL = 0, tresL = 0;
     if (live) {
         const PairLevel hL = lv[L];
         kL = hL.k;
         setL = hL.set;
         cntL = hL.cnt;
         offL = hL.off;
+        tchL = tp.chain[tbase + kL];
+        tresL = tp.resseq[tbase + kL];
     }
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
 #pragma unroll
@@ -371,10 +375,8 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         M.px[j] = en.x;
         M.py[j] = en.y;
         M.pz[j] = en.z;
-        M.pch[j] = en.ch;
-        M.pres[j] = en.res;
-        M.tch[j] = tp.chain[tbase + k];
-        M.tres[j] = tp.resseq[tbase + k];
+        M.pk[j] = make_int4(en.ch, en.res,
+                            (tp.chain[tbase + k] == tchL ? 1 : 0) | (tp.resseq[tbase + k] == tresL ? 2 : 0), 0);
         // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
         // Annulus_reuse patches/Annulus.c.patch:107-140
         const double d = tp.dist[tp.dist_off[t] + k * n + kL];
@@ -406,7 +408,6 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
             hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
         }
     }
-    const int tchL = live ? tp.chain[tbase + kL] : 0, tresL = live ? tp.resseq[tbase + kL] : 0;
     wave_sync();
     const int ignore = qp.ignore_chain;
     const long long bbase = (long long)offL / JAM_WAVE + setL;
@@ -414,36 +415,42 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
     const bool last = (L == n - 1);
     const int nblk = live ? (cntL + JAM_WAVE - 1) / JAM_WAVE : 0;
     const int ngroups = (nblk + TW - 1) / TW;
+    // a box (lo xyz, hi xyz: three 16-byte loads) meets the join box
+    auto meets = [&](const double *box) {
+        const double2 *b2 = reinterpret_cast<const double2 *>(box);
+        const double2 u = b2[0], v = b2[1], w = b2[2];   // (lo0, lo1), (lo2, hi0), (hi1, hi2)
+        return !((u.x > hi[0]) | (lo[0] > v.y) | (u.y > hi[1]) | (lo[1] > w.x) | (v.x > hi[2]) | (lo[2] > w.y));
+    };
     // one candidate per lane: the annuli of every placed atom and check(),
     // then the survivors staged (children, or complete matches at the last
-    // level)
+    // level). Branch-free: every lane evaluates every test.
     auto test = [&](bool ok, int q) {
-        SetEnt c;
+        SetEnt c{0.0, 0.0, 0.0, 0, 0};
         if (ok) c = st.kent[offL + q];
         for (int j = 0; j < L; j++) {
             // most candidates fail an early annulus: stop as soon as no lane
             // of the wave is still in
             if (ballot(ok) == 0ull) break;
-            if (ok) {
-                // _Annulus_po: squared distance summed x, y, z from 0.0
-                double tx = M.px[j] - c.x, ty = M.py[j] - c.y, tz = M.pz[j] - c.z;
-                double s2 = 0.0;
-                s2 += tx * tx;
-                s2 += ty * ty;
-                s2 += tz * tz;
-                if (s2 < M.r2lo[j] || s2 > M.r2hi[j]) ok = false;
-                // TessTemplate_check
-                const bool same_chain = M.pch[j] == c.ch;
-                const bool tsame_chain = M.tch[j] == tchL;
-                if (ignore == 0) {
-                    if (same_chain != tsame_chain) ok = false;
-                    if (!same_chain) continue;
-                }
-                const bool same_res = M.pres[j] == c.res;
-                const bool tsame_res = M.tres[j] == tresL;
-                if (same_res != tsame_res) ok = false;
-                if (ignore != 2 && same_res && !same_chain) ok = false;
-            }
+            // _Annulus_po: squared distance summed x, y, z from 0.0; inside
+            // unless below the inner or above the outer squared radius
+            const double tx = M.px[j] - c.x, ty = M.py[j] - c.y, tz = M.pz[j] - c.z;
+            double s2 = 0.0;
+            s2 += tx * tx;
+            s2 += ty * ty;
+            s2 += tz * tz;
+            const bool ann = !(s2 < M.r2lo[j]) & !(s2 > M.r2hi[j]);
+            // TessTemplate_check (patches/TessTemplate.c.patch:101-156): chain
+            // and residue equality must match the template's; with
+            // ignore_chain none the residue test applies within a chain only,
+            // with "residues" a residue of another chain does not match
+            const int4 k4 = M.pk[j];
+            const bool sc = k4.x == c.ch, sr = k4.y == c.res;
+            const bool tsc = (k4.z & 1) != 0, tsr = (k4.z & 2) != 0;
+            const bool res_ok = sr == tsr;
+            const bool chk = ignore == 0   ? ((sc == tsc) & (!sc | res_ok))
+                             : ignore == 1 ? (res_ok & !(sr & !sc))
+                                           : res_ok;
+            ok = ok & ann & chk;
         }
         const unsigned long long cm = ballot(ok && !last), dm = ballot(ok && last);
         if (cm) {
@@ -479,15 +486,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         // small sets: the test is a prefilter only)
         bool inb = false;
         const int b = g * TW + tl;
-        if (g < ngroups && b < nblk) {
-            if (nblk <= JOIN_SMALL) {
-                inb = true;
-            } else {
-                const double *bb = st.bbox + 6 * (bbase + b);
-                inb = !(bb[0] > hi[0] || lo[0] > bb[3] || bb[1] > hi[1] || lo[1] > bb[4] || bb[2] > hi[2] ||
-                        lo[2] > bb[5]);
-            }
-        }
+        if (g < ngroups && b < nblk) inb = nblk <= JOIN_SMALL || meets(st.bbox + 6 * (bbase + b));
         unsigned long long bm = (ballot(inb) >> (team * TW)) & TMASK;
         if (parts > 1) {
             // keep the in-box blocks whose running index is part mod parts
@@ -508,12 +507,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
             for (int k = 0; k < tl / SUBS; k++) m &= m - 1;
             for (int k = 0; k < BPR; k++) bm &= bm - 1;
             const int q0 = m ? (g * TW + first_bit(m)) * JAM_WAVE + (tl % SUBS) * JAM_SUB : 0;
-            bool sin = false;
-            if (m && q0 < cntL) {
-                const double *sb = st.sbox + 6 * (sbase + q0 / JAM_SUB);
-                sin = !(sb[0] > hi[0] || lo[0] > sb[3] || sb[1] > hi[1] || lo[1] > sb[4] || sb[2] > hi[2] ||
-                        lo[2] > sb[5]);
-            }
+            const bool sin = m && q0 < cntL && meets(st.sbox + 6 * (sbase + q0 / JAM_SUB));
             unsigned long long sm = (ballot(sin) >> (team * TW)) & TMASK;
             // the in-box sub-blocks, one at a time: a candidate per lane
             for (;;) {
@@ -535,9 +529,18 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
 // Waves loop over the input rows (the grid is what stays resident on the
 // device); the output frontier's counter is also the next level's input
 // size, so levels run back to back without the host.
+// Workgroups are dealt to the XCDs round-robin (workgroup b runs on XCD
+// b % JOIN_XCDS), and each XCD has its own L2: the rows are cut into chunks
+// of one XCD's team slots, chunk c going to XCD c % JOIN_XCDS, so that
+// neighbouring rows -- mostly the same pair, the same candidate sets --
+// share one L2.
+#ifndef JOIN_XCDS
+#define JOIN_XCDS 8
+#endif
 template <int MAXN, int TW>
-__global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(DevMols ms, DevTpls tp, DevSets st, DevPairs pr,
-                                                              DevJoin jn, QueryParams qp)
+__global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(DevMols ms, DevTpls tp, DevSets st,
+                                                                              DevPairs pr, DevJoin jn,
+                                                                              QueryParams qp)
 {
     constexpr int NTEAM = JAM_WAVE / TW;
     __shared__ JoinWaveLds<MAXN, TW> SW[JOIN_WAVES];
@@ -551,6 +554,12 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
     // in-box blocks are shared by `parts` teams
     const int parts = (n_in > 0 && n_in < slots) ? (int)min((long long)JOIN_SPLIT, slots / n_in) : 1;
     const long long items = n_in * parts;
+    const int xcds = (gridDim.x % JOIN_XCDS == 0) ? JOIN_XCDS : 1;
+    const int xcd = (int)(blockIdx.x % xcds);
+    const long long per = slots / xcds;   // items of one chunk
+    const long long tslot = ((long long)(blockIdx.x / xcds) * JOIN_WAVES + w) * NTEAM;
+    // the first item of this wave's teams in round i
+    auto round_base = [&](long long i) { return (i * xcds + xcd) * per + tslot; };
     // the pair and this lane's entries of the team's next row are loaded
     // while the current one is processed
     constexpr int NE = (MAXN + TW - 1) / TW;
@@ -569,16 +578,18 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
             if (j < L) e_next[jj] = jn.in_ent[r * jn.stride + j];
         }
     };
-    long long base = ((long long)blockIdx.x * JOIN_WAVES + w) * NTEAM;
+    long long base = round_base(0);
     if (base < items) fetch(base + team);
-    for (; base < items; base += slots) {
+    for (long long i = 0; base < items; i++) {
         const long long it = base + team;
         const int p = p_next;
         int e[NE];
 #pragma unroll
         for (int jj = 0; jj < NE; jj++) e[jj] = e_next[jj];
-        if (base + slots < items) fetch(base + slots + team);
+        const long long next = round_base(i + 1);
+        if (next < items) fetch(next + team);
         join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, p, e, staged, dstaged);
+        base = next;
     }
     join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
     join_write(jn, SW[w].dp, SW[w].de, dstaged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
