@@ -80,40 +80,60 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+_native_lib = None
+NATIVE_PATH = os.path.join(HERE, "_native", "libjess_oracle.so")
+
+
+def native_lib():
+    """The same oracle compiled on this host for this host (-march=native,
+    still without contraction: the same results), for timing the CPU
+    baseline. Rebuilt once per process, since the library may have been built
+    on another machine."""
+    global _native_lib
+    if _native_lib is None:
+        subprocess.run(["make", "-s", "-C", HERE, "-B", "native"], check=True)
+        _native_lib = _declare(ctypes.CDLL(NATIVE_PATH))
+    return _native_lib
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.POINTER
-        L.jo_query.argtypes = [P(JoMol), P(JoTpls), P(JoParams), ctypes.c_void_p, ctypes.c_int32,
-                               P(ctypes.c_void_p)]
-        L.jo_query.restype = ctypes.c_int32
-        L.jo_stream_count.argtypes = [ctypes.c_void_p]
-        L.jo_stream_count.restype = ctypes.c_int64
-        L.jo_stream_atom_total.argtypes = [ctypes.c_void_p]
-        L.jo_stream_atom_total.restype = ctypes.c_int64
-        L.jo_stream_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
-        L.jo_stream_copy.restype = None
-        L.jo_stream_free.argtypes = [ctypes.c_void_p]
-        L.jo_stream_free.restype = None
-        L.jo_superpose.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 4
-        L.jo_superpose.restype = None
-        L.jo_bruteforce.argtypes = [P(JoMol), P(JoTpls), ctypes.c_int32, P(JoParams), ctypes.c_void_p,
-                                    ctypes.c_int64]
-        L.jo_bruteforce.restype = ctypes.c_int64
-        L.jo_candidates.argtypes = [P(JoMol), P(JoTpls), ctypes.c_int32, ctypes.c_int32,
-                                    ctypes.c_void_p, ctypes.c_int32]
-        L.jo_candidates.restype = ctypes.c_int32
-        L.jo_kd_order.argtypes = [P(JoMol), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
-        L.jo_kd_order.restype = ctypes.c_int32
-        L.jo_batch.argtypes = [P(JoMol), ctypes.c_int32, P(JoTpls), P(JoParams), ctypes.c_double,
-                               ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        L.jo_batch.restype = ctypes.c_int32
-        L.jo_work.argtypes = [P(JoMol), P(JoTpls), P(JoParams), ctypes.c_void_p]
-        L.jo_work.restype = ctypes.c_int32
-        _lib = L
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _declare(L):
+    """Argument and result types of the exported functions."""
+    P = ctypes.POINTER
+    L.jo_query.argtypes = [P(JoMol), P(JoTpls), P(JoParams), ctypes.c_void_p, ctypes.c_int32,
+                           P(ctypes.c_void_p)]
+    L.jo_query.restype = ctypes.c_int32
+    L.jo_stream_count.argtypes = [ctypes.c_void_p]
+    L.jo_stream_count.restype = ctypes.c_int64
+    L.jo_stream_atom_total.argtypes = [ctypes.c_void_p]
+    L.jo_stream_atom_total.restype = ctypes.c_int64
+    L.jo_stream_copy.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 7
+    L.jo_stream_copy.restype = None
+    L.jo_stream_free.argtypes = [ctypes.c_void_p]
+    L.jo_stream_free.restype = None
+    L.jo_superpose.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32] + [ctypes.c_void_p] * 4
+    L.jo_superpose.restype = None
+    L.jo_bruteforce.argtypes = [P(JoMol), P(JoTpls), ctypes.c_int32, P(JoParams), ctypes.c_void_p,
+                                ctypes.c_int64]
+    L.jo_bruteforce.restype = ctypes.c_int64
+    L.jo_candidates.argtypes = [P(JoMol), P(JoTpls), ctypes.c_int32, ctypes.c_int32,
+                                ctypes.c_void_p, ctypes.c_int32]
+    L.jo_candidates.restype = ctypes.c_int32
+    L.jo_kd_order.argtypes = [P(JoMol), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+    L.jo_kd_order.restype = ctypes.c_int32
+    L.jo_batch.argtypes = [P(JoMol), ctypes.c_int32, P(JoTpls), P(JoParams), ctypes.c_double,
+                           ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.jo_batch.restype = ctypes.c_int32
+    L.jo_work.argtypes = [P(JoMol), P(JoTpls), P(JoParams), ctypes.c_void_p]
+    L.jo_work.restype = ctypes.c_int32
+    return L
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -348,12 +368,13 @@ def kd_order(molecule, set_atoms):
 
 
 def batch(mol_views: Sequence[MolView], templates, rmsd_threshold, distance_cutoff,
-          max_dynamic_distance, ignore_chain=None, reorder=True, n_threads=1):
-    """Every (template, molecule) pair on ``n_threads`` host threads.
+          max_dynamic_distance, ignore_chain=None, reorder=True, n_threads=1, native=False):
+    """Every (template, molecule) pair on ``n_threads`` host threads (with
+    ``native``, the build for this host).
 
     Returns per-pair arrays (molecule-major) of complete-match counts, hit
     counts and order-independent hit digests."""
-    L = lib()
+    L = native_lib() if native else lib()
     tv = templates if isinstance(templates, TplView) else TplView(templates)
     arr = (JoMol * len(mol_views))(*[mv.c for mv in mol_views])
     npairs = len(mol_views) * tv.n_templates

@@ -49,3 +49,18 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "--gpus 2 is not supported" in r.stderr
     assert r.stdout.strip() == ""
+
+
+def test_native_oracle_build_gives_the_same_results():
+    """The host-tuned build used to time the CPU baseline (-march=native,
+    contraction still off) counts and digests the same hits as the portable
+    build used as the checker."""
+    mols, tpls, kw = synthetic.workload("batch-4k", seed=3, scale=2 / 4096)
+    q = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    mv = [oracle.MolView.of(m) for m in mols]
+    tv = oracle.TplView(tpls)
+    a = oracle.batch(mv, tv, *q, n_threads=2)
+    b = oracle.batch(mv, tv, *q, n_threads=2, native=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert a[1].sum() > 0

@@ -88,3 +88,37 @@ def test_fresh_context_grows_and_reruns():
         assert np.array_equal(res.pair_matches, cm)
         assert np.array_equal(gpu_hc, hc)
         assert np.array_equal(gpu_dg, dg.astype(np.uint64))
+
+
+def test_device_pair_digests_equal_the_oracle():
+    """jam_result_pair_digests (per-pair hit counts and digests computed on the
+    device from the events in HBM, as bench.py checks the library workload)
+    against the oracle's, with the molecules numbered from an offset."""
+    device()
+    import ctypes
+
+    from oracle import oracle
+
+    mols, tpls, kw = synthetic.workload("batch-4k", seed=9, scale=12 / 4096)
+    q = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    T = len(tpls)
+    cm, hc, dg = oracle.batch([oracle.MolView.of(m) for m in mols], oracle.TplView(tpls), *q, n_threads=4)
+    assert hc.sum() > 0
+    # the last 7 molecules as a batch of their own, numbered from 5
+    ctx = _native.context()
+    tset = _native.TemplateSet(ctx, tpls)
+    mset = _native.MoleculeSet(ctx, mols[5:])
+    L = _native.lib()
+    p = _native.Params(*q, 0, 1)
+    h = ctypes.c_void_p()
+    with ctx.lock:
+        _native._check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h)))
+        try:
+            n = L.jam_result_n_pairs(h)
+            gh = np.zeros(n, dtype=np.int64)
+            gd = np.zeros(n, dtype=np.uint64)
+            _native._check(L.jam_result_pair_digests(h, 5, T, _native._ptr(gh), _native._ptr(gd)))
+        finally:
+            L.jam_result_free(h)
+    assert np.array_equal(gh, hc[5 * T:])
+    assert np.array_equal(gd, dg[5 * T:].astype(np.uint64))
