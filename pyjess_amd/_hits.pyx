@@ -1,6 +1,6 @@
 # cython: language_level=3, boundscheck=False, wraparound=False
-"""`Hit`, the hit list builder and the iteration of `Query`, compiled
-(reference ``src/pyjess/_jess.pyx:2090-2621``).
+"""`Hit`, the hit lists (eager for `Query`, lazy for `Jess.query_batch`) and
+the iteration of `Query`, compiled (reference ``src/pyjess/_jess.pyx:2090-2621``).
 
 A query returns thousands of hits; each is a small object that refers to a
 row of its query's result arrays (rotation, centroids, matched atoms), read
@@ -208,6 +208,138 @@ def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const i
         h._molecule = molecules[mi]
         lst.append(h)
     return nan
+
+
+cdef class HitList:
+    """The hits of one molecule of `Jess.query_batch`, in pyjess order: a
+    read-only sequence over rows of the batch's shared result arrays. A `Hit`
+    object is made when an element is read (a library-scale batch has tens of
+    millions of hits, most of which a caller never touches); reading the same
+    element twice gives two equal-valued `Hit` objects."""
+
+    cdef object _block
+    cdef const double[:] _rmsd
+    cdef const int[:] _pair
+    cdef const long long[:] _events
+    cdef int _n_templates
+    cdef list _templates
+    cdef object _molecule
+
+    def __init__(self, block, const double[:] rmsd, const int[:] pair, const long long[:] events,
+                 int n_templates, list templates, molecule):
+        self._block = block
+        self._rmsd = rmsd
+        self._pair = pair
+        self._events = events
+        self._n_templates = n_templates
+        self._templates = templates
+        self._molecule = molecule
+
+    cdef Hit _hit(self, Py_ssize_t k):
+        cdef Py_ssize_t e = self._events[k]
+        cdef Hit h = Hit.__new__(Hit)
+        h.rmsd = self._rmsd[e]
+        h._block = self._block
+        h._row = e
+        h._template = self._templates[self._pair[e] % self._n_templates]
+        h._molecule = self._molecule
+        return h
+
+    def __len__(self):
+        return self._events.shape[0]
+
+    def __getitem__(self, index):
+        cdef Py_ssize_t n = self._events.shape[0], k
+        if isinstance(index, slice):
+            return [self._hit(k) for k in range(*index.indices(n))]
+        k = index
+        if k < 0:
+            k += n
+        if k < 0 or k >= n:
+            raise IndexError("hit index out of range")
+        return self._hit(k)
+
+    def __iter__(self):
+        return _HitListIter(self)
+
+    def __bool__(self):
+        return self._events.shape[0] > 0
+
+    def __eq__(self, other):
+        try:
+            n = len(other)
+        except TypeError:
+            return NotImplemented
+        if n != len(self):
+            return False
+        cdef HitList o
+        cdef Py_ssize_t k
+        if isinstance(other, HitList):
+            # the same rows of the same result arrays
+            o = <HitList>other
+            if o._block is not self._block:
+                return False
+            for k in range(n):
+                if o._events[k] != self._events[k]:
+                    return False
+            return True
+        return all(a == b for a, b in zip(self, other))
+
+    def __repr__(self):
+        return f"<HitList of {len(self)} hits>"
+
+
+cdef class _HitListIter:
+    cdef HitList _list
+    cdef Py_ssize_t _k
+
+    def __init__(self, HitList lst):
+        self._list = lst
+        self._k = 0
+
+    def __iter__(self):
+        return self
+
+    def __next__(self):
+        if self._k >= self._list._events.shape[0]:
+            raise StopIteration
+        self._k += 1
+        return self._list._hit(self._k - 1)
+
+
+def split_events(const int[:] pair, const int[:] flag, chosen, int n_templates, int n_mols):
+    """The chosen events (all of them when ``chosen`` is None) cut by molecule
+    for `HitList`: the hit events in order, each molecule's range of them
+    (``bounds[m]:bounds[m+1]``) and its number of NaN-rotation events, which
+    are left out. One pass; events arrive grouped by molecule."""
+    cdef Py_ssize_t n = pair.shape[0] if chosen is None else len(chosen)
+    cdef const long long[:] ch
+    if chosen is not None:
+        ch = chosen
+    ev = np.empty(max(n, 1), dtype=np.int64)
+    bounds = np.zeros(n_mols + 1, dtype=np.int64)
+    nan = np.zeros(n_mols, dtype=np.int64)
+    cdef long long[:] o = ev
+    cdef long long[:] b = bounds
+    cdef long long[:] nn = nan
+    cdef Py_ssize_t i, w = 0
+    cdef long long e
+    cdef int m, cur = 0
+    for i in range(n):
+        e = i if chosen is None else ch[i]
+        m = pair[e] // n_templates
+        while cur < m:
+            cur += 1
+            b[cur] = w
+        if flag[e] == 2:
+            nn[m] += 1
+            continue
+        o[w] = e
+        w += 1
+    while cur < n_mols:
+        cur += 1
+        b[cur] = w
+    return ev[:w], bounds, nan
 
 
 def select_events(const int[:] pair, const long long[:] rank, const int[:] flag, const double[:] rmsd,

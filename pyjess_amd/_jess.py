@@ -33,7 +33,11 @@ def _warn_nan():
 
 
 from ._hits import Block as _Block, Hit, HitIterator as _HitIterator, build_hits as _build_hits  # noqa: E402
-from ._hits import select_events as _select_events  # noqa: E402
+from ._hits import HitList, select_events as _select_events, split_events as _split_events  # noqa: E402
+
+import collections.abc as _abc  # noqa: E402
+
+_abc.Sequence.register(HitList)
 
 
 class Query(_HitIterator):
@@ -119,12 +123,22 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         mc = -1
     if not best_match and mc == -1:
         # no policy can drop an event: every event is reported, in order
-        chosen = np.arange(n, dtype=np.int64)
+        chosen = None if not keep_none else np.arange(n, dtype=np.int64)
     else:
         chosen = _select_events(pair, np.ascontiguousarray(res.rank, dtype=np.int64),
                                 np.ascontiguousarray(res.flag, dtype=np.int32),
                                 np.ascontiguousarray(res.rmsd, dtype=np.float64), best_match, mc)
     block = _Block(res.rotation, res.c0, res.c1, res.atoms)
+    if not keep_none:
+        # lazy lists: the chosen hits (NaN rotations dropped and counted), cut
+        # by molecule (events are grouped by molecule, in order)
+        hit_ev, bounds, nan = _split_events(pair, np.ascontiguousarray(res.flag, dtype=np.int32), chosen, T,
+                                            len(molecules))
+        rmsd = np.ascontiguousarray(res.rmsd, dtype=np.float64)
+        templates = list(jess._templates)
+        out = [HitList(block, rmsd, pair, hit_ev[bounds[m]:bounds[m + 1]], T, templates, mol)
+               for m, mol in enumerate(molecules)]
+        return out, nan
     nan = _build_hits(out, chosen, (pair // T).astype(np.int32), (pair % T).astype(np.int32),
                       np.ascontiguousarray(res.flag, dtype=np.int32),
                       np.ascontiguousarray(res.rmsd, dtype=np.float64), block, list(jess._templates),
@@ -204,12 +218,14 @@ class Jess:
 
     def query_batch(self, molecules: Sequence[Molecule], rmsd_threshold: float, distance_cutoff: float,
                     max_dynamic_distance: float, *, max_candidates=None, ignore_chain=None,
-                    best_match: bool = False, reorder: bool = True) -> List[List[Hit]]:
+                    best_match: bool = False, reorder: bool = True) -> List["HitList"]:
         """All hits of many molecules in one device pass.
 
         Equivalent to ``[list(self.query(m, ...)) for m in molecules]`` (the
         ThreadPool-over-molecules pattern of the reference README) but runs
-        every (template, molecule) pair in a single launch chain.
+        every (template, molecule) pair in a single launch chain. Each
+        molecule's hits come as a `HitList`, a sequence that makes each `Hit`
+        when it is read.
         """
         ignore_chain = _normalize_ignore_chain(ignore_chain)
         molecules = list(molecules)

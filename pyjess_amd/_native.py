@@ -324,14 +324,15 @@ class Result:
         self.stride = L.jam_result_stride(handle)
         self.n_pairs = L.jam_result_n_pairs(handle)
         m = max(n, 1)
-        self.pair = np.zeros(m, dtype=np.int32)
-        self.rank = np.zeros(m, dtype=np.int64)
-        self.flag = np.zeros(m, dtype=np.int32)
-        self.rmsd = np.zeros(m, dtype=np.float64)
-        self.rotation = np.zeros((m, 9), dtype=np.float64)
-        self.c0 = np.zeros((m, 3), dtype=np.float64)
-        self.c1 = np.zeros((m, 3), dtype=np.float64)
-        self.atoms = np.zeros((m, max(self.stride, 1)), dtype=np.int32)
+        # every row is written by the copy
+        self.pair = np.empty(m, dtype=np.int32)
+        self.rank = np.empty(m, dtype=np.int64)
+        self.flag = np.empty(m, dtype=np.int32)
+        self.rmsd = np.empty(m, dtype=np.float64)
+        self.rotation = np.empty((m, 9), dtype=np.float64)
+        self.c0 = np.empty((m, 3), dtype=np.float64)
+        self.c1 = np.empty((m, 3), dtype=np.float64)
+        self.atoms = np.empty((m, max(self.stride, 1)), dtype=np.int32)
         _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
                                  _ptr(self.rmsd), _ptr(self.rotation), _ptr(self.c0), _ptr(self.c1),
                                  _ptr(self.atoms)))

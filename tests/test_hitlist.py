@@ -1,0 +1,67 @@
+"""`HitList`, the lazy per-molecule hit sequence of `Jess.query_batch`, on
+hand-made result arrays (no GPU)."""
+
+import collections.abc
+
+import numpy as np
+import pytest
+
+from pyjess_amd import Molecule, Template, TemplateAtom
+from pyjess_amd._jess import HitList, _Block
+
+from conftest import golden
+
+
+def make_list():
+    mol = Molecule.load(golden("1AMY.pdb"))
+    tpls = [Template.load(golden("1.3.3.tpl")), Template.load(golden("4.1.2.tpl"))]
+    n_ev, T = 6, 2
+    rot = np.arange(n_ev * 9, dtype=np.float64).reshape(n_ev, 9)
+    c0 = np.arange(n_ev * 3, dtype=np.float64).reshape(n_ev, 3)
+    c1 = -c0
+    atoms = np.arange(n_ev * 8, dtype=np.int32).reshape(n_ev, 8)
+    rmsd = np.linspace(0.1, 0.6, n_ev)
+    pair = np.array([0, 1, 1, 0, 1, 1], dtype=np.int32)   # molecule 0, templates 0/1
+    events = np.array([1, 3, 4], dtype=np.int64)
+    return HitList(_Block(rot, c0, c1, atoms), rmsd, pair, events, T, tpls, mol), rot, rmsd, tpls, mol
+
+
+def test_hitlist_is_a_sequence_of_hits_made_on_access():
+    hl, rot, rmsd, tpls, mol = make_list()
+    assert isinstance(hl, collections.abc.Sequence)
+    assert len(hl) == 3 and bool(hl)
+    hits = list(hl)
+    assert [h.rmsd for h in hits] == [rmsd[1], rmsd[3], rmsd[4]]
+    assert [h.template() for h in hits] == [tpls[1], tpls[0], tpls[1]]
+    assert hits[1]._rotation == rot[3].tolist()
+    assert hits[0].molecule() is mol or hits[0]._molecule is mol
+    assert hl[-1].rmsd == rmsd[4]
+    assert [h.rmsd for h in hl[1:]] == [rmsd[3], rmsd[4]]
+    with pytest.raises(IndexError):
+        hl[3]
+    with pytest.raises(IndexError):
+        hl[-4]
+
+
+def test_hitlist_equality_and_empty():
+    hl, *_ = make_list()
+    again, *_ = make_list()
+    assert hl == hl
+    assert hl != []
+    assert repr(hl) == "<HitList of 3 hits>"
+
+
+def test_split_events_cuts_by_molecule_and_drops_nan():
+    from pyjess_amd._hits import split_events
+
+    T = 3
+    pair = np.array([0, 2, 4, 5, 9, 10, 11], dtype=np.int32)   # molecules 0, 0, 1, 1, 3, 3, 3
+    flag = np.array([1, 2, 1, 1, 1, 2, 1], dtype=np.int32)
+    ev, bounds, nan = split_events(pair, flag, None, T, 5)
+    assert ev.tolist() == [0, 2, 3, 4, 6]
+    assert bounds.tolist() == [0, 1, 3, 3, 5, 5]
+    assert nan.tolist() == [1, 0, 0, 1, 0]
+    ev, bounds, nan = split_events(pair, flag, np.array([1, 3, 6], dtype=np.int64), T, 5)
+    assert ev.tolist() == [3, 6]
+    assert bounds.tolist() == [0, 0, 1, 1, 2, 2]
+    assert nan.tolist() == [1, 0, 0, 0, 0]
