@@ -109,6 +109,59 @@ int32_t jam_molecules_create(jam_context *ctx, const jam_molecule_desc *mols, in
                              jam_molecules **out);
 void jam_molecules_free(jam_molecules *m);
 
+/* flags of jam_molecules_parse_pdb (PDBMoleculeParser's skip_hetatm and
+ * ignore_endmdl, src/pyjess/_io.py:52-66) */
+#define JAM_PDB_SKIP_HETATM 1
+#define JAM_PDB_IGNORE_ENDMDL 2
+
+/* Replaces PDBMoleculeParser.load (src/pyjess/_io.py:68-103), Atom.loads for
+ * every ATOM and HETATM record, Molecule.__init__ and ResIndex_create
+ * (src/pyjess/_jess.pyx:632-684, patches/ResIndex.c.patch:34-119) for n PDB
+ * texts at once, on the device: text k is bytes [text_off[k], text_off[k+1])
+ * of `text` and becomes molecule k of the batch. Lines end at '\n', '\r\n'
+ * or '\r'; ATOM records (and HETATM records unless JAM_PDB_SKIP_HETATM) up to
+ * the first ENDMDL record (all of them with JAM_PDB_IGNORE_ENDMDL) are read in
+ * fixed columns, numbers parsed exactly as Python's int() and float() parse
+ * decimal text. JAM_EINVAL, naming the text and line, for an mmCIF data tag,
+ * a numeric field that does not parse (or that uses a form other than
+ * decimal digits with an optional point and exponent), or a byte above 0x7F
+ * in the first 80 columns of a record. HEADER records are not read. */
+int32_t jam_molecules_parse_pdb(jam_context *ctx, const char *text, const int64_t *text_off, int32_t n,
+                                int32_t flags, jam_molecules **out);
+
+/* Every atom field, host buffers of one entry per atom of the batch (any may
+ * be NULL): names, resnames, segid 4 bytes, chains (chainID1, chainID2) and
+ * element 2 bytes per atom, in the encodings of jam_molecule_desc. */
+typedef struct {
+    double *xyz;
+    char *names;
+    char *resnames;
+    char *chains;
+    int32_t *resseq;
+    int32_t *serial;
+    char *altloc;
+    char *icode;
+    float *occupancy;
+    float *tempfactor;
+    char *segid;
+    char *element;
+    int32_t *charge;
+} jam_atom_fields;
+
+/* Atom offsets of the molecules of a batch (n + 1 entries) and the number
+ * of residue-name buckets (either may be NULL). No reference counterpart. */
+int32_t jam_molecules_sizes(const jam_molecules *m, int64_t *atom_off, int64_t *n_buckets);
+/* Copies the atom fields of a batch to the host (Molecule's arrays,
+ * src/pyjess/_jess.pyx:243-907); fields other than xyz, names, resnames and
+ * resseq exist only for a batch made by jam_molecules_parse_pdb. */
+int32_t jam_molecules_copy_atoms(const jam_molecules *m, const jam_atom_fields *fields);
+/* Copies the residue-name index of a batch (ResIndex, patches/ResIndex.c.patch:34-119):
+ * perm (per molecule, its atoms in (upper-cased residue name, serial)
+ * order, molecule-local indices), res_off (n + 1: each molecule's buckets),
+ * and per bucket its key (upper-cased packed name) and range in perm. */
+int32_t jam_molecules_residue_index(const jam_molecules *m, int32_t *perm, int32_t *res_off, uint32_t *res_key,
+                                    int32_t *res_begin, int32_t *res_end);
+
 /* ---- queries ---------------------------------------------------------- */
 
 /* Replaces Jess_query + the JessQuery_next loop + JessQuery_superposition +

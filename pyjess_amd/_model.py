@@ -21,6 +21,7 @@ import datetime
 import io
 import math
 import os
+import re
 import typing
 from typing import Iterable, Optional, Sequence, Tuple
 
@@ -83,6 +84,41 @@ def _chain_bytes(chain_id: str) -> Tuple[int, int]:
 def _decode(raw: bytes) -> str:
     """``decode_token``: ``_`` and NUL become blanks."""
     return "".join(" " if c in (0x5F, 0) else chr(c) for c in raw)
+
+
+# texts of at least this many bytes are parsed on the device by Molecule.load
+DEVICE_PARSE_BYTES = 1 << 20
+
+
+def _read_bytes(file) -> bytes:
+    """The bytes of a path or of a file object (text is encoded as UTF-8)."""
+    if isinstance(file, (str, os.PathLike)):
+        with open(file, "rb") as f:
+            return f.read()
+    data = file.read()
+    return data.encode("utf-8") if isinstance(data, str) else bytes(data)
+
+
+def _pdb_header(text: str, id_, date, name, ignore_endmdl: bool):
+    """id, date and name from the HEADER records read before the parse stops
+    (the HEADER branch of ``PDBMoleculeParser.load``, ``src/pyjess/_io.py:81-92``)."""
+    text = text.replace("\r\n", "\n").replace("\r", "\n")
+    end = len(text)
+    if not ignore_endmdl:
+        m = re.search(r"(?m)^ENDMDL", text)
+        if m:
+            end = m.start()
+    for m in re.finditer(r"(?m)^HEADER.*$", text[:end]):
+        line = m.group(0)
+        if id_ is None:
+            id_ = line[62:66].strip() or None
+        if date is None:
+            t = line[50:59].strip()
+            if t:
+                date = datetime.datetime.strptime(t, "%d-%b-%y").date()
+        if name is None:
+            name = line[10:50].strip() or None
+    return id_, date, name
 
 
 def _fmt(value: float, spec: str) -> str:
@@ -528,9 +564,13 @@ class Molecule:
     @classmethod
     def load(cls, file, format: str = "detect", *, id: Optional[str] = None, date=None,
              name: Optional[str] = None, ignore_endmdl: bool = False,
-             use_author: bool = False, skip_hetatm: bool = False) -> "Molecule":
+             use_author: bool = False, skip_hetatm: bool = False, device: Optional[bool] = None) -> "Molecule":
         """Load a PDB file (``PDBMoleculeParser.load``, ``src/pyjess/_io.py:68-103``).
 
+        ``device``: parse the records on the GPU (`Molecule.load_many`), which
+        also builds the molecule's device copy and residue-name index there;
+        None (the default) does so for texts of ``DEVICE_PARSE_BYTES`` or more
+        when a device is present. The two parsers give the same molecule.
         mmCIF input needs ``gemmi``, which the reference also imports lazily;
         it is not available here and raises `ValueError`.
         """
@@ -538,10 +578,62 @@ class Molecule:
             raise ValueError(f"invalid value for `format` argument: {format!r}")
         if format == "cif":
             raise ValueError("mmCIF parsing is not supported: load the structure from PDB text")
-        if isinstance(file, (str, os.PathLike)):
-            with open(file) as f:
-                return cls._load_pdb(f, id, date, name, ignore_endmdl, skip_hetatm)
-        return cls._load_pdb(file, id, date, name, ignore_endmdl, skip_hetatm)
+        if device is False:
+            if isinstance(file, (str, os.PathLike)):
+                with open(file) as f:
+                    return cls._load_pdb(f, id, date, name, ignore_endmdl, skip_hetatm)
+            return cls._load_pdb(file, id, date, name, ignore_endmdl, skip_hetatm)
+        data = _read_bytes(file)
+        if device is None:
+            from . import _native
+
+            device = len(data) >= DEVICE_PARSE_BYTES and _native.device_available()
+        if not device:
+            text = data.decode("utf-8")
+            return cls._load_pdb(io.StringIO(text, newline=None), id, date, name, ignore_endmdl, skip_hetatm)
+        return cls._load_many_bytes([data], [(id, date, name)], ignore_endmdl, skip_hetatm)[0]
+
+    @classmethod
+    def load_many(cls, files, *, ignore_endmdl: bool = False, skip_hetatm: bool = False) -> typing.List["Molecule"]:
+        """Load many PDB files at once, their records parsed on the GPU in one
+        pass (``jam_molecules_parse_pdb``), each molecule's residue-name index
+        built there too. The molecules are the ones `Molecule.load` reads,
+        and their device copy is kept: a `Jess.query_batch` over the same
+        list, or a `Jess.query` of one that was loaded alone, uploads nothing."""
+        datas = [_read_bytes(f) for f in files]
+        return cls._load_many_bytes(datas, [(None, None, None)] * len(datas), ignore_endmdl, skip_hetatm)
+
+    @classmethod
+    def _load_many_bytes(cls, datas, headers, ignore_endmdl, skip_hetatm) -> typing.List["Molecule"]:
+        from . import _native
+
+        ctx = _native.context()
+        # the header fields come from the text (decoding also rejects what a
+        # text-mode read would)
+        heads = [_pdb_header(d.decode("utf-8"), *h, ignore_endmdl) for d, h in zip(datas, headers)]
+        batch = _native.ParsedMolecules(ctx, datas, skip_hetatm=skip_hetatm, ignore_endmdl=ignore_endmdl)
+        off = batch.atom_offsets()
+        a = batch.arrays()
+        mols = []
+        for k, (id_, date, name) in enumerate(heads):
+            s0, s1 = int(off[k]), int(off[k + 1])
+            mol = cls.from_arrays(a["xyz"][s0:s1], a["names"][s0:s1], a["resnames"][s0:s1], a["chains"][s0:s1],
+                                  a["resseq"][s0:s1], a["serial"][s0:s1], id=id_,
+                                  occupancy=a["occupancy"][s0:s1], tempfactor=a["tempfactor"][s0:s1],
+                                  element=a["element"][s0:s1])
+            mol._date, mol._name = date, name
+            mol.altloc = a["altloc"][s0:s1]
+            mol.icode = a["icode"][s0:s1]
+            mol.segid = a["segid"][s0:s1]
+            mol.charge = a["charge"][s0:s1]
+            mols.append(mol)
+        if len(mols) == 1:
+            mols[0]._device[id(ctx)] = batch
+        else:
+            from . import _jess
+
+            _jess._last_batch[:] = [ctx, list(mols), [m.__dict__.get("_version", 0) for m in mols], batch]
+        return mols
 
     @classmethod
     def _load_pdb(cls, f, id_, date, name, ignore_endmdl, skip_hetatm) -> "Molecule":

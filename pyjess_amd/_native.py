@@ -28,7 +28,8 @@ MAX_TEMPLATE_ATOMS = 32
 
 _SYMBOLS = (
     "jam_context_create", "jam_context_free", "jam_templates_create", "jam_templates_free",
-    "jam_molecules_create", "jam_molecules_free", "jam_query", "jam_result_n_events",
+    "jam_molecules_create", "jam_molecules_free", "jam_molecules_parse_pdb",
+    "jam_molecules_sizes", "jam_molecules_copy_atoms", "jam_molecules_residue_index", "jam_query", "jam_result_n_events",
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
     "jam_result_timings", "jam_result_pair_digests", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_rmsd_forward",
     "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch",
@@ -164,6 +165,14 @@ def lib():
         L.jam_superpose_batch_noalloc.restype = i32
         L.jam_result_pair_digests.argtypes = [vp, i32, i32, vp, vp]
         L.jam_result_pair_digests.restype = i32
+        L.jam_molecules_parse_pdb.argtypes = [vp, ctypes.c_char_p, vp, i32, i32, P(vp)]
+        L.jam_molecules_parse_pdb.restype = i32
+        L.jam_molecules_sizes.argtypes = [vp, vp, vp]
+        L.jam_molecules_sizes.restype = i32
+        L.jam_molecules_copy_atoms.argtypes = [vp, P(AtomFields)]
+        L.jam_molecules_copy_atoms.restype = i32
+        L.jam_molecules_residue_index.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.jam_molecules_residue_index.restype = i32
         L.jam_kd_order.argtypes = [vp, vp, vp, i32, vp]
         L.jam_kd_order.restype = i32
         L.jam_last_error.argtypes = []
@@ -282,6 +291,16 @@ class TemplateSet:
             self.handle = None
 
 
+class AtomFields(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "xyz", "names", "resnames", "chains", "resseq", "serial", "altloc", "icode", "occupancy", "tempfactor",
+        "segid", "element", "charge")]
+
+
+JAM_PDB_SKIP_HETATM = 1
+JAM_PDB_IGNORE_ENDMDL = 2
+
+
 class MoleculeSet:
     """Device copy of a batch of molecules."""
 
@@ -310,6 +329,72 @@ class MoleculeSet:
         if h and _lib is not None:
             _lib.jam_molecules_free(h)
             self.handle = None
+
+    def atom_offsets(self) -> np.ndarray:
+        off = np.zeros(self.n_molecules + 1, dtype=np.int64)
+        _check(lib().jam_molecules_sizes(self.handle, _ptr(off), None))
+        return off
+
+    def residue_index(self):
+        """(perm, res_off, res_key, res_begin, res_end) of the batch."""
+        nb = ctypes.c_int64(0)
+        off = self.atom_offsets()
+        _check(lib().jam_molecules_sizes(self.handle, None, ctypes.byref(nb)))
+        perm = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        res_off = np.zeros(self.n_molecules + 1, dtype=np.int32)
+        key = np.zeros(max(nb.value, 1), dtype=np.uint32)
+        begin = np.zeros(max(nb.value, 1), dtype=np.int32)
+        end = np.zeros(max(nb.value, 1), dtype=np.int32)
+        _check(lib().jam_molecules_residue_index(self.handle, _ptr(perm), _ptr(res_off), _ptr(key), _ptr(begin),
+                                                 _ptr(end)))
+        return perm[:off[-1]], res_off, key[:nb.value], begin[:nb.value], end[:nb.value]
+
+
+class ParsedMolecules(MoleculeSet):
+    """A batch of molecules parsed from PDB texts on the device
+    (jam_molecules_parse_pdb): molecule k of the batch is text k."""
+
+    def __init__(self, ctx: Context, texts: Sequence[bytes], skip_hetatm: bool = False,
+                 ignore_endmdl: bool = False):
+        texts = [bytes(t) for t in texts]
+        off = np.zeros(len(texts) + 1, dtype=np.int64)
+        np.cumsum([len(t) for t in texts], out=off[1:])
+        buf = b"".join(texts)
+        flags = (JAM_PDB_SKIP_HETATM if skip_hetatm else 0) | (JAM_PDB_IGNORE_ENDMDL if ignore_endmdl else 0)
+        h = ctypes.c_void_p()
+        with ctx.lock:
+            rc = lib().jam_molecules_parse_pdb(ctx.handle, buf, _ptr(off), len(texts), flags, ctypes.byref(h))
+        if rc == JAM_EINVAL:
+            raise ValueError(lib().jam_last_error().decode("utf-8", "replace"))
+        _check(rc)
+        self.ctx = ctx
+        self.handle = h
+        self.n_molecules = len(texts)
+
+    def arrays(self) -> dict:
+        """Every atom field of the batch, as numpy arrays (all molecules)."""
+        n = int(self.atom_offsets()[-1])
+        m = max(n, 1)
+        a = {
+            "xyz": np.empty((m, 3), dtype=np.float64),
+            "names": np.empty((m, 4), dtype=np.uint8),
+            "resnames": np.empty((m, 4), dtype=np.uint8),
+            "chains": np.empty((m, 2), dtype=np.uint8),
+            "resseq": np.empty(m, dtype=np.int32),
+            "serial": np.empty(m, dtype=np.int32),
+            "altloc": np.empty(m, dtype=np.uint8),
+            "icode": np.empty(m, dtype=np.uint8),
+            "occupancy": np.empty(m, dtype=np.float32),
+            "tempfactor": np.empty(m, dtype=np.float32),
+            "segid": np.empty((m, 4), dtype=np.uint8),
+            "element": np.empty((m, 2), dtype=np.uint8),
+            "charge": np.empty(m, dtype=np.int32),
+        }
+        f = AtomFields(*(_ptr(a[k]) for k in ("xyz", "names", "resnames", "chains", "resseq", "serial", "altloc",
+                                              "icode", "occupancy", "tempfactor", "segid", "element", "charge")))
+        with self.ctx.lock:
+            _check(lib().jam_molecules_copy_atoms(self.handle, ctypes.byref(f)))
+        return {k: v[:n] for k, v in a.items()}
 
 
 class Result:

@@ -27,6 +27,7 @@
 #include "../../include/jess_amd.h"
 #include "jam_common.h"
 #include "jam_grad.h"
+#include "jam_ingest.h"
 #include "jam_kernels.h"
 
 namespace {
@@ -197,7 +198,11 @@ struct jam_molecules {
     jam_context *ctx = nullptr;
     int n_mols = 0;
     int64_t n_atoms = 0;
+    int64_t n_buckets = 0;
     DBuf atom_off, xyz, name, resname, chain, resseq, perm, res_off, res_key, res_begin, res_end;
+    // the other atom fields, kept by jam_molecules_parse_pdb for the host's copy
+    bool parsed = false;
+    DBuf serial, chains, altloc, icode, occupancy, tempfactor, segid, element, charge;
     DevMols dev() const
     {
         DevMols d;
@@ -525,6 +530,7 @@ extern "C" int32_t jam_molecules_create(jam_context *ctx, const jam_molecule_des
     HIP_TRY(upload(M->res_key, res_key, s));
     HIP_TRY(upload(M->res_begin, res_begin, s));
     HIP_TRY(upload(M->res_end, res_end, s));
+    M->n_buckets = (int64_t)res_key.size();
     *out = M.release();
     return JAM_OK;
 }
@@ -534,6 +540,388 @@ extern "C" void jam_molecules_free(jam_molecules *m)
     if (!m) return;
     (void)hipSetDevice(m->ctx->device);
     delete m;
+}
+
+// ---------------------------------------------------------------------------
+// PDB ingestion on the device
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int SCAN_BLOCK = 1024;
+
+// Exclusive scan of 1024-element blocks of `in` into `out` (64-bit), each
+// block's total into `sums`.
+template <class T>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(const T *in, long long n, long long *out, long long *sums)
+{
+    __shared__ long long w[SCAN_BLOCK / JAM_WAVE];
+    const long long i = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    const long long v = i < n ? (long long)in[i] : 0;
+    const int lane = jam::lane_id(), wv = threadIdx.x / JAM_WAVE;
+    long long x = v;
+    for (int sh = 1; sh < JAM_WAVE; sh <<= 1) {
+        const long long y = __shfl_up(x, sh, JAM_WAVE);
+        if (lane >= sh) x += y;
+    }
+    if (lane == JAM_WAVE - 1) w[wv] = x;
+    __syncthreads();
+    long long base = 0, total = 0;
+    for (int k = 0; k < SCAN_BLOCK / JAM_WAVE; k++) {
+        if (k < wv) base += w[k];
+        total += w[k];
+    }
+    if (i < n) out[i] = base + x - v;
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(long long *out, long long n, const long long *offs)
+{
+    const long long i = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    if (i < n) out[i] += offs[blockIdx.x];
+}
+
+__global__ void k_fill_u64(unsigned long long *v, long long n, unsigned long long x)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        v[i] = x;
+}
+
+__global__ void k_fill_i32(int *v, long long n, int x)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        v[i] = x;
+}
+
+// out[0, n] = exclusive scan of in[0, n) followed by the total (n + 1
+// values), on stream s; temporary space from `tmp`, grown as needed
+template <class T>
+int32_t device_scan(const T *in, long long n, long long *out, std::vector<std::unique_ptr<DBuf>> &tmp, size_t level,
+                    hipStream_t s)
+{
+    const long long blocks = std::max<long long>(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK);
+    if (tmp.size() <= level) tmp.emplace_back(new DBuf());
+    HIP_TRY(tmp[level]->reserve(sizeof(long long) * (size_t)(blocks + 1)));
+    long long *sums = tmp[level]->as<long long>();
+    hipLaunchKernelGGL(k_scan_blocks<T>, dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, in, n, out, sums);
+    HIP_TRY(hipGetLastError());
+    if (blocks == 1) {
+        // the total goes after the scan
+        HIP_TRY(hipMemcpyAsync(out + n, sums, sizeof(long long), hipMemcpyDeviceToDevice, s));
+        return JAM_OK;
+    }
+    // scan of the block totals (with their total at [blocks]), added back
+    if (tmp.size() <= level + 1) tmp.emplace_back(new DBuf());
+    HIP_TRY(tmp[level + 1]->reserve(sizeof(long long) * (size_t)(blocks + 1)));
+    long long *offs = tmp[level + 1]->as<long long>();
+    int32_t rc = device_scan<long long>(sums, blocks, offs, tmp, level + 2, s);
+    if (rc != JAM_OK) return rc;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, out, n, offs);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out + n, offs + blocks, sizeof(long long), hipMemcpyDeviceToDevice, s));
+    return JAM_OK;
+}
+
+template <class T>
+int32_t read_one(const T *dev, T &host, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(&host, dev, sizeof(T), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, const int64_t *text_off, int32_t n,
+                                           int32_t flags, jam_molecules **out)
+{
+    if (!c || !out || !text_off || n < 0) return fail(JAM_EINVAL, "invalid argument");
+    *out = nullptr;
+    if (text_off[0] != 0) return fail(JAM_EINVAL, "text_off[0] must be 0");
+    for (int32_t k = 0; k < n; k++)
+        if (text_off[k + 1] < text_off[k]) return fail(JAM_EINVAL, "text offsets decrease");
+    if (n > 0 && text_off[n] > 0 && !text) return fail(JAM_EINVAL, "NULL text");
+    if (flags & ~(JAM_PDB_SKIP_HETATM | JAM_PDB_IGNORE_ENDMDL)) return fail(JAM_EINVAL, "unknown flags");
+    // the texts, each ending with a line feed
+    std::vector<long long> toff(n + 1, 0);
+    std::vector<unsigned char> buf;
+    buf.reserve((size_t)text_off[n] + (size_t)n);
+    for (int32_t k = 0; k < n; k++) {
+        toff[k] = (long long)buf.size();
+        buf.insert(buf.end(), (const unsigned char *)text + text_off[k], (const unsigned char *)text + text_off[k + 1]);
+        if (buf.size() > (size_t)toff[k] && buf.back() != '\n') buf.push_back('\n');
+    }
+    toff[n] = (long long)buf.size();
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    std::unique_ptr<jam_molecules> M(new jam_molecules());
+    M->ctx = c;
+    M->n_mols = n;
+    M->parsed = true;
+    std::vector<std::unique_ptr<DBuf>> tmp;
+    DBuf d_buf, d_toff, d_blk, d_blkoff, d_starts, d_kind, d_textof, d_first, d_lineatom, d_atomline, d_textrng;
+    HIP_TRY(upload(d_buf, buf, s));
+    HIP_TRY(upload(d_toff, toff, s));
+    jam::IngText tx;
+    tx.buf = d_buf.as<const unsigned char>();
+    tx.n_bytes = (long long)buf.size();
+    tx.text_off = d_toff.as<const long long>();
+    tx.n_texts = n;
+    tx.flags = flags;
+    const long long per_block = (long long)ING_THREADS * ING_BPT;
+    const long long byte_blocks = std::max<long long>(1, (tx.n_bytes + per_block - 1) / per_block);
+    // ---- lines
+    HIP_TRY(d_blk.reserve(sizeof(long long) * (size_t)byte_blocks));
+    HIP_TRY(d_blkoff.reserve(sizeof(long long) * (size_t)(byte_blocks + 1)));
+    hipLaunchKernelGGL(jam::k_ing_count, dim3((unsigned)byte_blocks), dim3(ING_THREADS), 0, s, tx,
+                       d_blk.as<long long>());
+    int32_t rc;
+    if ((rc = device_scan<long long>(d_blk.as<long long>(), byte_blocks, d_blkoff.as<long long>(), tmp, 0, s)) != JAM_OK)
+        return rc;
+    long long n_lines = 0;
+    if ((rc = read_one(d_blkoff.as<long long>() + byte_blocks, n_lines, s)) != JAM_OK) return rc;
+    if (tx.n_bytes == 0) n_lines = 0;
+    const long long lines_cap = std::max<long long>(n_lines, 1);
+    HIP_TRY(d_starts.reserve(sizeof(long long) * (size_t)lines_cap));
+    if (n_lines > 0)
+        hipLaunchKernelGGL(jam::k_ing_starts, dim3((unsigned)byte_blocks), dim3(ING_THREADS), 0, s, tx,
+                           d_blkoff.as<const long long>(), d_starts.as<long long>());
+    // ---- record kinds, first ENDMDL / mmCIF tag per text
+    HIP_TRY(d_kind.reserve((size_t)lines_cap));
+    HIP_TRY(d_textof.reserve(sizeof(int) * (size_t)lines_cap));
+    HIP_TRY(d_first.reserve(sizeof(unsigned long long) * 2 * (size_t)std::max(n, 1)));
+    unsigned long long *first_endmdl = d_first.as<unsigned long long>();
+    unsigned long long *first_mmcif = first_endmdl + std::max(n, 1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(16), dim3(256), 0, s, first_endmdl, 2 * (long long)std::max(n, 1),
+                       (unsigned long long)n_lines);
+    const long long line_blocks = std::max<long long>(1, (n_lines + ING_THREADS - 1) / ING_THREADS);
+    if (n_lines > 0)
+        hipLaunchKernelGGL(jam::k_ing_classify, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
+                           d_starts.as<const long long>(), n_lines, d_kind.as<unsigned char>(), d_textof.as<int>(),
+                           first_endmdl, first_mmcif);
+    // ---- atom records
+    HIP_TRY(d_blk.reserve(sizeof(long long) * (size_t)line_blocks));
+    HIP_TRY(d_blkoff.reserve(sizeof(long long) * (size_t)(line_blocks + 1)));
+    HIP_TRY(d_lineatom.reserve(sizeof(long long) * (size_t)lines_cap));
+    long long n_atoms = 0;
+    if (n_lines > 0) {
+        hipLaunchKernelGGL(jam::k_ing_select_count, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
+                           d_kind.as<const unsigned char>(), d_textof.as<const int>(),
+                           (const unsigned long long *)first_endmdl, n_lines, d_blk.as<long long>());
+        if ((rc = device_scan<long long>(d_blk.as<long long>(), line_blocks, d_blkoff.as<long long>(), tmp, 0, s)) !=
+            JAM_OK)
+            return rc;
+        if ((rc = read_one(d_blkoff.as<long long>() + line_blocks, n_atoms, s)) != JAM_OK) return rc;
+    }
+    if (n_atoms >= INT32_MAX) return fail(JAM_ETOOBIG, "too many atoms in one batch");
+    const long long atoms_cap = std::max<long long>(n_atoms, 1);
+    HIP_TRY(d_atomline.reserve(sizeof(long long) * (size_t)atoms_cap));
+    if (n_lines > 0)
+        hipLaunchKernelGGL(jam::k_ing_select, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
+                           d_kind.as<const unsigned char>(), d_textof.as<const int>(),
+                           (const unsigned long long *)first_endmdl, n_lines, d_blkoff.as<const long long>(),
+                           d_atomline.as<long long>(), d_lineatom.as<long long>());
+    HIP_TRY(d_textrng.reserve(sizeof(long long) * 2 * (size_t)(n + 1)));
+    long long *text_line = d_textrng.as<long long>(), *text_atom = text_line + (n + 1);
+    hipLaunchKernelGGL(jam::k_ing_text_ranges, dim3((unsigned)((n + 1 + 63) / 64)), dim3(64), 0, s, tx,
+                       d_starts.as<const long long>(), n_lines, d_lineatom.as<const long long>(), n_atoms, text_line,
+                       text_atom);
+    HIP_TRY(hipGetLastError());
+    std::vector<long long> h_rng(2 * (size_t)(n + 1)), h_first(2 * (size_t)std::max(n, 1));
+    HIP_TRY(hipMemcpyAsync(h_rng.data(), text_line, sizeof(long long) * h_rng.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_first.data(), first_endmdl, sizeof(long long) * h_first.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // an mmCIF tag before the parse stops (PDBMoleculeParser raises)
+    for (int32_t k = 0; k < n; k++) {
+        const long long tag = h_first[std::max(n, 1) + k], end = h_first[k];
+        if (tag < n_lines && ((flags & JAM_PDB_IGNORE_ENDMDL) || tag < end))
+            return fail(JAM_EINVAL, "text " + std::to_string(k) + ": mmCIF data tags found, file is not in PDB format");
+    }
+    std::vector<int> aoff(n + 1);
+    for (int32_t k = 0; k <= n; k++) aoff[k] = (int)h_rng[(size_t)(n + 1) + k];
+    M->n_atoms = n_atoms;
+    // ---- fields
+    const size_t na = (size_t)atoms_cap;
+    HIP_TRY(upload(M->atom_off, aoff, s));
+    HIP_TRY(M->xyz.reserve(sizeof(double) * 3 * na));
+    HIP_TRY(M->name.reserve(4 * na));
+    HIP_TRY(M->resname.reserve(4 * na));
+    HIP_TRY(M->chain.reserve(sizeof(int) * na));
+    HIP_TRY(M->chains.reserve(2 * na));
+    HIP_TRY(M->resseq.reserve(sizeof(int) * na));
+    HIP_TRY(M->serial.reserve(sizeof(int) * na));
+    HIP_TRY(M->altloc.reserve(na));
+    HIP_TRY(M->icode.reserve(na));
+    HIP_TRY(M->occupancy.reserve(sizeof(float) * na));
+    HIP_TRY(M->tempfactor.reserve(sizeof(float) * na));
+    HIP_TRY(M->segid.reserve(4 * na));
+    HIP_TRY(M->element.reserve(2 * na));
+    HIP_TRY(M->charge.reserve(sizeof(int) * na));
+    DBuf d_err, d_firsterr;
+    HIP_TRY(d_err.reserve(sizeof(unsigned int) * na));
+    HIP_TRY(d_firsterr.reserve(sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, d_firsterr.as<unsigned long long>(), 1ll,
+                       (unsigned long long)n_atoms);
+    jam::IngAtoms at;
+    at.xyz = M->xyz.as<double>();
+    at.name = M->name.as<uint32_t>();
+    at.resname = M->resname.as<uint32_t>();
+    at.chain = M->chain.as<int>();
+    at.chains = M->chains.as<unsigned char>();
+    at.resseq = M->resseq.as<int>();
+    at.serial = M->serial.as<int>();
+    at.altloc = M->altloc.as<unsigned char>();
+    at.icode = M->icode.as<unsigned char>();
+    at.occupancy = M->occupancy.as<float>();
+    at.tempfactor = M->tempfactor.as<float>();
+    at.segid = M->segid.as<uint32_t>();
+    at.element = M->element.as<unsigned short>();
+    at.charge = M->charge.as<int>();
+    at.err = d_err.as<unsigned int>();
+    at.first_err = d_firsterr.as<unsigned long long>();
+    if (n_atoms > 0)
+        hipLaunchKernelGGL(jam::k_ing_parse, dim3((unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS)),
+                           dim3(ING_THREADS), 0, s, tx, d_starts.as<const long long>(), n_lines,
+                           d_atomline.as<const long long>(), n_atoms, at);
+    HIP_TRY(hipGetLastError());
+    unsigned long long first_err = 0;
+    if ((rc = read_one(d_firsterr.as<unsigned long long>(), first_err, s)) != JAM_OK) return rc;
+    if ((long long)first_err < n_atoms) {
+        long long li = 0;
+        unsigned int kind = 0;
+        HIP_TRY(hipMemcpyAsync(&li, d_atomline.as<long long>() + first_err, sizeof(li), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&kind, d_err.as<unsigned int>() + first_err, sizeof(kind), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        int t = 0;
+        while (t + 1 < n && h_rng[t + 1] <= li) t++;
+        return fail(JAM_EINVAL, "text " + std::to_string(t) + ", line " + std::to_string(li - h_rng[t] + 1) +
+                                    ": failed to parse atom (" +
+                                    ((kind & ING_ENONASCII) ? "a non-ASCII character in the first 80 columns"
+                                                            : "a numeric field does not parse") +
+                                    ")");
+    }
+    // ---- residue-name index
+    DBuf d_keys, d_nnames, d_resoff64, d_bucket, d_bcount, d_bstart, d_unsorted, d_scr;
+    HIP_TRY(d_keys.reserve(sizeof(uint32_t) * ING_MAX_NAMES * (size_t)std::max(n, 1)));
+    HIP_TRY(d_nnames.reserve(sizeof(int) * (size_t)std::max(n, 1)));
+    if (n > 0)
+        hipLaunchKernelGGL(jam::k_ing_names, dim3((unsigned)n), dim3(1024), 0, s, M->atom_off.as<const int>(),
+                           M->resname.as<const uint32_t>(), d_keys.as<uint32_t>(), d_nnames.as<int>());
+    HIP_TRY(hipGetLastError());
+    std::vector<int> nnames(std::max(n, 1), 0);
+    HIP_TRY(hipMemcpyAsync(nnames.data(), d_nnames.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<long long> resoff64(n + 1, 0);
+    std::vector<int> resoff(n + 1, 0);
+    for (int32_t k = 0; k < n; k++) {
+        if (nnames[k] < 0)
+            return fail(JAM_ETOOBIG, "text " + std::to_string(k) + " has more than " + std::to_string(ING_MAX_NAMES) +
+                                         " distinct residue names");
+        resoff64[k + 1] = resoff64[k] + nnames[k];
+        resoff[k + 1] = (int)resoff64[k + 1];
+    }
+    const long long nb = resoff64[n];
+    M->n_buckets = nb;
+    const size_t nbc = (size_t)std::max<long long>(nb, 1);
+    HIP_TRY(upload(M->res_off, resoff, s));
+    HIP_TRY(upload(d_resoff64, resoff64, s));
+    HIP_TRY(M->res_key.reserve(sizeof(uint32_t) * nbc));
+    HIP_TRY(M->res_begin.reserve(sizeof(int) * nbc));
+    HIP_TRY(M->res_end.reserve(sizeof(int) * nbc));
+    HIP_TRY(M->perm.reserve(sizeof(int) * na));
+    if (n > 0)
+        hipLaunchKernelGGL(jam::k_ing_keys, dim3((unsigned)n), dim3(256), 0, s, d_keys.as<const uint32_t>(),
+                           d_nnames.as<const int>(), d_resoff64.as<const long long>(), n, M->res_key.as<uint32_t>());
+    HIP_TRY(d_bucket.reserve(sizeof(int) * na));
+    HIP_TRY(d_bcount.reserve(sizeof(int) * nbc));
+    HIP_TRY(d_bstart.reserve(sizeof(long long) * (nbc + 1)));
+    HIP_TRY(d_unsorted.reserve(sizeof(int) * nbc));
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, d_bcount.as<int>(), (long long)nbc, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, d_unsorted.as<int>(), (long long)nbc, 0);
+    if (n_atoms > 0) {
+        hipLaunchKernelGGL(jam::k_ing_bucket, dim3((unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS)),
+                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), n, n_atoms,
+                           M->resname.as<const uint32_t>(), M->res_key.as<const uint32_t>(),
+                           d_resoff64.as<const long long>(), d_bucket.as<int>(), d_bcount.as<int>());
+        if ((rc = device_scan<int>(d_bcount.as<int>(), nb, d_bstart.as<long long>(), tmp, 0, s)) != JAM_OK) return rc;
+        const long long waves_per_block = ING_THREADS / JAM_WAVE;
+        hipLaunchKernelGGL(jam::k_ing_partition, dim3((unsigned)((nb + waves_per_block - 1) / waves_per_block)),
+                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), d_resoff64.as<const long long>(), n,
+                           nb, d_bstart.as<const long long>(), d_bucket.as<const int>(), M->serial.as<const int>(),
+                           M->perm.as<int>(), M->res_begin.as<int>(), M->res_end.as<int>(), d_unsorted.as<int>());
+        HIP_TRY(d_scr.reserve(sizeof(unsigned long long) * 2 * na));
+        hipLaunchKernelGGL(jam::k_ing_fixup, dim3((unsigned)nb), dim3(1024), 0, s, M->atom_off.as<const int>(),
+                           d_resoff64.as<const long long>(), n, nb, d_unsorted.as<const int>(),
+                           M->res_begin.as<const int>(), M->res_end.as<const int>(), M->serial.as<const int>(),
+                           M->perm.as<int>(), d_scr.as<unsigned long long>());
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = M.release();
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_molecules_sizes(const jam_molecules *m, int64_t *atom_off, int64_t *n_buckets)
+{
+    if (!m) return fail(JAM_EINVAL, "NULL molecules");
+    if (n_buckets) *n_buckets = m->n_buckets;
+    if (atom_off) {
+        std::vector<int> a((size_t)m->n_mols + 1, 0);
+        if (m->n_mols > 0) {
+            HIP_TRY(hipSetDevice(m->ctx->device));
+            HIP_TRY(hipMemcpy(a.data(), m->atom_off.ptr, sizeof(int) * a.size(), hipMemcpyDeviceToHost));
+        }
+        for (size_t k = 0; k < a.size(); k++) atom_off[k] = a[k];
+    }
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_molecules_copy_atoms(const jam_molecules *m, const jam_atom_fields *f)
+{
+    if (!m || !f) return fail(JAM_EINVAL, "NULL argument");
+    const size_t na = (size_t)m->n_atoms;
+    if (na == 0) return JAM_OK;
+    const bool extra = f->serial || f->chains || f->altloc || f->icode || f->occupancy || f->tempfactor || f->segid ||
+                       f->element || f->charge;
+    if (extra && !m->parsed) return fail(JAM_EINVAL, "only molecules parsed on the device keep every atom field");
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    auto get = [&](void *dst, const DBuf &src, size_t bytes) -> hipError_t {
+        if (!dst) return hipSuccess;
+        return hipMemcpyAsync(dst, src.ptr, bytes, hipMemcpyDeviceToHost, s);
+    };
+    HIP_TRY(get(f->xyz, m->xyz, sizeof(double) * 3 * na));
+    HIP_TRY(get(f->names, m->name, 4 * na));
+    HIP_TRY(get(f->resnames, m->resname, 4 * na));
+    HIP_TRY(get(f->resseq, m->resseq, sizeof(int) * na));
+    HIP_TRY(get(f->chains, m->chains, 2 * na));
+    HIP_TRY(get(f->serial, m->serial, sizeof(int) * na));
+    HIP_TRY(get(f->altloc, m->altloc, na));
+    HIP_TRY(get(f->icode, m->icode, na));
+    HIP_TRY(get(f->occupancy, m->occupancy, sizeof(float) * na));
+    HIP_TRY(get(f->tempfactor, m->tempfactor, sizeof(float) * na));
+    HIP_TRY(get(f->segid, m->segid, 4 * na));
+    HIP_TRY(get(f->element, m->element, 2 * na));
+    HIP_TRY(get(f->charge, m->charge, sizeof(int) * na));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_molecules_residue_index(const jam_molecules *m, int32_t *perm, int32_t *res_off,
+                                               uint32_t *res_key, int32_t *res_begin, int32_t *res_end)
+{
+    if (!m) return fail(JAM_EINVAL, "NULL molecules");
+    HIP_TRY(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    const size_t na = (size_t)m->n_atoms, nb = (size_t)m->n_buckets;
+    if (perm && na) HIP_TRY(hipMemcpyAsync(perm, m->perm.ptr, sizeof(int) * na, hipMemcpyDeviceToHost, s));
+    if (res_off) HIP_TRY(hipMemcpyAsync(res_off, m->res_off.ptr, sizeof(int) * ((size_t)m->n_mols + 1),
+                                        hipMemcpyDeviceToHost, s));
+    if (res_key && nb) HIP_TRY(hipMemcpyAsync(res_key, m->res_key.ptr, 4 * nb, hipMemcpyDeviceToHost, s));
+    if (res_begin && nb) HIP_TRY(hipMemcpyAsync(res_begin, m->res_begin.ptr, 4 * nb, hipMemcpyDeviceToHost, s));
+    if (res_end && nb) HIP_TRY(hipMemcpyAsync(res_end, m->res_end.ptr, 4 * nb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return JAM_OK;
 }
 
 // ---------------------------------------------------------------------------
