@@ -16,9 +16,12 @@ objects) on one chunk; ``plain`` is the drop-in ``list(Jess(templates).query(mol
 ...))`` on one 8,000-atom protein x 500 templates; ``batched`` is
 ``Jess.query_batch`` on 4,096 molecules x 200 templates; ``differentiation``
 is the differentiable superposition RMSD (forward + backward) over 1M
-point-set pairs. Each carries a CPU baseline (the C restatement of the
+point-set pairs; ``ingestion`` is the device PDB parse (records, exact
+decimals, residue-name index) of one chunk of the library's assemblies
+written as PDB text. Each carries a CPU baseline (the C restatement of the
 reference algorithm built for the host, on the host's cores, on a bounded
-sample; torch autograd through an SVD for the RMSD) and the search legs the
+sample; torch autograd through an SVD for the RMSD; the host PDB parser for
+ingestion) and the search legs the
 roofline of the dominant kernel, the join's k_join launches (one per
 level), timed with HIP events on the library's stream in one more pass
 after the timed ones.
@@ -268,7 +271,8 @@ def measure(name, scale, seed, steps, warmup, cpu_seconds, cpu, work_sample):
     return out
 
 
-def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu, parity_all=True, search_only=False):
+def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu, parity_all=True, search_only=False,
+                    ingest_steps=0):
     """Library-scale batched search: 64 assemblies of 100,000 atoms x 7,000
     templates, streamed through jam_query in chunks of ``chunk`` assemblies
     (one batched call each, every (template, molecule) pair of the chunk).
@@ -499,6 +503,89 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu, parity_
                                        "oracle_seconds": full_s, "oracle_threads": threads}
         log(f"library: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads; hit-set parity on "
             f"{n_par}/{len(mols)} assemblies ({int(hc_all.sum())} hits, oracle {full_s:.1f}s): {parity}")
+    if ingest_steps > 0:
+        out["ingestion"] = measure_ingest(mols[:chunk], ingest_steps, cpu_seconds, cpu)
+    return out
+
+
+def measure_ingest(mols, steps, cpu_seconds, cpu):
+    """PDB ingestion of one chunk of library assemblies: their PDB texts
+    (``Molecule.dumps`` of the assemblies translated to non-negative
+    coordinates, serial numbers wrapped at 99,999 as the five serial
+    columns force) parsed on the device into a batch with its residue-name
+    index (jam_molecules_parse_pdb: text upload from pageable host memory,
+    line split, records, exact decimal conversion, index; ``value``), and
+    the drop-in ``Molecule.load_many`` (host ``Molecule`` objects as well).
+    The CPU baseline is the host parser, ``Molecule.load(..., device=False)``
+    (a pure-Python restatement of PDBMoleculeParser.load + Atom.loads), on
+    one core over the first text. Parity: every field of the first molecule
+    equals the host parse, bit for bit, and every molecule's coordinates,
+    names, residues, chains and serials equal the ones written."""
+    import io
+
+    from pyjess_amd import Molecule, _native
+
+    t0 = time.time()
+    written, texts = [], []
+    for m in mols:
+        # translated to non-negative coordinates (eight columns hold -999.999
+        # to 9999.999), rounded to the 0.001 A the text keeps
+        xyz = np.round(m.xyz - m.xyz.min(axis=0), 3)
+        wrapped = np.arange(len(m), dtype=np.int32) % 99999 + 1
+        w = Molecule.from_arrays(xyz, m.names, m.resnames, m.chains, m.resseq, wrapped, id=m.id)
+        written.append(w)
+        texts.append(w.dumps().encode())
+    n_bytes = sum(len(t) for t in texts)
+    n_atoms = sum(len(m) for m in mols)
+    log(f"ingestion: {len(texts)} PDB texts, {n_bytes / 1e6:.1f} MB, {n_atoms} atoms, written in "
+        f"{time.time() - t0:.1f}s")
+    ctx = _native.context()
+    _native.ParsedMolecules(ctx, texts)          # warm-up
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        batch = _native.ParsedMolecules(ctx, texts)
+        times.append(time.perf_counter() - t0)
+        del batch
+    dev_s = float(np.median(times))
+    t0 = time.perf_counter()
+    loaded = Molecule.load_many([io.BytesIO(t) for t in texts])
+    drop_s = time.perf_counter() - t0
+    fields = ("xyz", "names", "resnames", "chains", "resseq", "serial", "altloc", "icode", "occupancy",
+              "tempfactor", "segid", "element", "charge")
+
+    def bits(a):
+        return a.view(np.dtype(f"u{a.itemsize}")) if a.dtype.kind == "f" else a
+
+    parity = all(np.array_equal(bits(getattr(d, f)), bits(getattr(w, f)))
+                 for d, w in zip(loaded, written) for f in fields[:6])
+    out = {
+        "value": n_atoms / dev_s, "unit": "atoms/s", "ms_per_chunk": dev_s * 1e3,
+        "text_GB_per_s": n_bytes / dev_s / 1e9, "steps": steps,
+        "config": {"texts": len(texts), "bytes": n_bytes, "atoms": n_atoms},
+        "drop_in": {"value": n_atoms / drop_s, "unit": "atoms/s", "ms": drop_s * 1e3,
+                    "note": "Molecule.load_many: device parse, header fields, Molecule objects on the host"},
+        "cpu_baseline": None,
+    }
+    if cpu:
+        text0 = texts[0].decode()
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            host = Molecule.load(io.StringIO(text0), device=False)
+            reps += 1
+            if time.perf_counter() - t0 >= cpu_seconds:
+                break
+        cpu_s = time.perf_counter() - t0
+        cpu_rate = reps * len(mols[0]) / cpu_s
+        parity = parity and all(np.array_equal(bits(getattr(loaded[0], f)), bits(getattr(host, f))) for f in fields)
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "atoms/s", "cores": 1, "kind": "port",
+            "sample": f"the first text ({len(texts[0]) / 1e6:.1f} MB, {len(mols[0])} atoms) parsed {reps} times in "
+                      f"{cpu_s:.1f}s by Molecule.load(device=False) (pure-Python PDBMoleculeParser.load + Atom.loads)"}
+        out["gpu_vs_cpu"] = out["value"] / cpu_rate
+    out["parity"] = bool(parity)
+    log(f"ingestion: device {dev_s * 1e3:.1f} ms per chunk ({out['value'] / 1e6:.1f} M atoms/s, "
+        f"{out['text_GB_per_s']:.2f} GB/s of text), load_many {drop_s * 1e3:.0f} ms, parity {parity}")
     return out
 
 
@@ -641,7 +728,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="library-64",
                     help="library-64 (the headline, with the other workloads beside it) | plain-2k | plain-8k | "
-                         "batch-4k | rmsd-grad-1m (one workload alone)")
+                         "batch-4k | rmsd-grad-1m | ingest-8 (one workload alone)")
     ap.add_argument("--plain-steps", type=int, default=200,
                     help="timed steps of the plain-8k measurement beside the headline (0: skip it)")
     ap.add_argument("--parity-first-chunk", action="store_true",
@@ -654,6 +741,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--search-only", action="store_true",
                     help="library-64: only the device-resident search (no event copy, no query_batch leg)")
+    ap.add_argument("--ingest-steps", type=int, default=3,
+                    help="timed device parses of one chunk of library PDB texts (0: skip the ingestion leg)")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the 4,096 molecules of the batched measurement (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -676,6 +765,19 @@ def main():
                "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"], "differentiation": d}
         print(json.dumps(out), flush=True)
         return
+    if args.workload == "ingest-8":
+        # the PDB ingestion of one chunk of library assemblies alone
+        from pyjess_amd import synthetic
+
+        mols, _, _ = synthetic.workload("library-64", seed=args.seed, scale=args.library_chunk / 64)
+        g = measure_ingest(mols, args.steps, args.cpu_seconds, not args.no_cpu)
+        out = {"metric": "PDB atoms ingested/sec (device parse + residue-name index), 1 GPU vs CPU",
+               "value": g["value"], "unit": g["unit"], "n_gpus": 1, "steps": args.steps, "warmup": 1,
+               "ms_per_step": g["ms_per_chunk"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "u8", "data": "synthetic", "config": dict(g["config"], workload="ingest-8"),
+               "roofline": None, "cpu_baseline": g["cpu_baseline"], "ingestion": g}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload in ("plain-2k", "plain-8k", "batch-4k"):
         # one search workload alone
         if args.workload == "batch-4k":
@@ -694,7 +796,7 @@ def main():
     # 100,000 atoms x 7,000 templates), exactly --steps timed passes
     library = measure_library(args.scale, args.seed, args.library_chunk, args.steps, args.warmup,
                               args.cpu_seconds, not args.no_cpu, parity_all=not args.parity_first_chunk,
-                              search_only=args.search_only)
+                              search_only=args.search_only, ingest_steps=args.ingest_steps)
     plain = None
     if args.plain_steps > 0:
         plain = measure("plain-8k", 1.0, args.seed, args.plain_steps, 3, args.cpu_seconds, not args.no_cpu, None)

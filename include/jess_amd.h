@@ -117,16 +117,18 @@ void jam_molecules_free(jam_molecules *m);
 /* Replaces PDBMoleculeParser.load (src/pyjess/_io.py:68-103), Atom.loads for
  * every ATOM and HETATM record, Molecule.__init__ and ResIndex_create
  * (src/pyjess/_jess.pyx:632-684, patches/ResIndex.c.patch:34-119) for n PDB
- * texts at once, on the device: text k is bytes [text_off[k], text_off[k+1])
- * of `text` and becomes molecule k of the batch. Lines end at '\n', '\r\n'
- * or '\r'; ATOM records (and HETATM records unless JAM_PDB_SKIP_HETATM) up to
- * the first ENDMDL record (all of them with JAM_PDB_IGNORE_ENDMDL) are read in
- * fixed columns, numbers parsed exactly as Python's int() and float() parse
- * decimal text. JAM_EINVAL, naming the text and line, for an mmCIF data tag,
- * a numeric field that does not parse (or that uses a form other than
- * decimal digits with an optional point and exponent), or a byte above 0x7F
- * in the first 80 columns of a record. HEADER records are not read. */
-int32_t jam_molecules_parse_pdb(jam_context *ctx, const char *text, const int64_t *text_off, int32_t n,
+ * texts at once, on the device: text k is the `lengths[k]` bytes at
+ * `texts[k]` (host memory, copied to the device as it is) and becomes
+ * molecule k of the batch. Lines end at '\n', '\r\n' or '\r'; ATOM records
+ * (and HETATM records unless JAM_PDB_SKIP_HETATM) up to the first ENDMDL
+ * record (all of them with JAM_PDB_IGNORE_ENDMDL) are read in fixed columns,
+ * numbers parsed exactly as Python's int() and float() parse decimal text.
+ * JAM_EINVAL, naming the text and line, for an mmCIF data tag, a numeric
+ * field that does not parse (or that uses a form other than decimal digits
+ * with an optional point and exponent), or a byte above 0x7F in the first 80
+ * columns of a record. HEADER records are not read. The context keeps the
+ * parse's temporary device buffers for its next parse. */
+int32_t jam_molecules_parse_pdb(jam_context *ctx, const char *const *texts, const int64_t *lengths, int32_t n,
                                 int32_t flags, jam_molecules **out);
 
 /* Every atom field, host buffers of one entry per atom of the batch (any may

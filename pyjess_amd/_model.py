@@ -99,17 +99,33 @@ def _read_bytes(file) -> bytes:
     return data.encode("utf-8") if isinstance(data, str) else bytes(data)
 
 
-def _pdb_header(text: str, id_, date, name, ignore_endmdl: bool):
+def _record_starts(data: bytes, tag: bytes, end: int):
+    """Offsets of the lines of ``data[:end]`` that begin with ``tag`` (lines
+    end at LF, CR LF or a lone CR, as a text-mode read splits them)."""
+    i = data.find(tag, 0, end)
+    while i >= 0:
+        if i == 0 or data[i - 1] in (10, 13):
+            yield i
+        i = data.find(tag, i + 1, end)
+
+
+def _pdb_header(data: bytes, id_, date, name, ignore_endmdl: bool):
     """id, date and name from the HEADER records read before the parse stops
-    (the HEADER branch of ``PDBMoleculeParser.load``, ``src/pyjess/_io.py:81-92``)."""
-    text = text.replace("\r\n", "\n").replace("\r", "\n")
-    end = len(text)
+    (the HEADER branch of ``PDBMoleculeParser.load``, ``src/pyjess/_io.py:81-92``),
+    found in the bytes of a PDB text without decoding all of it; a text that
+    is not valid UTF-8 raises `UnicodeDecodeError` as a text-mode read does."""
+    if not data.isascii():
+        data.decode("utf-8")
+    end = len(data)
     if not ignore_endmdl:
-        m = re.search(r"(?m)^ENDMDL", text)
-        if m:
-            end = m.start()
-    for m in re.finditer(r"(?m)^HEADER.*$", text[:end]):
-        line = m.group(0)
+        end = next(_record_starts(data, b"ENDMDL", end), end)
+    for i in _record_starts(data, b"HEADER", end):
+        j = len(data)
+        for sep in (b"\n", b"\r"):
+            k = data.find(sep, i)
+            if 0 <= k < j:
+                j = k
+        line = data[i:j].decode("utf-8")
         if id_ is None:
             id_ = line[62:66].strip() or None
         if date is None:
@@ -123,6 +139,58 @@ def _pdb_header(text: str, id_, date, name, ignore_endmdl: bool):
 
 def _fmt(value: float, spec: str) -> str:
     return format(value, spec)
+
+
+def _fixed_columns(values: np.ndarray, width: int, decimals: int) -> Tuple[np.ndarray, np.ndarray]:
+    """``format(v, f"{width}.{decimals}f")`` (``decimals`` 0: ``format(v,
+    f"{width}d")`` of integers) of every value, as an (n, width) byte array,
+    and the rows it cannot write exactly: a number wider than its columns, a
+    NaN or infinity, or a float whose scaled value lies within 1e-6 of a
+    rounding tie (those need the exact decimal expansion that `format` does;
+    below 10**8 the scaling itself errs by less than 1.2e-8)."""
+    n = values.shape[0]
+    if decimals:
+        v = values.astype(np.float64)
+        scaled = np.abs(v) * float(10 ** decimals)
+        with np.errstate(invalid="ignore"):
+            slow = ~(scaled < 1e8) | (np.abs(scaled - np.floor(scaled) - 0.5) < 1e-6)
+        mag = np.where(slow, 0.0, np.rint(np.where(slow, 0.0, scaled))).astype(np.int64)
+        neg = np.signbit(v)
+    else:
+        v = values.astype(np.int64)
+        mag, neg = np.abs(v), v < 0
+        slow = mag < 0                         # the most negative int64
+        mag = np.where(slow, 0, mag)
+    ip = mag // (10 ** decimals)
+    digits = np.ones(n, dtype=np.int64)
+    t = ip // 10
+    while (t > 0).any():
+        digits += t > 0
+        t //= 10
+    point = decimals + 1 if decimals else 0
+    slow |= digits + neg + point > width
+    out = np.full((n, width), ord(" "), dtype=np.uint8)
+    m = np.where(slow, 0, mag)
+    col = width - 1
+    for _ in range(decimals):
+        out[:, col] = 48 + m % 10
+        m //= 10
+        col -= 1
+    if decimals:
+        out[:, col] = ord(".")
+        col -= 1
+    for j in range(col + 1):
+        has = j < digits
+        out[has, col - j] = 48 + m[has] % 10
+        m //= 10
+    rows = np.nonzero(neg & ~slow)[0]
+    out[rows, col - digits[rows]] = ord("-")
+    return out, slow
+
+
+def _decoded(tokens: np.ndarray) -> np.ndarray:
+    """``_decode`` of a byte array: ``_`` and NUL become blanks."""
+    return np.where((tokens == 0x5F) | (tokens == 0), np.uint8(32), tokens).astype(np.uint8)
 
 
 # --------------------------------------------------------------------------
@@ -610,7 +678,7 @@ class Molecule:
         ctx = _native.context()
         # the header fields come from the text (decoding also rejects what a
         # text-mode read would)
-        heads = [_pdb_header(d.decode("utf-8"), *h, ignore_endmdl) for d, h in zip(datas, headers)]
+        heads = [_pdb_header(d, *h, ignore_endmdl) for d, h in zip(datas, headers)]
         batch = _native.ParsedMolecules(ctx, datas, skip_hetatm=skip_hetatm, ignore_endmdl=ignore_endmdl)
         off = batch.atom_offsets()
         a = batch.arrays()
@@ -756,14 +824,45 @@ class Molecule:
     def dumps(self, format: str = "pdb", write_header: bool = True) -> str:
         if format != "pdb":
             raise ValueError(f"invalid value for `format` argument: {format!r}")
-        out = []
+        head = ""
         if write_header:
             name = (self.name or "")[:40]
             id_ = (self.id or "")[:4]
             date = "" if self.date is None else self.date.strftime("%d-%b-%y").upper()
-            out.append(f"{'HEADER':<6}{'':4}{name:40}{date:9}{'':3}{id_:4}\n")
-        out.extend(atom.dumps() for atom in self)
-        return "".join(out)
+            head = f"{'HEADER':<6}{'':4}{name:40}{date:9}{'':3}{id_:4}\n"
+        return head + self._atom_records()
+
+    def _atom_records(self) -> str:
+        """``"".join(atom.dumps() for atom in self)``, the columns of all atoms
+        written at once; a row with a field that `_fixed_columns` cannot
+        write exactly is formatted by `Atom.dumps`."""
+        n = len(self)
+        if n == 0:
+            return ""
+        buf = np.full((n, 79), ord(" "), dtype=np.uint8)
+        buf[:, 0:6] = np.frombuffer(b"ATOM  ", dtype=np.uint8)
+        slow = np.zeros(n, dtype=bool)
+        for col, width, dec, values in ((6, 5, 0, self.serial), (22, 4, 0, self.resseq),
+                                        (30, 8, 3, self.xyz[:, 0]), (38, 8, 3, self.xyz[:, 1]),
+                                        (46, 8, 3, self.xyz[:, 2]), (54, 6, 2, self.occupancy),
+                                        (60, 6, 2, self.tempfactor)):
+            cols, s = _fixed_columns(values, width, dec)
+            buf[:, col:col + width] = cols
+            slow |= s
+        buf[:, 12:16] = _decoded(self.names)
+        buf[:, 16] = self.altloc
+        buf[:, 17:20] = _decoded(self.resnames[:, :3])
+        buf[:, 20:22] = self.chains
+        buf[:, 26] = self.icode
+        buf[:, 76:78] = _decoded(self.element)
+        buf[:, 78] = ord("\n")
+        text = buf.tobytes().decode("latin-1")     # one character per byte, as chr() gives
+        if not slow.any():
+            return text
+        lines = [text[79 * i:79 * i + 79] for i in range(n)]
+        for i in np.nonzero(slow)[0]:
+            lines[i] = self._atom(int(i)).dumps()
+        return "".join(lines)
 
     def dump(self, file, format: str = "pdb", write_header: bool = True) -> None:
         file.write(self.dumps(format=format, write_header=write_header))

@@ -165,7 +165,7 @@ def lib():
         L.jam_superpose_batch_noalloc.restype = i32
         L.jam_result_pair_digests.argtypes = [vp, i32, i32, vp, vp]
         L.jam_result_pair_digests.restype = i32
-        L.jam_molecules_parse_pdb.argtypes = [vp, ctypes.c_char_p, vp, i32, i32, P(vp)]
+        L.jam_molecules_parse_pdb.argtypes = [vp, P(ctypes.c_char_p), vp, i32, i32, P(vp)]
         L.jam_molecules_parse_pdb.restype = i32
         L.jam_molecules_sizes.argtypes = [vp, vp, vp]
         L.jam_molecules_sizes.restype = i32
@@ -356,14 +356,15 @@ class ParsedMolecules(MoleculeSet):
 
     def __init__(self, ctx: Context, texts: Sequence[bytes], skip_hetatm: bool = False,
                  ignore_endmdl: bool = False):
-        texts = [bytes(t) for t in texts]
-        off = np.zeros(len(texts) + 1, dtype=np.int64)
-        np.cumsum([len(t) for t in texts], out=off[1:])
-        buf = b"".join(texts)
+        # the library reads every text where it lies (c_char_p points into
+        # the bytes object), during the call
+        texts = [t if isinstance(t, bytes) else bytes(t) for t in texts]
+        ptrs = (ctypes.c_char_p * max(len(texts), 1))(*texts)
+        lengths = np.array([len(t) for t in texts] or [0], dtype=np.int64)
         flags = (JAM_PDB_SKIP_HETATM if skip_hetatm else 0) | (JAM_PDB_IGNORE_ENDMDL if ignore_endmdl else 0)
         h = ctypes.c_void_p()
         with ctx.lock:
-            rc = lib().jam_molecules_parse_pdb(ctx.handle, buf, _ptr(off), len(texts), flags, ctypes.byref(h))
+            rc = lib().jam_molecules_parse_pdb(ctx.handle, ptrs, _ptr(lengths), len(texts), flags, ctypes.byref(h))
         if rc == JAM_EINVAL:
             raise ValueError(lib().jam_last_error().decode("utf-8", "replace"))
         _check(rc)

@@ -164,6 +164,10 @@ struct jam_context {
     DBuf o_xyz, o_aoff, o_zero;
     // per-pair hit counts and digests of jam_result_pair_digests
     DBuf dig;
+    // temporary buffers of jam_molecules_parse_pdb, kept for the next parse
+    DBuf g_buf, g_toff, g_nl, g_blk, g_blkoff, g_starts, g_kind, g_textof, g_first, g_lineatom, g_atomline, g_textrng;
+    DBuf g_err, g_firsterr, g_keys, g_nnames, g_resoff64, g_bucket, g_bcount, g_bstart, g_unsorted, g_scr;
+    std::vector<std::unique_ptr<DBuf>> g_scan;
 };
 
 struct jam_templates {
@@ -587,6 +591,13 @@ __global__ void k_fill_u64(unsigned long long *v, long long n, unsigned long lon
         v[i] = x;
 }
 
+// a line feed at each of the n positions of buf
+__global__ void k_put_newlines(const long long *pos, long long n, unsigned char *buf)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) buf[pos[i]] = '\n';
+}
+
 __global__ void k_fill_i32(int *v, long long n, int x)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -632,99 +643,102 @@ int32_t read_one(const T *dev, T &host, hipStream_t s)
 
 }  // namespace
 
-extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, const int64_t *text_off, int32_t n,
-                                           int32_t flags, jam_molecules **out)
+extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *const *texts, const int64_t *lengths,
+                                           int32_t n, int32_t flags, jam_molecules **out)
 {
-    if (!c || !out || !text_off || n < 0) return fail(JAM_EINVAL, "invalid argument");
+    if (!c || !out || n < 0 || (n > 0 && (!texts || !lengths))) return fail(JAM_EINVAL, "invalid argument");
     *out = nullptr;
-    if (text_off[0] != 0) return fail(JAM_EINVAL, "text_off[0] must be 0");
-    for (int32_t k = 0; k < n; k++)
-        if (text_off[k + 1] < text_off[k]) return fail(JAM_EINVAL, "text offsets decrease");
-    if (n > 0 && text_off[n] > 0 && !text) return fail(JAM_EINVAL, "NULL text");
     if (flags & ~(JAM_PDB_SKIP_HETATM | JAM_PDB_IGNORE_ENDMDL)) return fail(JAM_EINVAL, "unknown flags");
-    // the texts, each ending with a line feed
-    std::vector<long long> toff(n + 1, 0);
-    std::vector<unsigned char> buf;
-    buf.reserve((size_t)text_off[n] + (size_t)n);
+    // where the texts go on the device: each one's bytes, then a line feed
+    // when it does not end with one (its last line ends there)
+    std::vector<long long> toff(n + 1, 0), nl;
     for (int32_t k = 0; k < n; k++) {
-        toff[k] = (long long)buf.size();
-        buf.insert(buf.end(), (const unsigned char *)text + text_off[k], (const unsigned char *)text + text_off[k + 1]);
-        if (buf.size() > (size_t)toff[k] && buf.back() != '\n') buf.push_back('\n');
+        if (lengths[k] < 0 || (lengths[k] > 0 && !texts[k]))
+            return fail(JAM_EINVAL, "text " + std::to_string(k) + ": negative length or NULL text");
+        toff[k + 1] = toff[k] + lengths[k];
+        if (lengths[k] > 0 && texts[k][lengths[k] - 1] != '\n') nl.push_back(toff[k + 1]++);
     }
-    toff[n] = (long long)buf.size();
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     std::unique_ptr<jam_molecules> M(new jam_molecules());
     M->ctx = c;
     M->n_mols = n;
     M->parsed = true;
-    std::vector<std::unique_ptr<DBuf>> tmp;
-    DBuf d_buf, d_toff, d_blk, d_blkoff, d_starts, d_kind, d_textof, d_first, d_lineatom, d_atomline, d_textrng;
-    HIP_TRY(upload(d_buf, buf, s));
-    HIP_TRY(upload(d_toff, toff, s));
+    std::vector<std::unique_ptr<DBuf>> &tmp = c->g_scan;
+    HIP_TRY(c->g_buf.reserve((size_t)std::max<long long>(toff[n], 1)));
+    for (int32_t k = 0; k < n; k++)
+        if (lengths[k] > 0)
+            HIP_TRY(hipMemcpyAsync(c->g_buf.as<unsigned char>() + toff[k], texts[k], (size_t)lengths[k],
+                                   hipMemcpyHostToDevice, s));
+    if (!nl.empty()) {
+        HIP_TRY(upload(c->g_nl, nl, s));
+        hipLaunchKernelGGL(k_put_newlines, dim3((unsigned)((nl.size() + 255) / 256)), dim3(256), 0, s,
+                           c->g_nl.as<const long long>(), (long long)nl.size(), c->g_buf.as<unsigned char>());
+    }
+    HIP_TRY(upload(c->g_toff, toff, s));
     jam::IngText tx;
-    tx.buf = d_buf.as<const unsigned char>();
-    tx.n_bytes = (long long)buf.size();
-    tx.text_off = d_toff.as<const long long>();
+    tx.buf = c->g_buf.as<const unsigned char>();
+    tx.n_bytes = toff[n];
+    tx.text_off = c->g_toff.as<const long long>();
     tx.n_texts = n;
     tx.flags = flags;
     const long long per_block = (long long)ING_THREADS * ING_BPT;
     const long long byte_blocks = std::max<long long>(1, (tx.n_bytes + per_block - 1) / per_block);
     // ---- lines
-    HIP_TRY(d_blk.reserve(sizeof(long long) * (size_t)byte_blocks));
-    HIP_TRY(d_blkoff.reserve(sizeof(long long) * (size_t)(byte_blocks + 1)));
+    HIP_TRY(c->g_blk.reserve(sizeof(long long) * (size_t)byte_blocks));
+    HIP_TRY(c->g_blkoff.reserve(sizeof(long long) * (size_t)(byte_blocks + 1)));
     hipLaunchKernelGGL(jam::k_ing_count, dim3((unsigned)byte_blocks), dim3(ING_THREADS), 0, s, tx,
-                       d_blk.as<long long>());
+                       c->g_blk.as<long long>());
     int32_t rc;
-    if ((rc = device_scan<long long>(d_blk.as<long long>(), byte_blocks, d_blkoff.as<long long>(), tmp, 0, s)) != JAM_OK)
+    if ((rc = device_scan<long long>(c->g_blk.as<long long>(), byte_blocks, c->g_blkoff.as<long long>(), tmp, 0, s)) != JAM_OK)
         return rc;
     long long n_lines = 0;
-    if ((rc = read_one(d_blkoff.as<long long>() + byte_blocks, n_lines, s)) != JAM_OK) return rc;
+    if ((rc = read_one(c->g_blkoff.as<long long>() + byte_blocks, n_lines, s)) != JAM_OK) return rc;
     if (tx.n_bytes == 0) n_lines = 0;
     const long long lines_cap = std::max<long long>(n_lines, 1);
-    HIP_TRY(d_starts.reserve(sizeof(long long) * (size_t)lines_cap));
+    HIP_TRY(c->g_starts.reserve(sizeof(long long) * (size_t)lines_cap));
     if (n_lines > 0)
         hipLaunchKernelGGL(jam::k_ing_starts, dim3((unsigned)byte_blocks), dim3(ING_THREADS), 0, s, tx,
-                           d_blkoff.as<const long long>(), d_starts.as<long long>());
+                           c->g_blkoff.as<const long long>(), c->g_starts.as<long long>());
     // ---- record kinds, first ENDMDL / mmCIF tag per text
-    HIP_TRY(d_kind.reserve((size_t)lines_cap));
-    HIP_TRY(d_textof.reserve(sizeof(int) * (size_t)lines_cap));
-    HIP_TRY(d_first.reserve(sizeof(unsigned long long) * 2 * (size_t)std::max(n, 1)));
-    unsigned long long *first_endmdl = d_first.as<unsigned long long>();
+    HIP_TRY(c->g_kind.reserve((size_t)lines_cap));
+    HIP_TRY(c->g_textof.reserve(sizeof(int) * (size_t)lines_cap));
+    HIP_TRY(c->g_first.reserve(sizeof(unsigned long long) * 2 * (size_t)std::max(n, 1)));
+    unsigned long long *first_endmdl = c->g_first.as<unsigned long long>();
     unsigned long long *first_mmcif = first_endmdl + std::max(n, 1);
     hipLaunchKernelGGL(k_fill_u64, dim3(16), dim3(256), 0, s, first_endmdl, 2 * (long long)std::max(n, 1),
                        (unsigned long long)n_lines);
     const long long line_blocks = std::max<long long>(1, (n_lines + ING_THREADS - 1) / ING_THREADS);
     if (n_lines > 0)
         hipLaunchKernelGGL(jam::k_ing_classify, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
-                           d_starts.as<const long long>(), n_lines, d_kind.as<unsigned char>(), d_textof.as<int>(),
+                           c->g_starts.as<const long long>(), n_lines, c->g_kind.as<unsigned char>(), c->g_textof.as<int>(),
                            first_endmdl, first_mmcif);
     // ---- atom records
-    HIP_TRY(d_blk.reserve(sizeof(long long) * (size_t)line_blocks));
-    HIP_TRY(d_blkoff.reserve(sizeof(long long) * (size_t)(line_blocks + 1)));
-    HIP_TRY(d_lineatom.reserve(sizeof(long long) * (size_t)lines_cap));
+    HIP_TRY(c->g_blk.reserve(sizeof(long long) * (size_t)line_blocks));
+    HIP_TRY(c->g_blkoff.reserve(sizeof(long long) * (size_t)(line_blocks + 1)));
+    HIP_TRY(c->g_lineatom.reserve(sizeof(long long) * (size_t)lines_cap));
     long long n_atoms = 0;
     if (n_lines > 0) {
         hipLaunchKernelGGL(jam::k_ing_select_count, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
-                           d_kind.as<const unsigned char>(), d_textof.as<const int>(),
-                           (const unsigned long long *)first_endmdl, n_lines, d_blk.as<long long>());
-        if ((rc = device_scan<long long>(d_blk.as<long long>(), line_blocks, d_blkoff.as<long long>(), tmp, 0, s)) !=
+                           c->g_kind.as<const unsigned char>(), c->g_textof.as<const int>(),
+                           (const unsigned long long *)first_endmdl, n_lines, c->g_blk.as<long long>());
+        if ((rc = device_scan<long long>(c->g_blk.as<long long>(), line_blocks, c->g_blkoff.as<long long>(), tmp, 0, s)) !=
             JAM_OK)
             return rc;
-        if ((rc = read_one(d_blkoff.as<long long>() + line_blocks, n_atoms, s)) != JAM_OK) return rc;
+        if ((rc = read_one(c->g_blkoff.as<long long>() + line_blocks, n_atoms, s)) != JAM_OK) return rc;
     }
     if (n_atoms >= INT32_MAX) return fail(JAM_ETOOBIG, "too many atoms in one batch");
     const long long atoms_cap = std::max<long long>(n_atoms, 1);
-    HIP_TRY(d_atomline.reserve(sizeof(long long) * (size_t)atoms_cap));
+    HIP_TRY(c->g_atomline.reserve(sizeof(long long) * (size_t)atoms_cap));
     if (n_lines > 0)
         hipLaunchKernelGGL(jam::k_ing_select, dim3((unsigned)line_blocks), dim3(ING_THREADS), 0, s, tx,
-                           d_kind.as<const unsigned char>(), d_textof.as<const int>(),
-                           (const unsigned long long *)first_endmdl, n_lines, d_blkoff.as<const long long>(),
-                           d_atomline.as<long long>(), d_lineatom.as<long long>());
-    HIP_TRY(d_textrng.reserve(sizeof(long long) * 2 * (size_t)(n + 1)));
-    long long *text_line = d_textrng.as<long long>(), *text_atom = text_line + (n + 1);
+                           c->g_kind.as<const unsigned char>(), c->g_textof.as<const int>(),
+                           (const unsigned long long *)first_endmdl, n_lines, c->g_blkoff.as<const long long>(),
+                           c->g_atomline.as<long long>(), c->g_lineatom.as<long long>());
+    HIP_TRY(c->g_textrng.reserve(sizeof(long long) * 2 * (size_t)(n + 1)));
+    long long *text_line = c->g_textrng.as<long long>(), *text_atom = text_line + (n + 1);
     hipLaunchKernelGGL(jam::k_ing_text_ranges, dim3((unsigned)((n + 1 + 63) / 64)), dim3(64), 0, s, tx,
-                       d_starts.as<const long long>(), n_lines, d_lineatom.as<const long long>(), n_atoms, text_line,
+                       c->g_starts.as<const long long>(), n_lines, c->g_lineatom.as<const long long>(), n_atoms, text_line,
                        text_atom);
     HIP_TRY(hipGetLastError());
     std::vector<long long> h_rng(2 * (size_t)(n + 1)), h_first(2 * (size_t)std::max(n, 1));
@@ -757,10 +771,9 @@ extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, con
     HIP_TRY(M->segid.reserve(4 * na));
     HIP_TRY(M->element.reserve(2 * na));
     HIP_TRY(M->charge.reserve(sizeof(int) * na));
-    DBuf d_err, d_firsterr;
-    HIP_TRY(d_err.reserve(sizeof(unsigned int) * na));
-    HIP_TRY(d_firsterr.reserve(sizeof(unsigned long long)));
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, d_firsterr.as<unsigned long long>(), 1ll,
+    HIP_TRY(c->g_err.reserve(sizeof(unsigned int) * na));
+    HIP_TRY(c->g_firsterr.reserve(sizeof(unsigned long long)));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, c->g_firsterr.as<unsigned long long>(), 1ll,
                        (unsigned long long)n_atoms);
     jam::IngAtoms at;
     at.xyz = M->xyz.as<double>();
@@ -777,20 +790,20 @@ extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, con
     at.segid = M->segid.as<uint32_t>();
     at.element = M->element.as<unsigned short>();
     at.charge = M->charge.as<int>();
-    at.err = d_err.as<unsigned int>();
-    at.first_err = d_firsterr.as<unsigned long long>();
+    at.err = c->g_err.as<unsigned int>();
+    at.first_err = c->g_firsterr.as<unsigned long long>();
     if (n_atoms > 0)
         hipLaunchKernelGGL(jam::k_ing_parse, dim3((unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS)),
-                           dim3(ING_THREADS), 0, s, tx, d_starts.as<const long long>(), n_lines,
-                           d_atomline.as<const long long>(), n_atoms, at);
+                           dim3(ING_THREADS), 0, s, tx, c->g_starts.as<const long long>(), n_lines,
+                           c->g_atomline.as<const long long>(), n_atoms, at);
     HIP_TRY(hipGetLastError());
     unsigned long long first_err = 0;
-    if ((rc = read_one(d_firsterr.as<unsigned long long>(), first_err, s)) != JAM_OK) return rc;
+    if ((rc = read_one(c->g_firsterr.as<unsigned long long>(), first_err, s)) != JAM_OK) return rc;
     if ((long long)first_err < n_atoms) {
         long long li = 0;
         unsigned int kind = 0;
-        HIP_TRY(hipMemcpyAsync(&li, d_atomline.as<long long>() + first_err, sizeof(li), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(&kind, d_err.as<unsigned int>() + first_err, sizeof(kind), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&li, c->g_atomline.as<long long>() + first_err, sizeof(li), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&kind, c->g_err.as<unsigned int>() + first_err, sizeof(kind), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         int t = 0;
         while (t + 1 < n && h_rng[t + 1] <= li) t++;
@@ -801,15 +814,14 @@ extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, con
                                     ")");
     }
     // ---- residue-name index
-    DBuf d_keys, d_nnames, d_resoff64, d_bucket, d_bcount, d_bstart, d_unsorted, d_scr;
-    HIP_TRY(d_keys.reserve(sizeof(uint32_t) * ING_MAX_NAMES * (size_t)std::max(n, 1)));
-    HIP_TRY(d_nnames.reserve(sizeof(int) * (size_t)std::max(n, 1)));
+    HIP_TRY(c->g_keys.reserve(sizeof(uint32_t) * ING_MAX_NAMES * (size_t)std::max(n, 1)));
+    HIP_TRY(c->g_nnames.reserve(sizeof(int) * (size_t)std::max(n, 1)));
     if (n > 0)
         hipLaunchKernelGGL(jam::k_ing_names, dim3((unsigned)n), dim3(1024), 0, s, M->atom_off.as<const int>(),
-                           M->resname.as<const uint32_t>(), d_keys.as<uint32_t>(), d_nnames.as<int>());
+                           M->resname.as<const uint32_t>(), c->g_keys.as<uint32_t>(), c->g_nnames.as<int>());
     HIP_TRY(hipGetLastError());
     std::vector<int> nnames(std::max(n, 1), 0);
-    HIP_TRY(hipMemcpyAsync(nnames.data(), d_nnames.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(nnames.data(), c->g_nnames.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     std::vector<long long> resoff64(n + 1, 0);
     std::vector<int> resoff(n + 1, 0);
@@ -824,36 +836,36 @@ extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *text, con
     M->n_buckets = nb;
     const size_t nbc = (size_t)std::max<long long>(nb, 1);
     HIP_TRY(upload(M->res_off, resoff, s));
-    HIP_TRY(upload(d_resoff64, resoff64, s));
+    HIP_TRY(upload(c->g_resoff64, resoff64, s));
     HIP_TRY(M->res_key.reserve(sizeof(uint32_t) * nbc));
     HIP_TRY(M->res_begin.reserve(sizeof(int) * nbc));
     HIP_TRY(M->res_end.reserve(sizeof(int) * nbc));
     HIP_TRY(M->perm.reserve(sizeof(int) * na));
     if (n > 0)
-        hipLaunchKernelGGL(jam::k_ing_keys, dim3((unsigned)n), dim3(256), 0, s, d_keys.as<const uint32_t>(),
-                           d_nnames.as<const int>(), d_resoff64.as<const long long>(), n, M->res_key.as<uint32_t>());
-    HIP_TRY(d_bucket.reserve(sizeof(int) * na));
-    HIP_TRY(d_bcount.reserve(sizeof(int) * nbc));
-    HIP_TRY(d_bstart.reserve(sizeof(long long) * (nbc + 1)));
-    HIP_TRY(d_unsorted.reserve(sizeof(int) * nbc));
-    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, d_bcount.as<int>(), (long long)nbc, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, d_unsorted.as<int>(), (long long)nbc, 0);
+        hipLaunchKernelGGL(jam::k_ing_keys, dim3((unsigned)n), dim3(256), 0, s, c->g_keys.as<const uint32_t>(),
+                           c->g_nnames.as<const int>(), c->g_resoff64.as<const long long>(), n, M->res_key.as<uint32_t>());
+    HIP_TRY(c->g_bucket.reserve(sizeof(int) * na));
+    HIP_TRY(c->g_bcount.reserve(sizeof(int) * nbc));
+    HIP_TRY(c->g_bstart.reserve(sizeof(long long) * (nbc + 1)));
+    HIP_TRY(c->g_unsorted.reserve(sizeof(int) * nbc));
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, c->g_bcount.as<int>(), (long long)nbc, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, c->g_unsorted.as<int>(), (long long)nbc, 0);
     if (n_atoms > 0) {
         hipLaunchKernelGGL(jam::k_ing_bucket, dim3((unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS)),
                            dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), n, n_atoms,
                            M->resname.as<const uint32_t>(), M->res_key.as<const uint32_t>(),
-                           d_resoff64.as<const long long>(), d_bucket.as<int>(), d_bcount.as<int>());
-        if ((rc = device_scan<int>(d_bcount.as<int>(), nb, d_bstart.as<long long>(), tmp, 0, s)) != JAM_OK) return rc;
+                           c->g_resoff64.as<const long long>(), c->g_bucket.as<int>(), c->g_bcount.as<int>());
+        if ((rc = device_scan<int>(c->g_bcount.as<int>(), nb, c->g_bstart.as<long long>(), tmp, 0, s)) != JAM_OK) return rc;
         const long long waves_per_block = ING_THREADS / JAM_WAVE;
         hipLaunchKernelGGL(jam::k_ing_partition, dim3((unsigned)((nb + waves_per_block - 1) / waves_per_block)),
-                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), d_resoff64.as<const long long>(), n,
-                           nb, d_bstart.as<const long long>(), d_bucket.as<const int>(), M->serial.as<const int>(),
-                           M->perm.as<int>(), M->res_begin.as<int>(), M->res_end.as<int>(), d_unsorted.as<int>());
-        HIP_TRY(d_scr.reserve(sizeof(unsigned long long) * 2 * na));
+                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), c->g_resoff64.as<const long long>(), n,
+                           nb, c->g_bstart.as<const long long>(), c->g_bucket.as<const int>(), M->serial.as<const int>(),
+                           M->perm.as<int>(), M->res_begin.as<int>(), M->res_end.as<int>(), c->g_unsorted.as<int>());
+        HIP_TRY(c->g_scr.reserve(sizeof(unsigned long long) * 2 * na));
         hipLaunchKernelGGL(jam::k_ing_fixup, dim3((unsigned)nb), dim3(1024), 0, s, M->atom_off.as<const int>(),
-                           d_resoff64.as<const long long>(), n, nb, d_unsorted.as<const int>(),
+                           c->g_resoff64.as<const long long>(), n, nb, c->g_unsorted.as<const int>(),
                            M->res_begin.as<const int>(), M->res_end.as<const int>(), M->serial.as<const int>(),
-                           M->perm.as<int>(), d_scr.as<unsigned long long>());
+                           M->perm.as<int>(), c->g_scr.as<unsigned long long>());
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));

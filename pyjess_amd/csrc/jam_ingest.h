@@ -481,28 +481,46 @@ __global__ void k_ing_keys(const uint32_t *keys, const int *n_names, const long 
         res_key[res_off[m] + k] = keys[(size_t)m * ING_MAX_NAMES + k];
 }
 
-// bucket (global index) of every atom, and the bucket sizes
+// bucket (global index) of every atom, and the bucket sizes: one atomic add
+// per distinct bucket of a wave (atoms of a residue are adjacent, so a wave
+// holds few distinct names) instead of one per atom on a few hot words
 __global__ void __launch_bounds__(ING_THREADS) k_ing_bucket(const int *atom_off, int n_mols, long long n_atoms,
                                                            const uint32_t *resname, const uint32_t *res_key,
                                                            const long long *res_off, int *bucket, int *bucket_count)
 {
     const long long a = (long long)blockIdx.x * ING_THREADS + threadIdx.x;
-    if (a >= n_atoms) return;
-    int lo = 0, hi = n_mols;   // molecule of atom a: the last m with atom_off[m] <= a
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (atom_off[mid] <= a) lo = mid;
-        else hi = mid;
+    const int lane = lane_id();
+    long long b0 = 0;
+    bool pending = a < n_atoms;
+    if (pending) {
+        int lo = 0, hi = n_mols;   // molecule of atom a: the last m with atom_off[m] <= a
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (atom_off[mid] <= a) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t key = jam_upper4(resname[a]);
+        long long b1 = res_off[lo + 1];
+        b0 = res_off[lo];
+        while (b0 < b1) {
+            const long long mid = (b0 + b1) >> 1;
+            if (res_key[mid] < key) b0 = mid + 1;
+            else b1 = mid;
+        }
+        bucket[a] = (int)b0;
     }
-    const uint32_t key = jam_upper4(resname[a]);
-    long long b0 = res_off[lo], b1 = res_off[lo + 1];
-    while (b0 < b1) {
-        const long long mid = (b0 + b1) >> 1;
-        if (res_key[mid] < key) b0 = mid + 1;
-        else b1 = mid;
+    // every lane takes part in the ballots and shuffles; each round retires
+    // the lanes of the lowest pending lane's bucket (at most 64 rounds)
+    for (int round = 0; round < JAM_WAVE; round++) {
+        const unsigned long long m = ballot(pending);
+        if (m == 0ull) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const long long lb = __shfl(b0, leader, JAM_WAVE);
+        const bool same = pending && b0 == lb;
+        const unsigned long long peers = ballot(same);
+        if (lane == leader) atomicAdd(&bucket_count[lb], __popcll(peers));
+        if (same) pending = false;
     }
-    bucket[a] = (int)b0;
-    atomicAdd(&bucket_count[b0], 1);
 }
 
 // Wave per bucket: the bucket's atoms in input order at consecutive positions

@@ -171,3 +171,64 @@ def test_replacing_an_array_drops_device_copies():
     assert mol._device == {}
     assert mol.__dict__["_version"] == v0 + 1
     assert not mol.xyz.flags.writeable
+
+
+def _dumps_atom_by_atom(mol):
+    return "".join(atom.dumps() for atom in mol)
+
+
+def test_molecule_dumps_equals_atom_dumps(mol_1amy):
+    """Molecule.dumps writes every atom's columns at once; its text is the
+    join of Atom.dumps, rows with numbers too wide for their columns, NaN
+    or rounding ties included (those are formatted one by one)."""
+    from pyjess_amd import synthetic
+
+    assert mol_1amy.dumps(write_header=False) == _dumps_atom_by_atom(mol_1amy)
+    rng = np.random.Generator(np.random.PCG64(5))
+    m = synthetic.protein(rng, 120)
+    n = len(m)
+    xyz = np.round(rng.uniform(-999, 9999, (n, 3)), 3)
+    xyz[:20, 0] = [0.0, -0.0, -0.0004, -0.0005, 0.0005, 0.0015, 1.0005, 2.0625, -0.0625, 9999.9994,
+                   9999.9996, -999.9994, -999.9996, np.nan, np.inf, -np.inf, 1e20, 123.4565, 5e-324, 12345.5]
+    serial = np.arange(n, dtype=np.int32) * 997 - 50
+    resseq = m.resseq.copy()
+    resseq[:4] = [-999, -1000, 9999, 10000]
+    occ = rng.uniform(0, 200, n).astype(np.float32)
+    occ[:3] = [0.125, np.nan, 1000.0]
+    m2 = Molecule.from_arrays(xyz, m.names, m.resnames, m.chains, resseq, serial, id="ODD1",
+                              occupancy=occ, tempfactor=rng.uniform(-99, 999, n))
+    text = m2.dumps()
+    assert text.splitlines(keepends=True)[0].startswith("HEADER")
+    assert text.split("\n", 1)[1] == _dumps_atom_by_atom(m2)
+    assert Molecule().dumps() == Molecule().dumps(write_header=True)
+    assert Molecule().dumps(write_header=False) == ""
+
+
+@pytest.mark.parametrize("ignore_endmdl", [False, True])
+def test_header_fields_from_bytes_equal_the_text_parse(tmp_path, ignore_endmdl):
+    """The device loader takes id, date and name from the bytes of a text
+    (no full decode); they equal what the line-by-line parser reads, line
+    ends of every kind, a HEADER after ENDMDL and tags inside lines included."""
+    from pyjess_amd._model import _pdb_header
+
+    atom = "ATOM      1  N   ALA A   1      11.104   6.134  -6.504  1.00  0.00           N  "
+    h1 = "HEADER    HYDROLASE                               01-JAN-99   1ABC              "
+    h2 = "HEADER    OTHER NAME                              02-FEB-03   2XYZ"
+    texts = [
+        h1 + "\n" + atom + "\n",
+        atom + "\r" + h1 + "\r\n" + atom + "\r",
+        atom + "\nENDMDL\n" + h1 + "\n",
+        "REMARK HEADER not a record\n" + atom + "\n" + "xHEADER\n",
+        "",
+        h2,
+        "HEADER\n" + h1 + "\n",
+    ]
+    for k, text in enumerate(texts):
+        path = tmp_path / f"t{k}.pdb"
+        path.write_bytes(text.encode())
+        host = Molecule.load(path, device=False, ignore_endmdl=ignore_endmdl)
+        got = _pdb_header(text.encode(), None, None, None, ignore_endmdl)
+        assert got == (host.id, host.date, host.name), text
+    assert _pdb_header(h1.encode(), "X", None, "N", False)[::2] == ("X", "N")
+    with pytest.raises(UnicodeDecodeError):
+        _pdb_header(b"REMARK \xff\n", None, None, None, False)
