@@ -622,21 +622,35 @@ def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = n_sets * steps / elapsed
-    # the two kernels alone, HIP events on the stream they run on (torch's)
+    # the two kernels alone: jam_rmsd_forward / jam_rmsd_backward called
+    # through the C ABI (no offset check, no torch reduction) between HIP
+    # events on the stream they are launched on (torch's current stream)
+    import ctypes
+
+    from pyjess_amd import _native
+
+    L = _native.lib()
+    rm = torch.empty(n_sets, dtype=torch.float64, device=dev)
+    rot = torch.empty((n_sets, 9), dtype=torch.float64, device=dev)
+    gr = torch.ones(n_sets, dtype=torch.float64, device=dev)
+    ga_k = torch.empty_like(ta)
+    gb_k = torch.empty_like(tb)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    ptr = lambda t: ctypes.c_void_p(t.data_ptr())
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     fwd_ms, bwd_ms = [], []
-    for _ in range(steps):
-        ta.grad = None
-        tb.grad = None
+    for _ in range(steps + 1):
         ev[0].record()
-        r = superposition_rmsd(ta, tb, toff)
+        _native._check(L.jam_rmsd_forward(ptr(ta), ptr(tb), ptr(toff), n_sets, n_pts, ptr(rm), ptr(rot), stream))
         ev[1].record()
-        r.sum().backward()
+        _native._check(L.jam_rmsd_backward(ptr(ta), ptr(tb), ptr(toff), n_sets, n_pts, ptr(rm), ptr(rot), ptr(gr),
+                                           ptr(ga_k), ptr(gb_k), stream))
         ev[2].record()
         torch.cuda.synchronize()
         fwd_ms.append(ev[0].elapsed_time(ev[1]))
         bwd_ms.append(ev[1].elapsed_time(ev[2]))
-    fwd_ms, bwd_ms = float(np.median(fwd_ms)), float(np.median(bwd_ms))
+    fwd_ms, bwd_ms = float(np.median(fwd_ms[1:])), float(np.median(bwd_ms[1:]))
+    del rm, rot, gr, ga_k, gb_k
     # algorithmic HBM bytes: forward reads a, b and the offsets, writes the
     # RMSD and the rotation (kept for the backward); backward reads a, b,
     # offsets, RMSD, rotation and upstream gradient, writes both gradients
@@ -645,7 +659,7 @@ def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
     bwd_gbs = bwd_bytes / (bwd_ms / 1000.0) / 1e9
     log(f"differentiation: {value:,.0f} matches/s forward+backward ({1000 * elapsed / steps:.3f} ms/step); "
         f"forward {fwd_ms:.3f} ms ({fwd_bytes / fwd_ms / 1e6:.0f} GB/s), backward {bwd_ms:.3f} ms "
-        f"({bwd_gbs:.0f} GB/s, includes torch's sum backward)")
+        f"({bwd_gbs:.0f} GB/s), the kernels alone")
     out = {
         "value": value,
         "unit": "matches/s (forward+backward)",
@@ -728,7 +742,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="library-64",
                     help="library-64 (the headline, with the other workloads beside it) | plain-2k | plain-8k | "
-                         "batch-4k | rmsd-grad-1m | ingest-8 (one workload alone)")
+                         "batch-4k | rmsd-grad-1m | ingest-8 (one workload alone) | copy-1g (the counter "
+                         "calibration copy)")
     ap.add_argument("--plain-steps", type=int, default=200,
                     help="timed steps of the plain-8k measurement beside the headline (0: skip it)")
     ap.add_argument("--parity-first-chunk", action="store_true",
@@ -764,6 +779,25 @@ def main():
                "dtype": "f64", "data": "synthetic", "config": dict(d["config"], parallelism="1 GPU"),
                "roofline": d["roofline"], "cpu_baseline": d["cpu_baseline"], "differentiation": d}
         print(json.dumps(out), flush=True)
+        return
+    if args.workload == "copy-1g":
+        # a plain device copy of 1 GiB: the reference for scaling the HBM
+        # counters (FETCH_SIZE, WRITE_SIZE) of the profiles to bytes
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * src.numel() * 8 / dt / 1e9,
+                          "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": dt * 1e3,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": "copy-1g", "bytes": src.numel() * 8}}),
+              flush=True)
         return
     if args.workload == "ingest-8":
         # the PDB ingestion of one chunk of library assemblies alone

@@ -15,6 +15,7 @@ library (``jam_rmsd_forward`` / ``jam_rmsd_backward``).
 from __future__ import annotations
 
 import ctypes
+import weakref
 from typing import Optional
 
 import torch
@@ -43,17 +44,31 @@ def _check_inputs(a: torch.Tensor, b: torch.Tensor, offsets: torch.Tensor) -> No
     _check_offsets(offsets, a.shape[0])
 
 
+# the offsets tensor last found valid: (weak reference, its version counter,
+# pointer, size and the number of points it was checked against)
+_last_valid = None
+
+
 def _check_offsets(offsets: torch.Tensor, n_points: int) -> None:
     """Every row belongs to exactly one set: the offsets start at 0, never
     decrease and end at the number of points (one read of a flag from the
-    device), so that the kernels write every gradient row."""
+    device), so that the kernels write every gradient row. A call with the
+    same offsets tensor as the last valid one, not modified since (torch's
+    version counter, which in-place writes through any view advance), and
+    the same number of points skips the device read."""
+    global _last_valid
     if offsets.dtype != torch.int64 or offsets.dim() != 1 or offsets.numel() < 1:
         raise ValueError("`offsets` must be a 1-D int64 tensor of n_sets + 1 entries")
+    key = (offsets._version, offsets.data_ptr(), offsets.numel(), n_points)
+    last = _last_valid
+    if last is not None and last[0]() is offsets and last[1] == key:
+        return
     ok = (offsets[0] == 0) & (offsets[-1] == n_points)
     if offsets.numel() > 1:
         ok = ok & torch.all(offsets[1:] >= offsets[:-1])
     if not bool(ok):
         raise ValueError("`offsets` must start at 0, never decrease and end at the number of points")
+    _last_valid = (weakref.ref(offsets), key)
 
 
 class SuperpositionRMSD(torch.autograd.Function):

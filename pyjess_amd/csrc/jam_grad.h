@@ -261,6 +261,32 @@ __device__ __attribute__((always_inline)) static inline void rmsd_grad_one(int n
 #define RG_SETS 64
 #define RG_CAP 768   // points per tile held in LDS (12 per set on average)
 
+// Copies m doubles of a and of b into LDS, coalesced: every lane issues the
+// loads of RG_BATCH elements of each array before its LDS stores, so that a
+// wave waits for HBM once per batch rather than once per element (the
+// kernels run one wave per SIMD, their LDS tiles allowing no more).
+#define RG_BATCH 12
+__device__ static inline void stage_pair(const double *a, const double *b, int m, double *la, double *lb, int lane)
+{
+    for (int q0 = 0; q0 < m; q0 += RG_SETS * RG_BATCH) {
+        double ra[RG_BATCH], rb[RG_BATCH];
+#pragma unroll
+        for (int u = 0; u < RG_BATCH; u++) {
+            const int q = q0 + lane + RG_SETS * u;
+            ra[u] = q < m ? a[q] : 0.0;
+            rb[u] = q < m ? b[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RG_BATCH; u++) {
+            const int q = q0 + lane + RG_SETS * u;
+            if (q < m) {
+                la[q] = ra[u];
+                lb[q] = rb[u];
+            }
+        }
+    }
+}
+
 // Range of set s, clamped into [lo, hi]: with valid offsets (off[0] = 0,
 // never decreasing, off[n_sets] = n_points) this is [off[s], off[s+1]);
 // malformed offsets give meaningless values, never an access outside a and b.
@@ -290,11 +316,7 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_fwd(const double *a, const dou
     set_range(off, s1 - 1, P0, n_points, q1, P1);
     const bool staged = P1 - P0 <= RG_CAP;
     if (staged) {
-        const int m = (int)(3 * (P1 - P0));
-        for (int q = lane; q < m; q += RG_SETS) {
-            la[q] = a[3 * P0 + q];
-            lb[q] = b[3 * P0 + q];
-        }
+        stage_pair(a + 3 * P0, b + 3 * P0, (int)(3 * (P1 - P0)), la, lb, lane);
         wave_sync();
     }
     const long long s = s0 + lane;
@@ -331,10 +353,7 @@ __global__ void __launch_bounds__(RG_SETS) k_rmsd_bwd(const double *a, const dou
     const bool staged = P1 - P0 <= RG_CAP;
     const int m = (int)(3 * (P1 - P0));
     if (staged) {
-        for (int q = lane; q < m; q += RG_SETS) {
-            la[q] = a[3 * P0 + q];
-            lb[q] = b[3 * P0 + q];
-        }
+        stage_pair(a + 3 * P0, b + 3 * P0, m, la, lb, lane);
         wave_sync();
     }
     const long long s = s0 + lane;

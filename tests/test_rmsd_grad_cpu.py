@@ -130,3 +130,22 @@ def test_valid_offsets_pass_and_host_tensors_are_refused():
     with pytest.raises(TypeError):
         autograd.superposition_rmsd(torch.zeros((2, 3, 3), dtype=torch.float64),
                                     torch.zeros((2, 3, 3), dtype=torch.float64))
+
+
+def test_unchanged_offsets_are_checked_once_and_changes_are_seen():
+    """The last valid offsets tensor is not read again while unmodified; an
+    in-place write (through a view too), another point count or another
+    tensor is checked again."""
+    from pyjess_amd import autograd
+
+    off = torch.tensor([0, 3, 3, 7], dtype=torch.int64)
+    autograd._check_offsets(off, 7)
+    autograd._check_offsets(off, 7)   # cached
+    with pytest.raises(ValueError):
+        autograd._check_offsets(off, 8)
+    off[1:3].fill_(9)                 # in place, through a view: now decreasing
+    with pytest.raises(ValueError):
+        autograd._check_offsets(off, 7)
+    other = torch.tensor([0, 5, 2, 7], dtype=torch.int64)
+    with pytest.raises(ValueError):
+        autograd._check_offsets(other, 7)
