@@ -9,7 +9,6 @@
 // chain, so results never depend on capacities.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -29,6 +28,7 @@
 #include "jam_grad.h"
 #include "jam_ingest.h"
 #include "jam_kernels.h"
+#include "jam_sort.h"
 
 namespace {
 
@@ -139,11 +139,13 @@ struct jam_context {
     hipEvent_t kd_fast[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
     bool kd_pending = false;    // side-stream work of the last kd launch may still run
     // sort of the complete matches
-    DBuf s_key[2], s_perm[2], s_pkey[2], s_tmp;
+    DBuf s_key[2], s_perm[2], s_pkey[2];
+    // radix-sort histograms and their scan; scan temporaries of the query
+    DBuf rs_hist, rs_off;
+    std::vector<std::unique_ptr<DBuf>> q_scan;
     int64_t sort_cap = 0;
     DBuf e_all;  // the events of the last query, event_layout(count, stride)
     DBuf keep_d, keep_i;  // superpositions kept between the two k_super passes (few matches)
-    DBuf scan_tmp, scan_tmp2;
     int64_t set_cap = 0, blk_cap = 0, sblk_cap = 0, event_cap = 0;
     int nsets_cap = 0, npairs_cap = 0;
     int64_t items_cap = 0;
@@ -166,7 +168,7 @@ struct jam_context {
     DBuf dig;
     // temporary buffers of jam_molecules_parse_pdb, kept for the next parse
     DBuf g_buf, g_toff, g_nl, g_blk, g_blkoff, g_starts, g_kind, g_textof, g_first, g_lineatom, g_atomline, g_textrng;
-    DBuf g_err, g_firsterr, g_keys, g_nnames, g_resoff64, g_bucket, g_bcount, g_bstart, g_unsorted, g_scr;
+    DBuf g_err, g_firsterr, g_keys, g_nnames, g_resoff64, g_bucket, g_bcount, g_bstart, g_skey, g_sval;
     std::vector<std::unique_ptr<DBuf>> g_scan;
 };
 
@@ -553,11 +555,14 @@ extern "C" void jam_molecules_free(jam_molecules *m)
 namespace {
 
 constexpr int SCAN_BLOCK = 1024;
+// arrays of at most this many SCAN_BLOCK-element blocks are scanned by one
+// workgroup in one launch
+constexpr long long SCAN_ONE_BLOCKS = 32;
 
-// Exclusive scan of 1024-element blocks of `in` into `out` (64-bit), each
-// block's total into `sums`.
-template <class T>
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(const T *in, long long n, long long *out, long long *sums)
+// Exclusive scan of 1024-element blocks of `in` into `out`, each block's
+// total into `sums` (sums in 64 bits).
+template <class T, class O>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(const T *in, long long n, O *out, long long *sums)
 {
     __shared__ long long w[SCAN_BLOCK / JAM_WAVE];
     const long long i = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
@@ -575,14 +580,52 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_blocks(const T *in, long lo
         if (k < wv) base += w[k];
         total += w[k];
     }
-    if (i < n) out[i] = base + x - v;
+    if (i < n) out[i] = (O)(base + x - v);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(long long *out, long long n, const long long *offs)
+// The whole exclusive scan in one workgroup, SCAN_BLOCK elements at a time
+// with a running carry, the total into out[n]: for small arrays, where the
+// launches of the multi-block scan would dominate.
+template <class T, class O>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_one(const T *in, long long n, O *out)
+{
+    __shared__ long long w[SCAN_BLOCK / JAM_WAVE];
+    const int lane = jam::lane_id(), wv = threadIdx.x / JAM_WAVE;
+    long long carry = 0;
+    for (long long b0 = 0; b0 < n; b0 += SCAN_BLOCK) {
+        const long long i = b0 + threadIdx.x;
+        const long long v = i < n ? (long long)in[i] : 0;
+        long long x = v;
+        for (int sh = 1; sh < JAM_WAVE; sh <<= 1) {
+            const long long y = __shfl_up(x, sh, JAM_WAVE);
+            if (lane >= sh) x += y;
+        }
+        if (lane == JAM_WAVE - 1) w[wv] = x;
+        __syncthreads();
+        long long base = 0, total = 0;
+        for (int k = 0; k < SCAN_BLOCK / JAM_WAVE; k++) {
+            if (k < wv) base += w[k];
+            total += w[k];
+        }
+        if (i < n) out[i] = (O)(carry + base + x - v);
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = (O)carry;
+}
+
+template <class O>
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_add(O *out, long long n, const long long *offs)
 {
     const long long i = (long long)blockIdx.x * SCAN_BLOCK + threadIdx.x;
-    if (i < n) out[i] += offs[blockIdx.x];
+    if (i < n) out[i] += (O)offs[blockIdx.x];
+}
+
+template <class O>
+__global__ void k_scan_total(O *dst, const long long *src)
+{
+    if (threadIdx.x == 0) *dst = (O)*src;
 }
 
 __global__ void k_fill_u64(unsigned long long *v, long long n, unsigned long long x)
@@ -604,32 +647,83 @@ __global__ void k_fill_i32(int *v, long long n, int x)
         v[i] = x;
 }
 
+// *out = the largest of in[0, n) (0 when n = 0): one workgroup
+__global__ void __launch_bounds__(1024) k_max_i32(const int *in, long long n, int *out)
+{
+    __shared__ int w[1024 / JAM_WAVE];
+    int m = 0;
+    for (long long i = threadIdx.x; i < n; i += 1024) m = max(m, in[i]);
+    for (int sh = JAM_WAVE / 2; sh >= 1; sh >>= 1) m = max(m, __shfl_xor(m, sh, JAM_WAVE));
+    if (jam::lane_id() == 0) w[threadIdx.x / JAM_WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 1024 / JAM_WAVE; k++) m = max(m, w[k]);
+        *out = max(m, w[0]);
+    }
+}
+
 // out[0, n] = exclusive scan of in[0, n) followed by the total (n + 1
 // values), on stream s; temporary space from `tmp`, grown as needed
-template <class T>
-int32_t device_scan(const T *in, long long n, long long *out, std::vector<std::unique_ptr<DBuf>> &tmp, size_t level,
+template <class T, class O>
+int32_t device_scan(const T *in, long long n, O *out, std::vector<std::unique_ptr<DBuf>> &tmp, size_t level,
                     hipStream_t s)
 {
     const long long blocks = std::max<long long>(1, (n + SCAN_BLOCK - 1) / SCAN_BLOCK);
-    if (tmp.size() <= level) tmp.emplace_back(new DBuf());
+    if (blocks <= SCAN_ONE_BLOCKS) {
+        hipLaunchKernelGGL((k_scan_one<T, O>), dim3(1), dim3(SCAN_BLOCK), 0, s, in, n, out);
+        HIP_TRY(hipGetLastError());
+        return JAM_OK;
+    }
+    while (tmp.size() <= level + 1) tmp.emplace_back(new DBuf());
     HIP_TRY(tmp[level]->reserve(sizeof(long long) * (size_t)(blocks + 1)));
     long long *sums = tmp[level]->as<long long>();
-    hipLaunchKernelGGL(k_scan_blocks<T>, dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, in, n, out, sums);
+    hipLaunchKernelGGL((k_scan_blocks<T, O>), dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, in, n, out, sums);
     HIP_TRY(hipGetLastError());
     if (blocks == 1) {
         // the total goes after the scan
-        HIP_TRY(hipMemcpyAsync(out + n, sums, sizeof(long long), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_scan_total<O>, dim3(1), dim3(64), 0, s, out + n, (const long long *)sums);
+        HIP_TRY(hipGetLastError());
         return JAM_OK;
     }
     // scan of the block totals (with their total at [blocks]), added back
-    if (tmp.size() <= level + 1) tmp.emplace_back(new DBuf());
     HIP_TRY(tmp[level + 1]->reserve(sizeof(long long) * (size_t)(blocks + 1)));
     long long *offs = tmp[level + 1]->as<long long>();
-    int32_t rc = device_scan<long long>(sums, blocks, offs, tmp, level + 2, s);
+    int32_t rc = device_scan<long long, long long>(sums, blocks, offs, tmp, level + 2, s);
     if (rc != JAM_OK) return rc;
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, out, n, offs);
+    hipLaunchKernelGGL(k_scan_add<O>, dim3((unsigned)blocks), dim3(SCAN_BLOCK), 0, s, out, n,
+                       (const long long *)offs);
+    hipLaunchKernelGGL(k_scan_total<O>, dim3(1), dim3(64), 0, s, out + n, (const long long *)(offs + blocks));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out + n, offs + blocks, sizeof(long long), hipMemcpyDeviceToDevice, s));
+    return JAM_OK;
+}
+
+// Stable LSD radix sort of n (key, value) pairs by bits [0, bits) of the
+// keys, 8 bits per pass, between the buffer pairs (k0, v0) -- the input --
+// and (k1, v1); `which` is set to the pair holding the result (0 or 1).
+template <class K>
+int32_t radix_sort(K *k0, int *v0, K *k1, int *v1, long long n, int bits, DBuf &hist, DBuf &offs,
+                   std::vector<std::unique_ptr<DBuf>> &tmp, hipStream_t s, int &which)
+{
+    which = 0;
+    if (n <= 1 || bits <= 0) return JAM_OK;
+    if (n >= INT32_MAX) return fail(JAM_ETOOBIG, "more than 2^31 keys in one sort");
+    const long long tiles = (n + RS_TILE - 1) / RS_TILE;
+    HIP_TRY(hist.reserve(sizeof(int) * (size_t)(RS_THREADS * tiles)));
+    HIP_TRY(offs.reserve(sizeof(int) * (size_t)(RS_THREADS * tiles + 1)));
+    K *ki = k0, *ko = k1;
+    int *vi = v0, *vo = v1;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(jam::k_rs_hist<K>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, (const K *)ki, n, shift,
+                           hist.as<int>(), tiles);
+        int32_t rc = device_scan<int, int>(hist.as<const int>(), RS_THREADS * tiles, offs.as<int>(), tmp, 0, s);
+        if (rc != JAM_OK) return rc;
+        hipLaunchKernelGGL(jam::k_rs_scatter<K>, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, (const K *)ki,
+                           (const int *)vi, n, shift, offs.as<const int>(), tiles, ko, vo);
+        HIP_TRY(hipGetLastError());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+        which ^= 1;
+    }
     return JAM_OK;
 }
 
@@ -847,25 +941,31 @@ extern "C" int32_t jam_molecules_parse_pdb(jam_context *c, const char *const *te
     HIP_TRY(c->g_bucket.reserve(sizeof(int) * na));
     HIP_TRY(c->g_bcount.reserve(sizeof(int) * nbc));
     HIP_TRY(c->g_bstart.reserve(sizeof(long long) * (nbc + 1)));
-    HIP_TRY(c->g_unsorted.reserve(sizeof(int) * nbc));
     hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, c->g_bcount.as<int>(), (long long)nbc, 0);
-    hipLaunchKernelGGL(k_fill_i32, dim3(64), dim3(256), 0, s, c->g_unsorted.as<int>(), (long long)nbc, 0);
     if (n_atoms > 0) {
-        hipLaunchKernelGGL(jam::k_ing_bucket, dim3((unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS)),
-                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), n, n_atoms,
-                           M->resname.as<const uint32_t>(), M->res_key.as<const uint32_t>(),
+        const unsigned ablocks = (unsigned)((n_atoms + ING_THREADS - 1) / ING_THREADS);
+        hipLaunchKernelGGL(jam::k_ing_bucket, dim3(ablocks), dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), n,
+                           n_atoms, M->resname.as<const uint32_t>(), M->res_key.as<const uint32_t>(),
                            c->g_resoff64.as<const long long>(), c->g_bucket.as<int>(), c->g_bcount.as<int>());
         if ((rc = device_scan<int>(c->g_bcount.as<int>(), nb, c->g_bstart.as<long long>(), tmp, 0, s)) != JAM_OK) return rc;
-        const long long waves_per_block = ING_THREADS / JAM_WAVE;
-        hipLaunchKernelGGL(jam::k_ing_partition, dim3((unsigned)((nb + waves_per_block - 1) / waves_per_block)),
-                           dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), c->g_resoff64.as<const long long>(), n,
-                           nb, c->g_bstart.as<const long long>(), c->g_bucket.as<const int>(), M->serial.as<const int>(),
-                           M->perm.as<int>(), M->res_begin.as<int>(), M->res_end.as<int>(), c->g_unsorted.as<int>());
-        HIP_TRY(c->g_scr.reserve(sizeof(unsigned long long) * 2 * na));
-        hipLaunchKernelGGL(jam::k_ing_fixup, dim3((unsigned)nb), dim3(1024), 0, s, M->atom_off.as<const int>(),
-                           c->g_resoff64.as<const long long>(), n, nb, c->g_unsorted.as<const int>(),
-                           M->res_begin.as<const int>(), M->res_end.as<const int>(), M->serial.as<const int>(),
-                           M->perm.as<int>(), c->g_scr.as<unsigned long long>());
+        // atoms sorted by (bucket, serial), stable
+        HIP_TRY(c->g_skey.reserve(sizeof(unsigned long long) * 2 * na));
+        HIP_TRY(c->g_sval.reserve(sizeof(int) * 2 * na));
+        unsigned long long *k0 = c->g_skey.as<unsigned long long>(), *k1 = k0 + na;
+        int *v0 = c->g_sval.as<int>(), *v1 = v0 + na;
+        hipLaunchKernelGGL(jam::k_ing_sortkeys, dim3(ablocks), dim3(ING_THREADS), 0, s, n_atoms,
+                           c->g_bucket.as<const int>(), M->serial.as<const int>(), k0, v0);
+        int bucket_bits = 0;
+        while ((1ll << bucket_bits) < nb) bucket_bits++;
+        int which = 0;
+        if ((rc = radix_sort<unsigned long long>(k0, v0, k1, v1, n_atoms, 32 + bucket_bits, c->rs_hist, c->rs_off, tmp,
+                                                 s, which)) != JAM_OK)
+            return rc;
+        hipLaunchKernelGGL(jam::k_ing_perm, dim3(ablocks), dim3(ING_THREADS), 0, s, M->atom_off.as<const int>(), n,
+                           n_atoms, (const int *)(which ? v1 : v0), M->perm.as<int>());
+        hipLaunchKernelGGL(jam::k_ing_ranges, dim3((unsigned)((nb + ING_THREADS - 1) / ING_THREADS)), dim3(ING_THREADS),
+                           0, s, M->atom_off.as<const int>(), c->g_resoff64.as<const long long>(), n, nb,
+                           c->g_bstart.as<const long long>(), M->res_begin.as<int>(), M->res_end.as<int>());
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
@@ -946,7 +1046,7 @@ int32_t grow_sets(jam_context *c, int64_t cap, int n_sets)
 {
     if (n_sets > c->nsets_cap) {
         HIP_TRY(c->cnt.reserve(sizeof(int) * (size_t)n_sets));
-        HIP_TRY(c->off.reserve(sizeof(int64_t) * (size_t)n_sets));
+        HIP_TRY(c->off.reserve(sizeof(int64_t) * ((size_t)n_sets + 1)));   // + the total
         HIP_TRY(c->kd_bucket.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
         HIP_TRY(c->kd_tied.reserve(sizeof(int) * KD_TIERS * (size_t)n_sets));
         c->nsets_cap = n_sets;
@@ -993,7 +1093,7 @@ int32_t grow_pairs(jam_context *c, int n_pairs)
     HIP_TRY(c->p_ord.reserve((size_t)JAM_MAX_LEVELS * n_pairs));
     HIP_TRY(c->p_lvl.reserve(sizeof(PairLevel) * (size_t)JAM_MAX_LEVELS * n_pairs));
     HIP_TRY(c->p_nchunk.reserve(sizeof(int) * (size_t)n_pairs));
-    HIP_TRY(c->p_base.reserve(sizeof(long long) * (size_t)n_pairs));
+    HIP_TRY(c->p_base.reserve(sizeof(long long) * ((size_t)n_pairs + 1)));   // + the total
     // [0]: event count, [1, n_pairs]: complete matches per pair, then the kd
     // passes' error word
     HIP_TRY(c->p_cm.reserve(sizeof(long long) * ((size_t)n_pairs + 2)));
@@ -1052,7 +1152,7 @@ int32_t grow_done(jam_context *c, int64_t rows, int stride)
     return JAM_OK;
 }
 
-// keys, permutations and the radix sort's temporary storage for n matches
+// keys and permutations of the sort of n matches
 int32_t grow_sort(jam_context *c, int64_t n)
 {
     if (n <= c->sort_cap) return JAM_OK;
@@ -1062,13 +1162,6 @@ int32_t grow_sort(jam_context *c, int64_t n)
         HIP_TRY(c->s_perm[b].reserve(sizeof(int) * (size_t)n));
         HIP_TRY(c->s_pkey[b].reserve(sizeof(unsigned int) * (size_t)n));
     }
-    size_t t1 = 0, t2 = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, t1, (const unsigned long long *)nullptr,
-                                               (unsigned long long *)nullptr, (const int *)nullptr, (int *)nullptr,
-                                               (int)n, 0, 64, c->stream));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, (const unsigned int *)nullptr, (unsigned int *)nullptr,
-                                               (const int *)nullptr, (int *)nullptr, (int)n, 0, 32, c->stream));
-    HIP_TRY(c->s_tmp.reserve(std::max(t1, t2)));
     c->sort_cap = n;
     return JAM_OK;
 }
@@ -1336,13 +1429,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
     HIP_TRY(c->j_count.reserve(sizeof(unsigned long long) * CNT_WORDS));
     DevMols dm = ms->dev();
     DevTpls dt = tp->dev();
-    size_t scan_sets = 0, scan_pairs = 0, scan_max = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_sets, (const int *)nullptr, (int64_t *)nullptr,
-                                             n_sets, s));
-    HIP_TRY(hipcub::DeviceReduce::Max(nullptr, scan_max, (const int *)nullptr, (int *)nullptr, n_sets, s));
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_pairs, (const int *)nullptr, (long long *)nullptr,
-                                             n_pairs, s));
-    HIP_TRY(c->scan_tmp.reserve(std::max(std::max(scan_sets, scan_pairs), scan_max)));
     const int pair_blocks = (n_pairs + 255) / 256;
 
     for (int attempt = 0; attempt < 8; attempt++) {
@@ -1383,12 +1469,12 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         hipLaunchKernelGGL(jam::k_candidates<false>, dim3(set_blocks), dim3(64 * wpb), 0, s, dm, dt, st, pr,
                            (long long *)nullptr);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_sets, st.cnt, st.off, n_sets, s));
-        HIP_TRY(hipcub::DeviceReduce::Max(c->scan_tmp.ptr, scan_max, st.cnt, st.maxcnt, n_sets, s));
+        if ((rc = device_scan<int, int64_t>(st.cnt, n_sets, st.off, c->q_scan, 0, s)) != JAM_OK) return rc;
+        hipLaunchKernelGGL(k_max_i32, dim3(1), dim3(1024), 0, s, (const int *)st.cnt, (long long)n_sets, st.maxcnt);
         hipLaunchKernelGGL(jam::k_pairs, dim3((unsigned)((n_pairs + 7) / 8)), dim3(256), 0, s, dt, st, pr, qp);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.ptr, scan_pairs, pr.nchunk, pr.item_base, n_pairs,
-                                                 s));
+        if ((rc = device_scan<int, long long>(pr.nchunk, n_pairs, pr.item_base, c->q_scan, 0, s)) != JAM_OK)
+            return rc;
         // no read-back here: the buffers keep the sizes of earlier queries,
         // every kernel below checks the sizes on the device, and the host
         // reads them with the join's counters, growing and running the query
@@ -1494,10 +1580,6 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         // levels first, then the pair
         if ((rc = grow_sort(c, n_done)) != JAM_OK) return rc;
         if ((rc = grow_events(c, n_done, stride)) != JAM_OK) return rc;
-        size_t scan_bytes = 0;
-        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int *)nullptr, (int *)nullptr,
-                                                 (int)std::max<int64_t>(n_done, 1), s));
-        HIP_TRY(c->s_tmp.reserve(scan_bytes));
         int cur = 0;
         const unsigned gs = (unsigned)((n_done + 255) / 256);
         int pbits = 1;
@@ -1532,24 +1614,29 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
             const int per = std::max(1, 64 / bits);
             const int nt = stride;
             hipLaunchKernelGGL(jam::k_iota, dim3(gs), dim3(256), 0, s, c->s_perm[0].as<int>(), (long long)n_done);
-            size_t tb = c->s_tmp.bytes;
             for (int l0 = ((nt - 1) / per) * per; l0 >= 0; l0 -= per) {
                 const int cnt_l = std::min(per, nt - l0);
                 // levels [l0, l0 + cnt_l) packed into cnt_l * bits bits
                 hipLaunchKernelGGL(jam::k_sortkey, dim3(gs), dim3(256), 0, s, st, dt, pr, jn,
                                    c->s_perm[cur].as<const int>(), (long long)n_done, l0, cnt_l, bits,
                                    c->s_key[0].as<unsigned long long>());
-                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-                    c->s_tmp.ptr, tb, c->s_key[0].as<unsigned long long>(), c->s_key[1].as<unsigned long long>(),
-                    c->s_perm[cur].as<int>(), c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, cnt_l * bits, s));
-                cur ^= 1;
+                int which = 0;
+                if ((rc = radix_sort<unsigned long long>(c->s_key[0].as<unsigned long long>(), c->s_perm[cur].as<int>(),
+                                                         c->s_key[1].as<unsigned long long>(),
+                                                         c->s_perm[cur ^ 1].as<int>(), (long long)n_done, cnt_l * bits,
+                                                         c->rs_hist, c->rs_off, c->q_scan, s, which)) != JAM_OK)
+                    return rc;
+                cur ^= which;
             }
             hipLaunchKernelGGL(jam::k_pairkey, dim3(gs), dim3(256), 0, s, jn, T, c->s_perm[cur].as<const int>(),
                                (long long)n_done, c->s_pkey[0].as<unsigned int>());
-            HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->s_tmp.ptr, tb, c->s_pkey[0].as<unsigned int>(),
-                                                       c->s_pkey[1].as<unsigned int>(), c->s_perm[cur].as<int>(),
-                                                       c->s_perm[cur ^ 1].as<int>(), (int)n_done, 0, pbits, s));
-            cur ^= 1;
+            int which = 0;
+            if ((rc = radix_sort<unsigned int>(c->s_pkey[0].as<unsigned int>(), c->s_perm[cur].as<int>(),
+                                               c->s_pkey[1].as<unsigned int>(), c->s_perm[cur ^ 1].as<int>(),
+                                               (long long)n_done, pbits, c->rs_hist, c->rs_off, c->q_scan, s,
+                                               which)) != JAM_OK)
+                return rc;
+            cur ^= which;
             HIP_TRY(hipGetLastError());
         }
         const int *perm = c->s_perm[cur].as<const int>();
@@ -1584,8 +1671,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                                c->p_first.as<long long>());
             hipLaunchKernelGGL(jam::k_super<false>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,
                                (long long)n_done, first, sel, (const int *)pos, keep_d, keep_i);
-            size_t tb = c->s_tmp.bytes;
-            HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->s_tmp.ptr, tb, (const int *)sel, pos, (int)n_done, s));
+            if ((rc = device_scan<int, int>((const int *)sel, (long long)n_done, pos, c->q_scan, 0, s)) != JAM_OK)
+                return rc;
             hipLaunchKernelGGL(jam::k_event_total, dim3(1), dim3(64), 0, s, (const int *)sel, (const int *)pos,
                                (long long)n_done, ev.count);
             hipLaunchKernelGGL(jam::k_super<true>, dim3(gs), dim3(256), 0, s, dm, dt, st, pr, jn, ev, qp, perm,

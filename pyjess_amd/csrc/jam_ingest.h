@@ -17,12 +17,11 @@
 //                               first ENDMDL and first mmCIF tag of every text
 //   k_ing_select                atom records in order (a scan of 0/1 flags)
 //   k_ing_parse                 thread per atom record: the fixed columns
-//   k_ing_names ... k_ing_fixup the residue-name index: distinct names per
+//   k_ing_names ... k_ing_ranges the residue-name index: distinct names per
 //                               molecule (LDS hash set, sorted), per-atom
-//                               bucket, a stable partition by bucket (wave per
-//                               bucket), and a sort by serial number inside
-//                               the buckets whose serials are not already in
-//                               input order
+//                               bucket, then one stable radix sort of the
+//                               atoms by (bucket, serial) and the buckets'
+//                               ranges
 // Numbers are parsed exactly: a decimal field of at most 19 significant
 // digits and a power of ten up to 22 is one correctly rounded division or
 // multiplication of two exact doubles (Clinger's fast path), which is what
@@ -523,103 +522,51 @@ __global__ void __launch_bounds__(ING_THREADS) k_ing_bucket(const int *atom_off,
     }
 }
 
-// Wave per bucket: the bucket's atoms in input order at consecutive positions
-// from its start (a stable partition by bucket); res_begin / res_end are the
-// bucket's range in its molecule's part of perm. `unsorted[b]` is set when the
-// bucket's serial numbers decrease somewhere in input order.
-__global__ void __launch_bounds__(ING_THREADS) k_ing_partition(const int *atom_off, const long long *res_off,
-                                                              int n_mols, long long n_buckets,
-                                                              const long long *bucket_start, const int *bucket,
-                                                              const int *serial, int *perm, int *res_begin,
-                                                              int *res_end, int *unsorted)
+// Sort key of every atom: its bucket, then its serial number, which the
+// stable radix sort orders as ResIndex does (upper-cased residue name, then
+// serial, input order among equals); the value is the atom's index.
+__global__ void __launch_bounds__(ING_THREADS) k_ing_sortkeys(long long n_atoms, const int *bucket, const int *serial,
+                                                             unsigned long long *key, int *val)
 {
-    const long long b = ((long long)blockIdx.x * ING_THREADS + threadIdx.x) / JAM_WAVE;
-    const int lane = lane_id();
-    if (b >= n_buckets) return;
-    int lo = 0, hi = n_mols;   // molecule of bucket b
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (res_off[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    const int a0 = atom_off[lo], a1 = atom_off[lo + 1];
-    long long pos = bucket_start[b];
-    int prev_serial = INT_MIN;
-    bool dec = false;
-    for (int q = a0; q < a1; q += JAM_WAVE) {
-        const int a = q + lane;
-        const bool mine = a < a1 && bucket[a] == (int)b;
-        const unsigned long long m = ballot(mine);
-        if (m == 0ull) continue;
-        const int sv = mine ? serial[a] : 0;
-        // serial numbers in input order: compare with the previous member
-        // (the highest member lane below this one, or the previous chunk's
-        // last); every lane takes part in the shuffle
-        const int r = rank_below(m);
-        const unsigned long long below = m & ((1ull << lane) - 1ull);
-        const int src = below ? 63 - __clzll((long long)below) : lane;
-        const int prev_in = __shfl(sv, src, JAM_WAVE);
-        const int before = r > 0 ? prev_in : prev_serial;
-        if (mine && sv < before) dec = true;
-        if (mine) perm[pos + r] = a - a0;
-        pos += __popcll(m);
-        prev_serial = __shfl(sv, 63 - __clzll((long long)m), JAM_WAVE);
-    }
-    if (ballot(dec) != 0ull && lane == 0) unsorted[b] = 1;
-    if (lane == 0) {
-        res_begin[b] = (int)(bucket_start[b] - a0);
-        res_end[b] = (int)(pos - a0);
-    }
+    const long long a = (long long)blockIdx.x * ING_THREADS + threadIdx.x;
+    if (a >= n_atoms) return;
+    key[a] = ((unsigned long long)(unsigned int)bucket[a] << 32) | ((unsigned int)serial[a] ^ 0x80000000u);
+    val[a] = (int)a;
 }
 
-// Block per bucket: the buckets flagged unsorted are sorted by (serial,
-// input index) -- the order of a stable sort by serial -- with a bitonic sort
-// of their range of perm, padded to a power of two, in global scratch.
-__global__ void __launch_bounds__(1024) k_ing_fixup(const int *atom_off, const long long *res_off, int n_mols,
-                                                    long long n_buckets, const int *unsorted, const int *res_begin,
-                                                    const int *res_end, const int *serial, int *perm,
-                                                    unsigned long long *scratch)
+// perm[i]: the molecule-local index of the i-th atom in sorted order (the
+// buckets are molecule-major, so position i lies in its atom's molecule)
+__global__ void __launch_bounds__(ING_THREADS) k_ing_perm(const int *atom_off, int n_mols, long long n_atoms,
+                                                         const int *sorted, int *perm)
 {
-    const long long b = blockIdx.x;
-    if (b >= n_buckets || !unsorted[b]) return;
-    int lo = 0, hi = n_mols;
+    const long long i = (long long)blockIdx.x * ING_THREADS + threadIdx.x;
+    if (i >= n_atoms) return;
+    const int a = sorted[i];
+    int lo = 0, hi = n_mols;   // the last m with atom_off[m] <= a
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (atom_off[mid] <= a) lo = mid;
+        else hi = mid;
+    }
+    perm[i] = a - atom_off[lo];
+}
+
+// res_begin / res_end of every bucket: its range in its molecule's part of
+// perm (bucket_start: the exclusive scan of the bucket sizes, nb + 1 values)
+__global__ void __launch_bounds__(ING_THREADS) k_ing_ranges(const int *atom_off, const long long *res_off, int n_mols,
+                                                           long long n_buckets, const long long *bucket_start,
+                                                           int *res_begin, int *res_end)
+{
+    const long long b = (long long)blockIdx.x * ING_THREADS + threadIdx.x;
+    if (b >= n_buckets) return;
+    int lo = 0, hi = n_mols;   // the last m with res_off[m] <= b: bucket b's molecule
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (res_off[mid] <= b) lo = mid;
         else hi = mid;
     }
-    const int a0 = atom_off[lo];
-    const int p0 = res_begin[b], n = res_end[b] - p0;
-    int n2 = 1;
-    while (n2 < n) n2 <<= 1;
-    // scratch of this bucket: 2 entries per atom of the molecule is enough
-    unsigned long long *w = scratch + 2 * (size_t)(a0 + p0);
-    for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-        if (k < n) {
-            const int loc = perm[a0 + p0 + k];
-            w[k] = ((unsigned long long)((unsigned int)serial[a0 + loc] ^ 0x80000000u) << 32) | (unsigned int)loc;
-        } else {
-            w[k] = ~0ull;
-        }
-    }
-    __syncthreads();
-    for (int k = 2; k <= n2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long u = w[i], v = w[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((u > v) == up) {
-                        w[i] = v;
-                        w[ixj] = u;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) perm[a0 + p0 + k] = (int)(w[k] & 0xFFFFFFFFull);
+    res_begin[b] = (int)(bucket_start[b] - atom_off[lo]);
+    res_end[b] = (int)(bucket_start[b + 1] - atom_off[lo]);
 }
 
 }  // namespace jam
