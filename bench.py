@@ -676,6 +676,16 @@ def measure_diff(n_sets, seed, steps, warmup, cpu_seconds, cpu):
     }
     for k in ("roofline", "roofline_backward"):
         out[k]["frac"] = out[k]["achieved"] / out[k]["peak"]
+    prof = os.path.join(ROOT, "profiles", "k_rmsd.rmsd-grad-1m.pmc.md")
+    if os.path.exists(prof) and n_sets == 1_000_000:
+        # HBM bytes per launch from the committed counter passes (FETCH_SIZE
+        # and WRITE_SIZE over this workload, calibrated against a copy)
+        for line in open(prof):
+            for key, k in (("traffic_bytes_per_launch_fwd:", "roofline"),
+                           ("traffic_bytes_per_launch_bwd:", "roofline_backward")):
+                if line.startswith(key):
+                    out[k]["traffic"] = float(line.split(":")[1])
+                    out[k]["traffic_source"] = os.path.relpath(prof, ROOT)
     if cpu:
         from oracle.kabsch_torch import kabsch_rmsd_grad
 
