@@ -225,6 +225,20 @@ int32_t jam_result_level_partials(const jam_result *r, int64_t *counts32);
 int32_t jam_context_set_profiling(jam_context *ctx, int32_t on);
 void jam_result_free(jam_result *r);
 
+/* Events kept in device memory of their own: one device-to-device copy of a
+ * result's events (queued on the context's stream, before any later query),
+ * so that a caller can copy some fields now (pair, rank, flag, rmsd: what
+ * Query.__next__'s policy needs, src/pyjess/_jess.pyx:2238-2276) and the
+ * others (rotation, centroids, atoms: what a Hit's transformation and atoms
+ * read, _jess.pyx:2343-2482) only if and when they are read. Same arguments
+ * and layout as jam_result_copy; they stay valid after later queries on the
+ * context, and hold their device memory until jam_events_free. */
+typedef struct jam_events jam_events;
+int32_t jam_result_keep(const jam_result *r, jam_events **out);
+int32_t jam_events_copy(const jam_events *e, int32_t *pair, int64_t *rank, int32_t *flag, double *rmsd,
+                        double *rotation9, double *c0_3, double *c1_3, int32_t *atoms_stride);
+void jam_events_free(jam_events *e);
+
 /* ---- standalone superposition ---------------------------------------- */
 
 /* Batched least-squares superposition of `n_sets` point-set pairs of `k`

@@ -128,7 +128,12 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         chosen = _select_events(pair, np.ascontiguousarray(res.rank, dtype=np.int64),
                                 np.ascontiguousarray(res.flag, dtype=np.int32),
                                 np.ascontiguousarray(res.rmsd, dtype=np.float64), best_match, mc)
-    block = _Block(res.rotation, res.c0, res.c1, res.atoms)
+    if getattr(res, "kept", None) is not None:
+        # the block owns the device copy from here (released once read)
+        block = _native.LazyBlock(res.kept)
+        res.kept = None
+    else:
+        block = _Block(res.rotation, res.c0, res.c1, res.atoms)
     if not keep_none:
         # lazy lists: the chosen hits (NaN rotations dropped and counted), cut
         # by molecule (events are grouped by molecule, in order)
@@ -191,7 +196,8 @@ class Jess:
         return self._tset
 
     def _device_query(self, molecules: Sequence[Molecule], rmsd_threshold, distance_cutoff,
-                      max_dynamic_distance, ignore_chain: int, reorder: bool) -> "_native.Result":
+                      max_dynamic_distance, ignore_chain: int, reorder: bool,
+                      keep: bool = False) -> "_native.Result":
         ctx = _native.context()
         tset = self._device_templates(ctx)
         if len(molecules) == 1:
@@ -202,7 +208,7 @@ class Jess:
         else:
             mset = _batch_cache(ctx, molecules)
         res = _native.query(tset, mset, rmsd_threshold, distance_cutoff, max_dynamic_distance,
-                            ignore_chain, reorder)
+                            ignore_chain, reorder, keep)
         self._last_result = res     # kernel timings and match counts of the latest query
         return res
 
@@ -225,7 +231,8 @@ class Jess:
         ThreadPool-over-molecules pattern of the reference README) but runs
         every (template, molecule) pair in a single launch chain. Each
         molecule's hits come as a `HitList`, a sequence that makes each `Hit`
-        when it is read.
+        when it is read; the rotations, centroids and matched atoms of all
+        hits are copied from the device when the first of them is read.
         """
         ignore_chain = _normalize_ignore_chain(ignore_chain)
         molecules = list(molecules)
@@ -237,8 +244,10 @@ class Jess:
             raise ValueError(f"invalid value for `max_candidates` argument: {max_candidates!r}")
         if not molecules or len(self) == 0:
             return [[] for _ in molecules]
+        # the hits' rotations, centroids and atoms stay in a device copy of
+        # the events until one of them is read
         res = self._device_query(molecules, rmsd_threshold, distance_cutoff, max_dynamic_distance,
-                                 _IGNORE[ignore_chain], reorder)
+                                 _IGNORE[ignore_chain], reorder, keep=True)
         hits, nan = _policy(self, molecules, res, best_match, mc, keep_none=False)
         for _ in range(int(np.count_nonzero(nan))):
             _warn_nan()

@@ -33,7 +33,8 @@ _SYMBOLS = (
     "jam_result_stride", "jam_result_n_pairs", "jam_result_copy", "jam_result_pair_matches",
     "jam_result_timings", "jam_result_pair_digests", "jam_result_join_kernels", "jam_context_set_profiling", "jam_result_level_partials", "jam_rmsd_forward",
     "jam_rmsd_backward", "jam_result_free", "jam_superpose_batch",
-    "jam_superpose_batch_noalloc", "jam_kd_order", "jam_last_error",
+    "jam_superpose_batch_noalloc", "jam_kd_order", "jam_result_keep", "jam_events_copy", "jam_events_free",
+    "jam_last_error",
     "jam_version", "jam_device_available", "jam_synchronize",
 )
 
@@ -173,6 +174,12 @@ def lib():
         L.jam_molecules_copy_atoms.restype = i32
         L.jam_molecules_residue_index.argtypes = [vp, vp, vp, vp, vp, vp]
         L.jam_molecules_residue_index.restype = i32
+        L.jam_result_keep.argtypes = [vp, P(vp)]
+        L.jam_result_keep.restype = i32
+        L.jam_events_copy.argtypes = [vp] * 9
+        L.jam_events_copy.restype = i32
+        L.jam_events_free.argtypes = [vp]
+        L.jam_events_free.restype = None
         L.jam_kd_order.argtypes = [vp, vp, vp, i32, vp]
         L.jam_kd_order.restype = i32
         L.jam_last_error.argtypes = []
@@ -398,13 +405,86 @@ class ParsedMolecules(MoleculeSet):
         return {k: v[:n] for k, v in a.items()}
 
 
+class KeptEvents:
+    """A device copy of a query's events (jam_result_keep), which later
+    queries on the context leave alone; its device memory is released when
+    the object goes."""
+
+    def __init__(self, ctx: "Context", handle, n: int, stride: int):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.jam_result_keep(handle, ctypes.byref(h)))
+        self.ctx = ctx
+        self.handle = h
+        self.n = n
+        self.stride = stride
+
+    def heavy(self):
+        """(rotation, c0, c1, atoms) of every event, copied to host memory."""
+        m = max(self.n, 1)
+        rot = np.empty((m, 9), dtype=np.float64)
+        c0 = np.empty((m, 3), dtype=np.float64)
+        c1 = np.empty((m, 3), dtype=np.float64)
+        atoms = np.empty((m, max(self.stride, 1)), dtype=np.int32)
+        with self.ctx.lock:
+            _check(lib().jam_events_copy(self.handle, None, None, None, None, _ptr(rot), _ptr(c0), _ptr(c1),
+                                         _ptr(atoms)))
+        n = self.n
+        return rot[:n], c0[:n], c1[:n], atoms[:n]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.jam_events_free(h)
+            self.handle = None
+
+
+class LazyBlock:
+    """The rotation, centroids and matched atoms of a batch's events, left in
+    their device copy until a hit first reads one of them; then all are
+    copied at once and the device copy is released."""
+
+    __slots__ = ("_kept", "_arrays", "_lock")
+
+    def __init__(self, kept: KeptEvents):
+        self._kept = kept
+        self._arrays = None
+        self._lock = threading.Lock()
+
+    def _load(self):
+        if self._arrays is None:
+            with self._lock:
+                if self._arrays is None:
+                    self._arrays = self._kept.heavy()
+                    self._kept = None
+        return self._arrays
+
+    @property
+    def rotation(self):
+        return self._load()[0]
+
+    @property
+    def c0(self):
+        return self._load()[1]
+
+    @property
+    def c1(self):
+        return self._load()[2]
+
+    @property
+    def atoms(self):
+        return self._load()[3]
+
+
 class Result:
-    """Events of one device query, as numpy arrays."""
+    """Events of one device query, as numpy arrays. With ``keep``, only the
+    fields the hit policy reads (pair, rank, flag, rmsd) are copied now; the
+    others stay in a device copy of the events (`LazyBlock`)."""
 
     __slots__ = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms", "pair_matches",
-                 "timings_ms", "join_kernel_ms", "join_launches", "level_partials", "n_pairs", "stride")
+                 "timings_ms", "join_kernel_ms", "join_launches", "level_partials", "n_pairs", "stride", "kept")
 
-    def __init__(self, handle):
+    def __init__(self, handle, ctx: Optional["Context"] = None, keep: bool = False):
         L = lib()
         n = L.jam_result_n_events(handle)
         self.stride = L.jam_result_stride(handle)
@@ -415,14 +495,23 @@ class Result:
         self.rank = np.empty(m, dtype=np.int64)
         self.flag = np.empty(m, dtype=np.int32)
         self.rmsd = np.empty(m, dtype=np.float64)
-        self.rotation = np.empty((m, 9), dtype=np.float64)
-        self.c0 = np.empty((m, 3), dtype=np.float64)
-        self.c1 = np.empty((m, 3), dtype=np.float64)
-        self.atoms = np.empty((m, max(self.stride, 1)), dtype=np.int32)
-        _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
-                                 _ptr(self.rmsd), _ptr(self.rotation), _ptr(self.c0), _ptr(self.c1),
-                                 _ptr(self.atoms)))
-        for name in ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms"):
+        self.kept = None
+        if keep:
+            self.kept = KeptEvents(ctx, handle, n, self.stride)
+            self.rotation = self.c0 = self.c1 = self.atoms = None
+            _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
+                                     _ptr(self.rmsd), None, None, None, None))
+            names = ("pair", "rank", "flag", "rmsd")
+        else:
+            self.rotation = np.empty((m, 9), dtype=np.float64)
+            self.c0 = np.empty((m, 3), dtype=np.float64)
+            self.c1 = np.empty((m, 3), dtype=np.float64)
+            self.atoms = np.empty((m, max(self.stride, 1)), dtype=np.int32)
+            _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
+                                     _ptr(self.rmsd), _ptr(self.rotation), _ptr(self.c0), _ptr(self.c1),
+                                     _ptr(self.atoms)))
+            names = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms")
+        for name in names:
             setattr(self, name, getattr(self, name)[:n])
         self.pair_matches = np.zeros(max(self.n_pairs, 1), dtype=np.int64)
         _check(L.jam_result_pair_matches(handle, _ptr(self.pair_matches)))
@@ -439,7 +528,9 @@ class Result:
 
 
 def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_cutoff: float,
-          max_dynamic_distance: float, ignore_chain: int, reorder: bool) -> Result:
+          max_dynamic_distance: float, ignore_chain: int, reorder: bool, keep: bool = False) -> Result:
+    """One `jam_query`; with ``keep`` the rotations, centroids and atoms stay
+    in a device copy until read (`Result`)."""
     L = lib()
     ctx = tset.ctx
     p = Params(float(rmsd_threshold), float(distance_cutoff), float(max_dynamic_distance),
@@ -449,7 +540,7 @@ def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_
         _check(L.jam_query(ctx.handle, tset.handle, mset.handle, ctypes.byref(p), ctypes.byref(h)))
         try:
             # the events live in the context's device buffers until the next query
-            return Result(h)
+            return Result(h, ctx, keep)
         finally:
             L.jam_result_free(h)
 

@@ -1762,15 +1762,13 @@ struct CopyPiece {
 
 }  // namespace
 
-extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
-                                   double *rmsd, double *rot, double *c0, double *c1, int32_t *atoms)
+// Copies the fields of n events packed at `base` (device memory, the layout
+// of event_layout(n, stride)) to host buffers (any may be NULL) through the
+// context's pinned staging halves, on the context's stream.
+static int32_t copy_event_fields(jam_context *c, const char *base, size_t n, int stride, int32_t *pair,
+                                 int64_t *rank, int32_t *flag, double *rmsd, double *rot, double *c0, double *c1,
+                                 int32_t *atoms)
 {
-    if (!r) return fail(JAM_EINVAL, "NULL result");
-    const size_t n = (size_t)r->n_events;
-    if (n == 0) return JAM_OK;
-    jam_context *c = r->ctx;
-    if (!c || c->query_gen != r->gen)
-        return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     if (!c->hstage) {
@@ -1781,8 +1779,7 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
     // (in the order of the packed layout, so that neighbouring pieces are
     // contiguous on the device)
     std::vector<CopyPiece> pieces;
-    const EventLayout L = event_layout((int64_t)n, r->stride);
-    const char *base = c->e_all.as<const char>();
+    const EventLayout L = event_layout((int64_t)n, stride);
     auto add = [&](void *dst, size_t at, size_t bytes) {
         if (!dst) return;
         for (size_t o = 0; o < bytes; o += JAM_STAGE_BYTES)
@@ -1795,7 +1792,7 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
     add(c1, L.c1, sizeof(double) * 3 * n);
     add(pair, L.pair, sizeof(int32_t) * n);
     add(flag, L.flag, sizeof(int32_t) * n);
-    add(atoms, L.atoms, sizeof(int32_t) * (size_t)r->stride * n);
+    add(atoms, L.atoms, sizeof(int32_t) * (size_t)stride * n);
     // batches of consecutive pieces filling one half each; the copy of batch
     // k + 1 runs while batch k is unpacked from the other half
     struct Batch {
@@ -1835,6 +1832,70 @@ extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *
         }
     }
     return JAM_OK;
+}
+
+extern "C" int32_t jam_result_copy(const jam_result *r, int32_t *pair, int64_t *rank, int32_t *flag,
+                                   double *rmsd, double *rot, double *c0, double *c1, int32_t *atoms)
+{
+    if (!r) return fail(JAM_EINVAL, "NULL result");
+    const size_t n = (size_t)r->n_events;
+    if (n == 0) return JAM_OK;
+    jam_context *c = r->ctx;
+    if (!c || c->query_gen != r->gen)
+        return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
+    return copy_event_fields(c, c->e_all.as<const char>(), n, r->stride, pair, rank, flag, rmsd, rot, c0, c1,
+                             atoms);
+}
+
+// events kept in device memory of their own (jam_result_keep)
+struct jam_events {
+    jam_context *ctx = nullptr;
+    int64_t n = 0;
+    int stride = 1;
+    void *dev = nullptr;
+};
+
+extern "C" int32_t jam_result_keep(const jam_result *r, jam_events **out)
+{
+    if (!r || !out) return fail(JAM_EINVAL, "NULL argument");
+    *out = nullptr;
+    jam_context *c = r->ctx;
+    if (!c || c->query_gen != r->gen)
+        return fail(JAM_ESTALE, "the events of this result were overwritten by a later query on its context");
+    HIP_TRY(hipSetDevice(c->device));
+    std::unique_ptr<jam_events> e(new jam_events());
+    e->ctx = c;
+    e->n = r->n_events;
+    e->stride = r->stride;
+    const size_t bytes = event_layout(e->n, e->stride).bytes;
+    if (bytes > 0) {
+        HIP_TRY(hipMalloc(&e->dev, bytes));
+        // ordered before the context's next query, which overwrites the source
+        const hipError_t err = hipMemcpyAsync(e->dev, c->e_all.ptr, bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (err != hipSuccess) {
+            (void)hipFree(e->dev);
+            return fail(JAM_EDEVICE, std::string("hipMemcpyAsync: ") + hipGetErrorString(err));
+        }
+    }
+    *out = e.release();
+    return JAM_OK;
+}
+
+extern "C" int32_t jam_events_copy(const jam_events *e, int32_t *pair, int64_t *rank, int32_t *flag, double *rmsd,
+                                   double *rot, double *c0, double *c1, int32_t *atoms)
+{
+    if (!e) return fail(JAM_EINVAL, "NULL events");
+    if (e->n == 0) return JAM_OK;
+    return copy_event_fields(e->ctx, (const char *)e->dev, (size_t)e->n, e->stride, pair, rank, flag, rmsd, rot, c0,
+                             c1, atoms);
+}
+
+extern "C" void jam_events_free(jam_events *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->ctx->device);
+    if (e->dev) (void)hipFree(e->dev);
+    delete e;
 }
 
 extern "C" int32_t jam_result_pair_matches(const jam_result *r, int64_t *counts)

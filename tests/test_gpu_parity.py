@@ -164,6 +164,30 @@ def test_query_batch_equals_single_queries():
     assert total > 0
 
 
+def test_query_batch_hits_survive_later_queries():
+    """query_batch leaves the hits' rotations, centroids and atoms in a device
+    copy of the events until the first of them is read; later queries, which
+    overwrite the context's own event buffer, leave that copy alone."""
+    mols, tpls, kw = synthetic.workload("batch-4k", seed=5, scale=4 / 4096)
+    args = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    jess = Jess(tpls)
+    first = jess.query_batch(mols, *args)
+    second = jess.query_batch(mols[::-1], *args, best_match=True)
+    single = [list(jess.query(mol, *args)) for mol in mols]
+    assert sum(len(h) for h in first) > 0
+    for mol, got, want in zip(mols, first, single):
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert g.template() is w.template() and key(mol, g) == key(mol, w)
+            assert same_bits(g.rmsd, w.rmsd)
+            assert same_bits(np.ravel(g._rotation), np.ravel(w._rotation))
+            assert same_bits(np.ravel(g._centre), np.ravel(w._centre))
+            assert np.array_equal(np.asarray(g.transformation), np.asarray(w.transformation))
+    for mol, got in zip(mols[::-1], second):
+        want = list(jess.query(mol, *args, best_match=True))
+        assert [key(mol, h) for h in got] == [key(mol, h) for h in want]
+
+
 def test_superpose_batch_bit_exact():
     rng = np.random.default_rng(0)
     for k in (3, 4, 6, 12, 24):
