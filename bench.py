@@ -8,7 +8,7 @@ HBM. The metric is template x molecule queries per second.
 
 The headline ``value`` is the library-scale batched search: 64 synthetic
 assemblies of 100,000 atoms in 20 chains x 7,000 templates of 2-8 residues
-(2.5 A cutoff), streamed through the C ABI in chunks of 8 assemblies, the
+(2.5 A cutoff), streamed through the C ABI in chunks of 16 assemblies, the
 events left in HBM; every assembly's hit set is checked against the oracle
 (per-pair counts and digests computed on the device). Beside it: the event
 copy to host memory and the drop-in ``Jess.query_batch`` (Python hit
@@ -17,8 +17,8 @@ objects) on one chunk; ``plain`` is the drop-in ``list(Jess(templates).query(mol
 ``Jess.query_batch`` on 4,096 molecules x 200 templates; ``differentiation``
 is the differentiable superposition RMSD (forward + backward) over 1M
 point-set pairs; ``ingestion`` is the device PDB parse (records, exact
-decimals, residue-name index) of one chunk of the library's assemblies
-written as PDB text. Each carries a CPU baseline (the C restatement of the
+decimals, residue-name index) of 8 of the library's assemblies written as
+PDB text. Each carries a CPU baseline (the C restatement of the
 reference algorithm built for the host, on the host's cores, on a bounded
 sample; torch autograd through an SVD for the RMSD; the host PDB parser for
 ingestion) and the search legs the
@@ -504,12 +504,13 @@ def measure_library(scale, seed, chunk, steps, warmup, cpu_seconds, cpu, parity_
         log(f"library: cpu baseline {cpu_rate:,.0f} queries/s on {threads} threads; hit-set parity on "
             f"{n_par}/{len(mols)} assemblies ({int(hc_all.sum())} hits, oracle {full_s:.1f}s): {parity}")
     if ingest_steps > 0:
-        out["ingestion"] = measure_ingest(mols[:chunk], ingest_steps, cpu_seconds, cpu)
+        # 8 assemblies (65.5 MB of PDB text), the ingest-8 workload
+        out["ingestion"] = measure_ingest(mols[:8], ingest_steps, cpu_seconds, cpu)
     return out
 
 
 def measure_ingest(mols, steps, cpu_seconds, cpu):
-    """PDB ingestion of one chunk of library assemblies: their PDB texts
+    """PDB ingestion of library assemblies (8 in the default run): their PDB texts
     (``Molecule.dumps`` of the assemblies translated to non-negative
     coordinates, serial numbers wrapped at 99,999 as the five serial
     columns force) parsed on the device into a batch with its residue-name
@@ -758,7 +759,7 @@ def main():
                     help="timed steps of the plain-8k measurement beside the headline (0: skip it)")
     ap.add_argument("--parity-first-chunk", action="store_true",
                     help="library-64: check the hit set of the first chunk of assemblies only")
-    ap.add_argument("--library-chunk", type=int, default=8, help="assemblies per batched call of library-64")
+    ap.add_argument("--library-chunk", type=int, default=16, help="assemblies per batched call of library-64")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="fraction of the molecules of the workload measured (library-64: of its 64 assemblies)")
     ap.add_argument("--seed", type=int, default=0)
@@ -813,7 +814,7 @@ def main():
         # the PDB ingestion of one chunk of library assemblies alone
         from pyjess_amd import synthetic
 
-        mols, _, _ = synthetic.workload("library-64", seed=args.seed, scale=args.library_chunk / 64)
+        mols, _, _ = synthetic.workload("library-64", seed=args.seed, scale=8 / 64)
         g = measure_ingest(mols, args.steps, args.cpu_seconds, not args.no_cpu)
         out = {"metric": "PDB atoms ingested/sec (device parse + residue-name index), 1 GPU vs CPU",
                "value": g["value"], "unit": g["unit"], "n_gpus": 1, "steps": args.steps, "warmup": 1,
