@@ -771,7 +771,7 @@ def main():
                     help="timed device parses of one chunk of library PDB texts (0: skip the ingestion leg)")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="fraction of the 4,096 molecules of the batched measurement (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--diff-sets", type=int, default=1_000_000,
                     help="point-set pairs of the differentiable-RMSD measurement (0: skip it)")
     args = ap.parse_args()
