@@ -324,11 +324,24 @@ struct JoinWaveLds {
     int dp[JOIN_STAGE], de[JOIN_STAGE];
 };
 
+// What a team keeps of the pair of its last row: the template data of a
+// level and the annulus windows of the placed atoms depend only on the pair
+// (and the level, fixed in a launch), so the next row of the same pair --
+// a team takes runs of nearby rows -- loads none of it again. The windows
+// themselves stay in the team's LDS (r2lo, r2hi).
+template <int NE>
+struct PairCache {
+    int p = -1;
+    int n = 0, setL = 0, cntL = 0, offL = 0;
+    double b[NE];      // outer radius of the window of placed atom tl + jj * TW
+    int flags[NE];     // its template atom's chainID2 (bit 0) and resSeq (bit 1) equal the new one's
+};
+
 template <int MAXN, int TW>
 __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                  const DevJoin &jn, const QueryParams &qp, JoinWaveLds<MAXN, TW> &S,
                                  bool active, long long it, int parts, int p, const int (&ent)[(MAXN + TW - 1) / TW],
-                                 int &staged, int &dstaged)
+                                 PairCache<(MAXN + TW - 1) / TW> &pc, int &staged, int &dstaged)
 {
     constexpr unsigned long long TMASK = (TW == 64) ? ~0ull : ((1ull << TW) - 1ull);
     const int lane = lane_id();
@@ -337,11 +350,49 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
     const int L = jn.level;
     const long long i = active ? it / parts : 0;
     const int part = active ? (int)(it - i * parts) : 0;
-    const int T = tp.n_templates;
-    const int t = p % T;
-    const int tbase = tp.atom_off[t];
-    const int n = tp.atom_off[t + 1] - tbase;
-    const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
+    constexpr int NE = (MAXN + TW - 1) / TW;
+    if (active && p != pc.p) {
+        // a new pair (team-uniform): its template data and annulus windows
+        const int t = p % tp.n_templates;
+        const int tbase = tp.atom_off[t];
+        const int n = tp.atom_off[t + 1] - tbase;
+        const PairLevel *lv = pr.lvl + (size_t)p * JAM_MAX_LEVELS;
+        pc.p = p;
+        pc.n = n;
+        if (L < n) {
+            const PairLevel hL = lv[L];
+            const int kL = hL.k;
+            pc.setL = hL.set;
+            pc.cntL = hL.cnt;
+            pc.offL = hL.off;
+            const int tchL = tp.chain[tbase + kL], tresL = tp.resseq[tbase + kL];
+#pragma unroll
+            for (int jj = 0; jj < NE; jj++) {
+                const int j = tl + jj * TW;
+                if (j >= L) break;
+                const int k = lv[j].k;
+                pc.flags[jj] = (tp.chain[tbase + k] == tchL ? 1 : 0) | (tp.resseq[tbase + k] == tresL ? 2 : 0);
+                // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
+                // Annulus_reuse patches/Annulus.c.patch:107-140
+                const double d = tp.dist[tp.dist_off[t] + k * n + kL];
+                double tau = qp.distance_cutoff + tp.weight[tbase + k] + tp.weight[tbase + kL];
+                if (tau > qp.max_dynamic_distance) tau = qp.max_dynamic_distance;
+                double a = d - tau, b = d + tau;
+                if (a < 0.5) a = 0.5;
+                if (b < a) {
+                    double tmp = a;
+                    a = b;
+                    b = tmp;
+                }
+                if (a < 0.0) a = 0.0;
+                if (b < 0.0) b = 0.0;
+                M.r2lo[j] = a * a;
+                M.r2hi[j] = b * b;
+                pc.b[jj] = b;
+            }
+        }
+    }
+    const int n = pc.n;
     if (active && L >= n && tl == 0 && part == 0) {
         // a one-atom template is complete at level 0
         const unsigned long long r = atomicAdd(jn.n_done, 1ull);
@@ -353,46 +404,19 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         }
     }
     const bool live = active && L < n;
-    int kL = 0, setL = 0, cntL = 0, offL = 0, tchL = 0, tresL = 0;
-    if (live) {
-        const PairLevel hL = lv[L];
-        kL = hL.k;
-        setL = hL.set;
-        cntL = hL.cnt;
-        offL = hL.off;
-        tchL = tp.chain[tbase + kL];
-        tresL = tp.resseq[tbase + kL];
-    }
+    const int setL = live ? pc.setL : 0, cntL = live ? pc.cntL : 0, offL = live ? pc.offL : 0;
     double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
 #pragma unroll
-    for (int jj = 0; jj < (MAXN + TW - 1) / TW; jj++) {
+    for (int jj = 0; jj < NE; jj++) {
         const int j = tl + jj * TW;
         if (!live || j >= L) break;
         // the atom placed at level j
-        const int e = ent[jj];
-        const int k = lv[j].k;
-        const SetEnt en = st.kent[e];
+        const SetEnt en = st.kent[ent[jj]];
         M.px[j] = en.x;
         M.py[j] = en.y;
         M.pz[j] = en.z;
-        M.pk[j] = make_int4(en.ch, en.res,
-                            (tp.chain[tbase + k] == tchL ? 1 : 0) | (tp.resseq[tbase + k] == tresL ? 2 : 0), 0);
-        // annulus window of the pair (k, kL), patches/Scanner.c.patch:274-294,
-        // Annulus_reuse patches/Annulus.c.patch:107-140
-        const double d = tp.dist[tp.dist_off[t] + k * n + kL];
-        double tau = qp.distance_cutoff + tp.weight[tbase + k] + tp.weight[tbase + kL];
-        if (tau > qp.max_dynamic_distance) tau = qp.max_dynamic_distance;
-        double a = d - tau, b = d + tau;
-        if (a < 0.5) a = 0.5;
-        if (b < a) {
-            double tmp = a;
-            a = b;
-            b = tmp;
-        }
-        if (a < 0.0) a = 0.0;
-        if (b < 0.0) b = 0.0;
-        M.r2lo[j] = a * a;
-        M.r2hi[j] = b * b;
+        M.pk[j] = make_int4(en.ch, en.res, pc.flags[jj], 0);
+        const double b = pc.b[jj];
         lo[0] = fmax(lo[0], en.x - b);
         lo[1] = fmax(lo[1], en.y - b);
         lo[2] = fmax(lo[2], en.z - b);
@@ -537,6 +561,9 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
 #ifndef JOIN_XCDS
 #define JOIN_XCDS 8
 #endif
+#ifndef JOIN_RUN
+#define JOIN_RUN 32
+#endif
 template <int MAXN, int TW>
 __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(DevMols ms, DevTpls tp, DevSets st,
                                                                               DevPairs pr, DevJoin jn,
@@ -558,8 +585,15 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
     const int xcd = (int)(blockIdx.x % xcds);
     const long long per = slots / xcds;   // items of one chunk
     const long long tslot = ((long long)(blockIdx.x / xcds) * JOIN_WAVES + w) * NTEAM;
-    // the first item of this wave's teams in round i
-    auto round_base = [&](long long i) { return (i * xcds + xcd) * per + tslot; };
+    // the first item of this wave's teams in round i: rounds come in runs of
+    // R over consecutive rows (so that a team meets rows of one pair one
+    // after the other), when the frontier is large enough for every wave to
+    // have several runs
+    const long long R = items >= 4 * slots * (long long)JOIN_RUN ? JOIN_RUN : 1;
+    auto round_base = [&](long long i) {
+        const long long k = i / R, j = i - k * R;
+        return ((k * xcds + xcd) * per + tslot) * R + j * NTEAM;
+    };
     // the pair and this lane's entries of the team's next row are loaded
     // while the current one is processed
     constexpr int NE = (MAXN + TW - 1) / TW;
@@ -578,6 +612,7 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
             if (j < L) e_next[jj] = jn.in_ent[r * jn.stride + j];
         }
     };
+    PairCache<NE> pc;
     long long base = round_base(0);
     if (base < items) fetch(base + team);
     for (long long i = 0; base < items; i++) {
@@ -588,7 +623,7 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
         for (int jj = 0; jj < NE; jj++) e[jj] = e_next[jj];
         const long long next = round_base(i + 1);
         if (next < items) fetch(next + team);
-        join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, p, e, staged, dstaged);
+        join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, p, e, pc, staged, dstaged);
         base = next;
     }
     join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
