@@ -131,7 +131,7 @@ struct jam_context {
     int64_t front_cap = 0, done_cap = 0;
     int front_stride = 0, done_stride = 0;
     int join_grid[4] = {0, 0, 0, 0};   // resident workgroups of the join kernel <8/16/24/32>
-    int kd_grid[12] = {};           // resident workgroups of the kd kernels
+    int kd_grid[14] = {};           // resident workgroups of the kd kernels
     // the exact kd pass of each size tier runs on a stream of its own, from
     // the moment the tier's fast pass has listed its tied sets, beside the
     // fast passes of the other tiers (all streams at the default priority)
@@ -1256,9 +1256,11 @@ void launch_join(jam_context *c, int stride, hipStream_t s, const DevMols &dm, c
 // Launches the kd passes over the candidate sets of `st`: the sets are
 // bucketed by size tier, then every tier gets its fast pass on `s` (every
 // set: <= 64 entries in one wave's registers, up to KD_BIG with one position
-// per thread, larger in global scratch; a set with a tie at a split point is
-// listed instead of written) and, on the tier's side stream from the end of
-// that fast pass, the exact QuickSelect pass over the listed sets. With one
+// per thread, above KD_LDS_MAX in global scratch; a set with a tie at a split
+// point is listed instead of written) and, on the tier's side stream from the
+// end of that fast pass, the exact QuickSelect pass over the listed sets. Sets
+// of KD_BIG+1 ... KD_LDS_MAX entries skip the fast pass (at library sizes
+// 25-100% of them have a tie) and all go to the exact pass in LDS. With one
 // molecule (few sets, latency-bound) the 513-1024 and 257-512 tiers skip the
 // fast pass and run the exact pass on all their sets from the start: for a
 // handful of big sets it is the kd phase's longest chain, and the fast pass
@@ -1295,6 +1297,8 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
         c->kd_grid[9] = resident_grid(jam::k_kdexact<512>, 512, d);
         c->kd_grid[10] = resident_grid(jam::k_kdexact<1024>, 1024, d);
         c->kd_grid[11] = resident_grid(jam::k_kdbig_exact, KD_EXACT_THREADS, d);
+        c->kd_grid[12] = resident_grid(jam::k_kdexact_lds<KD_LDS_SMALL_THREADS, 8>, KD_LDS_SMALL_THREADS, d);
+        c->kd_grid[13] = resident_grid(jam::k_kdexact_lds<KD_LDS_MAX / 8, 8>, KD_LDS_MAX / 8, d);
     }
     auto grid = [&](int k) { return dim3((unsigned)std::min(n_sets, c->kd_grid[k])); };
     auto launch_fast = [&](int k, hipStream_t x) {
@@ -1321,7 +1325,7 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
             break;
         default:
             hipLaunchKernelGGL(jam::k_kdbig, grid(5), dim3(KD_THREADS), 0, x, dm, dt, st, scr, bucket + o, nb + 5,
-                               tied + o, nt + 5);
+                               tied + o, nt + 5, KD_LDS_MAX);
             break;
         }
     };
@@ -1348,8 +1352,15 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
             hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, lst, lc);
             break;
         default:
-            hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr, lst,
-                               lc);
+            // every set of the tier up to KD_LDS_MAX entries (most of them
+            // have a tie) in LDS, then the larger sets the fast pass listed
+            // in global scratch
+            hipLaunchKernelGGL((jam::k_kdexact_lds<KD_LDS_SMALL_THREADS, 8>), grid(12), dim3(KD_LDS_SMALL_THREADS), 0,
+                               x, dm, dt, st, bucket + o, nb + 5, KD_BIG);
+            hipLaunchKernelGGL((jam::k_kdexact_lds<KD_LDS_MAX / 8, 8>), grid(13), dim3(KD_LDS_MAX / 8), 0, x, dm, dt,
+                               st, bucket + o, nb + 5, KD_LDS_SMALL_THREADS * 8);
+            hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr, lst, lc,
+                               KD_LDS_MAX);
             break;
         }
     };

@@ -34,7 +34,9 @@
 //
 // Sets are bucketed by size; sets of up to 64 entries run in one wave's
 // registers, up to KD_BIG entries with one position per thread (shuffles,
-// ballots and LDS exchanges), larger ones in global scratch. Every loop of the
+// ballots and LDS exchanges), up to KD_LDS_MAX entries straight to the exact
+// pass with 8 positions per thread in LDS (most sets that large have a tie),
+// larger ones in global scratch. Every loop of the
 // exact pass whose length depends on the data has a bound that a correct
 // chain never reaches; a pass that reaches it sets st.kd_err and leaves the
 // set unwritten, and the query reports an error.
@@ -621,6 +623,287 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
 }
 
 // ---------------------------------------------------------------------------
+// exact path, sets of (min_n, NT * P] entries: the partition step of
+// k_kdexact with P consecutive positions per thread, so that a set of a few
+// thousand entries stays in LDS. Positions, elements and prefix counts are
+// 16-bit, and only the coordinate of the current depth's axis is kept
+// (reloaded from HBM at every depth): 18 bytes per entry, 144 KB for 8,192
+// entries. A thread keeps the element, range and window of each of its
+// positions in registers; consecutive positions of one range share its pivot
+// and prefix reads.
+// ---------------------------------------------------------------------------
+
+// largest set of the LDS exact pass (1,024 threads); larger ones run in
+// global scratch. Sets up to 8 * KD_LDS_SMALL_THREADS entries run with fewer
+// threads and half the LDS, two workgroups per CU.
+#define KD_LDS_MAX 8192
+#define KD_LDS_SMALL_THREADS 512
+// a queue push that still names another push (src entries are 16-bit)
+#define KD_LINK 0x8000u
+
+template <int NT, int P>
+__global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, DevSets st, const int *list,
+                                                    const unsigned int *count, int min_n)
+{
+    constexpr int N = NT * P;
+    constexpr int NW = NT / JAM_WAVE;
+    static_assert(N <= KD_LDS_MAX && N < (int)KD_LINK, "16-bit positions with a link flag");
+    __shared__ double cc[N];             // coordinate on the current axis, by element
+    __shared__ unsigned short xa[N];     // arrangement: element at each position
+    __shared__ unsigned short xp1[N];    // exclusive prefix counts of the "<=" flags
+    __shared__ unsigned short src[N];    // queue pushes: element, or KD_LINK | push slot
+    __shared__ int wf[N];                // first ">" position of the window of the range starting here
+    __shared__ int wsum[NW], wact[NW];
+    const int t = threadIdx.x;
+    const int lane = lane_id();
+    const int wv = t / JAM_WAVE;
+    const int p0 = t * P;
+    if (*st.total > st.cap) return;
+    const int cnt = (int)*count;
+    for (int qs = blockIdx.x; qs < cnt; qs += gridDim.x) {
+        const int s = list[qs];
+        const int n = st.cnt[s];
+        if (n <= min_n || n > N) continue;
+        const int64_t off = st.off[s];
+        const int abase = ms.atom_off[s / tp.n_types];
+        for (int e = t; e < n; e += NT) {
+            cc[e] = ms.xyz[3 * (size_t)(abase + st.list[off + e])];
+            xa[e] = (unsigned short)e;
+        }
+        // per position: element, range [R, H), window [lw, rw] (rw < 0: none)
+        int a[P], R[P], H[P], lw[P], rw[P];
+#pragma unroll
+        for (int k = 0; k < P; k++) {
+            a[k] = p0 + k;
+            R[k] = 0;
+            H[k] = n;
+            lw[k] = 0;
+            rw[k] = (n >= 2 && p0 + k < n) ? n - 1 : -1;
+        }
+        __syncthreads();
+        int hmax = n;                      // largest range at this depth
+        bool broken = false;               // a loop bound was reached (block-uniform)
+        const long long round_bound = KD_ROUND_BOUND(n);
+        const int jump_bound = KD_JUMP_BOUND(n);
+        long long rounds = 0;
+        for (int d = 0; hmax >= 2;) {
+            if (++rounds > round_bound) {
+                broken = true;
+                break;
+            }
+            // pivot of each window: median of three, swapped with its right end
+            int piv[P], er[P];
+            unsigned lem = 0u, gtm = 0u;   // bit k: position k is "<=" / ">" in its window
+            bool act = false;
+            {
+                int cR = -1, c_piv = 0, c_er = 0;
+                double c_pv = 0.0;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const int p = p0 + k;
+                    piv[k] = -1;
+                    er[k] = 0;
+                    if (rw[k] < 0) continue;
+                    act = true;
+                    if (R[k] != cR) {
+                        const int l = lw[k], r = rw[k], mid = (l + r + 1) / 2;
+                        const int el = xa[l], em = xa[mid], e_r = xa[r];
+                        const double kl = cc[el], km = cc[em], kr = cc[e_r];
+                        if ((kl > km) != (kl > kr)) c_piv = l;
+                        else if ((km < kl) != (km < kr)) c_piv = mid;
+                        else c_piv = r;
+                        c_er = e_r;
+                        c_pv = c_piv == l ? kl : (c_piv == mid ? km : kr);
+                        cR = R[k];
+                    }
+                    piv[k] = c_piv;
+                    er[k] = c_er;
+                    if (p == c_piv) a[k] = c_er;
+                    if (p == rw[k]) a[k] = xa[c_piv];
+                    if (p == R[k]) wf[p] = rw[k];
+                    if (p >= lw[k] && p < rw[k]) {
+                        if (cc[a[k]] <= c_pv) lem |= 1u << k;
+                        else gtm |= 1u << k;
+                    }
+                }
+            }
+            // block exclusive scan of the "<=" counts, and whether any window is open
+            const int mine = __popc(lem);
+            int incl = mine;
+#pragma unroll
+            for (int sh = 1; sh < JAM_WAVE; sh <<= 1) {
+                const int o = __shfl_up(incl, sh, JAM_WAVE);
+                if (lane >= sh) incl += o;
+            }
+            const unsigned long long aball = ballot(act);
+            if (lane == JAM_WAVE - 1) {
+                wsum[wv] = incl;
+                wact[wv] = aball != 0ull;
+            }
+            __syncthreads();
+            bool any = false;
+            int base = incl - mine;
+            for (int w = 0; w < NW; w++) {
+                any |= wact[w] != 0;
+                if (w < wv) base += wsum[w];
+            }
+            if (!any) {
+                // every selection of this depth is done: split the ranges at
+                // their medians, open the windows of the next depth and load
+                // its axis (every read of cc at this depth came before the
+                // barrier above)
+                hmax -= hmax / 2;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const bool live = p0 + k < n;
+                    if (live && H[k] - R[k] >= 2) {
+                        const int m = R[k] + (H[k] - R[k]) / 2;
+                        if (p0 + k < m) H[k] = m;
+                        else R[k] = m;
+                    }
+                    lw[k] = R[k];
+                    rw[k] = (live && H[k] - R[k] >= 2) ? H[k] - 1 : -1;
+                }
+                d++;
+                if (hmax >= 2) {
+                    const int ax = d % 3;
+                    for (int e = t; e < n; e += NT) cc[e] = ms.xyz[3 * (size_t)(abase + st.list[off + e]) + ax];
+                }
+                __syncthreads();
+                continue;
+            }
+            // prefix counts; the first ">" of each window (a thread's first
+            // one there, its positions ascending)
+            int ones[P];
+            {
+                int run = base, lastR = -1;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const int p = p0 + k;
+                    ones[k] = run;
+                    if (p < n) xp1[p] = (unsigned short)run;
+                    run += (lem >> k) & 1u;
+                    if (((gtm >> k) & 1u) && R[k] != lastR) {
+                        atomicMin(&wf[R[k]], p);
+                        lastR = R[k];
+                    }
+                }
+            }
+            __syncthreads();
+            // queue pushes (as in k_kdexact); ones[k] becomes the "<="
+            // elements before the position in its window
+            int k1v[P], fv[P];
+            bool again = false;
+            {
+                int cR = -1, c_p1l = 0, c_p1r = 0, c_f = 0;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const int p = p0 + k;
+                    k1v[k] = 0;
+                    fv[k] = 0;
+                    if (rw[k] < 0) continue;
+                    if (R[k] != cR) {
+                        c_p1l = xp1[lw[k]];
+                        c_p1r = xp1[rw[k]];
+                        c_f = wf[R[k]];
+                        cR = R[k];
+                    }
+                    const int l = lw[k], r = rw[k], f = c_f;
+                    ones[k] -= c_p1l;
+                    k1v[k] = c_p1r - c_p1l;
+                    fv[k] = f;
+                    if (p >= l && p < r && p >= f) {
+                        if (!((lem >> k) & 1u)) {
+                            src[l + (p - f)] = (unsigned short)a[k];
+                        } else {
+                            // pop number rr takes push number rr, made at f + rr
+                            int rr = ones[k] - (f - l), q = f + rr;
+                            bool found = xp1[q + 1] == xp1[q];
+                            for (int hop = 1; hop < KD_CHASE && !found; hop++) {
+                                rr = (xp1[q] - c_p1l) - (f - l);
+                                q = f + rr;
+                                found = xp1[q + 1] == xp1[q];
+                            }
+                            if (found) {
+                                src[l + (p - f)] = (unsigned short)(q == piv[k] ? er[k] : xa[q]);
+                            } else {
+                                src[l + (p - f)] = (unsigned short)(KD_LINK | (unsigned)(l + rr));
+                                again = true;
+                            }
+                        }
+                    }
+                }
+            }
+            // pointer jumping until every push names an element; each pop
+            // names a lower push, so the chains end within jump_bound rounds
+            for (int jumps = 0; __syncthreads_or(again); jumps++) {
+                if (jumps >= jump_bound) {
+                    broken = true;
+                    break;
+                }
+                again = false;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    const int p = p0 + k;
+                    if (rw[k] < 0 || p < lw[k] || p - lw[k] >= rw[k] - fv[k]) continue;
+                    const unsigned v = src[p];
+                    if (v & KD_LINK) {
+                        const unsigned u = src[v & (KD_LINK - 1u)];
+                        src[p] = (unsigned short)u;
+                        again |= (u & KD_LINK) != 0u;
+                    }
+                }
+            }
+            if (broken) break;
+            // scatter: "<=" elements in order, the pivot after them, the
+            // queue from its second element on, its front at the right end
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                const int p = p0 + k;
+                if (rw[k] < 0) continue;
+                const int l = lw[k], r = rw[k], k1 = k1v[k];
+                if (p == r) {
+                    xa[l + k1] = (unsigned short)a[k];
+                } else if (p >= l && p < r) {
+                    if ((lem >> k) & 1u) xa[l + ones[k]] = (unsigned short)a[k];
+                    const int j = p - l;
+                    const int m = (r - l) - k1;            // ">" elements
+                    const int nrot = k1 - (fv[k] - l);     // pops
+                    if (j < m) xa[j == 0 ? r : l + k1 + j] = src[l + nrot + j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                if (p0 + k < n) a[k] = xa[p0 + k];
+                if (rw[k] < 0) continue;
+                // next window of the range's QuickSelect
+                const int pos = lw[k] + k1v[k];
+                const int kk = R[k] + (H[k] - R[k]) / 2;
+                int nl = lw[k], nr = rw[k];
+                if (pos == kk) nr = -1;
+                else if (kk < pos) nr = pos - 1;
+                else nl = pos + 1;
+                if (nr >= 0 && nl == nr) nr = -1;
+                lw[k] = nl;
+                rw[k] = nr;
+            }
+        }
+        __syncthreads();
+        if (broken) {
+            if (t == 0) atomicOr(st.kd_err, 1u);
+        } else {
+            // DFS order: position p holds element arrangement[n-1-p]
+            for (int b0 = wv * JAM_WAVE; b0 < n; b0 += NT) {
+                const int p = b0 + lane;
+                kd_emit_lane(ms, st, s, n, off, abase, p, p < n ? (int)xa[n - 1 - p] : 0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // sets larger than KD_BIG: both paths on work arrays in global scratch
 // (scr + off * KD_SCRATCH_BYTES_PER_ENTRY), KD_THREADS threads looping over
 // the positions
@@ -986,10 +1269,11 @@ __device__ static void kd_load_global(const DevMols &ms, const DevSets &st, int 
     __syncthreads();
 }
 
-// fast path of the sets of `list`; sets with a split tie go to `tied`
+// fast path of the sets of `list` larger than min_n; sets with a split tie
+// go to `tied`
 __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, DevSets st, unsigned char *scr,
                                                       const int *list, const unsigned int *count, int *tied,
-                                                      unsigned int *n_tied)
+                                                      unsigned int *n_tied, int min_n)
 {
     __shared__ int partial[KD_THREADS];
     __shared__ int flag[2];
@@ -998,6 +1282,7 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
     for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
         const int s = list[q];
         const int n = st.cnt[s];
+        if (n <= min_n) continue;
         const int64_t off = st.off[s];
         const int abase = ms.atom_off[s / tp.n_types];
         int n2 = 1;
@@ -1021,10 +1306,10 @@ __global__ void __launch_bounds__(KD_THREADS) k_kdbig(DevMols ms, DevTpls tp, De
     }
 }
 
-// exact path of the sets of `list`
+// exact path of the sets of `list` larger than min_n
 __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, DevTpls tp, DevSets st,
                                                                   unsigned char *scr, const int *list,
-                                                                  const unsigned int *count)
+                                                                  const unsigned int *count, int min_n)
 {
     __shared__ int partial[KD_EXACT_THREADS];
     __shared__ int flag[2];
@@ -1033,6 +1318,7 @@ __global__ void __launch_bounds__(KD_EXACT_THREADS) k_kdbig_exact(DevMols ms, De
     for (int q = blockIdx.x; q < cnt; q += gridDim.x) {
         const int s = list[q];
         const int n = st.cnt[s];
+        if (n <= min_n) continue;
         const int64_t off = st.off[s];
         const int abase = ms.atom_off[s / tp.n_types];
         double *cb = (double *)(scr + (size_t)off * KD_SCRATCH_BYTES_PER_ENTRY);

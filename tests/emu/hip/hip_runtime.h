@@ -258,6 +258,7 @@ inline T __builtin_amdgcn_readfirstlane(T v)
     return emu::shfl(v, 0);
 }
 
+inline int __popc(unsigned v) { return __builtin_popcount(v); }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __ffsll(long long v) { return __builtin_ffsll(v); }
 inline int __clzll(long long v) { return v == 0 ? 64 : __builtin_clzll((unsigned long long)v); }

@@ -2,7 +2,8 @@
 // the HIP emulation of hip/hip_runtime.h, in the order the library launches
 // them (launch_kd in jam_api.hip: bucketing by size tier, then per tier the
 // fast pass and the exact pass over the sets it found tied, or the exact
-// pass over every set of the 513-1024 and 257-512 tiers for one molecule),
+// pass over every set of the 513-1024 and 257-512 tiers for one molecule;
+// sets of 1,025-8,192 entries go to the LDS exact pass without a fast pass),
 // and writes each set's kd leaf order. Test infrastructure only.
 //
 //   kd_emu INPUT OUTPUT
@@ -115,7 +116,7 @@ int main(int argc, char **argv)
             case 2: emu::launch(k_kdblock<256>, g, dim3(256), ms, tp, st, (const int *)bk + o, (const unsigned *)nb + 2, td + o, nt + 2); break;
             case 3: emu::launch(k_kdblock<512>, g, dim3(512), ms, tp, st, (const int *)bk + o, (const unsigned *)nb + 3, td + o, nt + 3); break;
             case 4: emu::launch(k_kdblock<1024>, g, dim3(1024), ms, tp, st, (const int *)bk + o, (const unsigned *)nb + 4, td + o, nt + 4); break;
-            default: emu::launch(k_kdbig, g, dim3(KD_THREADS), ms, tp, st, scratch, (const int *)bk + o, (const unsigned *)nb + 5, td + o, nt + 5); break;
+            default: emu::launch(k_kdbig, g, dim3(KD_THREADS), ms, tp, st, scratch, (const int *)bk + o, (const unsigned *)nb + 5, td + o, nt + 5, KD_LDS_MAX); break;
             }
         }
         const int *lst = all ? bk + o : td + o;
@@ -126,7 +127,11 @@ int main(int argc, char **argv)
         case 2: emu::launch(k_kdexact<256>, g, dim3(256), ms, tp, st, lst, lc); break;
         case 3: emu::launch(k_kdexact<512>, g, dim3(512), ms, tp, st, lst, lc); break;
         case 4: emu::launch(k_kdexact<1024>, g, dim3(1024), ms, tp, st, lst, lc); break;
-        default: emu::launch(k_kdbig_exact, g, dim3(KD_EXACT_THREADS), ms, tp, st, scratch, lst, lc); break;
+        default:
+            emu::launch(k_kdexact_lds<KD_LDS_SMALL_THREADS, 8>, g, dim3(KD_LDS_SMALL_THREADS), ms, tp, st, (const int *)bk + o, (const unsigned *)nb + 5, KD_BIG);
+            emu::launch(k_kdexact_lds<KD_LDS_MAX / 8, 8>, g, dim3(KD_LDS_MAX / 8), ms, tp, st, (const int *)bk + o, (const unsigned *)nb + 5, KD_LDS_SMALL_THREADS * 8);
+            emu::launch(k_kdbig_exact, g, dim3(KD_EXACT_THREADS), ms, tp, st, scratch, lst, lc, KD_LDS_MAX);
+            break;
         }
     }
 

@@ -25,6 +25,10 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "kd_ref_orders.np
 SMALL = [2, 3, 4, 5, 6, 7, 8, 9, 11, 16, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128, 129]
 LARGE = [256, 257, 513, 627, 1024, 1025, 2000]
 HUGE_TIES = [512, 1024, 2000]
+# around the device's LDS limits (4,096 and 8,192 entries), appended after
+# the sets above so that those keep their coordinates
+BIG = {"int5": [4096, 4097, 6000, 8192, 8193], "int_chain": [4096, 4097, 6000, 8192, 8193],
+       "dup_pairs": [4097, 8192]}
 
 
 def walk(rng, n, step=3.8):
@@ -72,6 +76,9 @@ def plan():
         if name in ("same", "binary"):
             for n in HUGE_TIES:
                 sets.append((name, n))
+    for name, sizes in BIG.items():
+        for n in sizes:
+            sets.append((name, n))
     return sets
 
 

@@ -9,9 +9,11 @@ molecule). Their leaf orders must equal the reference QuickSelect's
 patches/qselect.h.patch), and no pass may reach its loop bound.
 
 A workgroup runs as one host thread per lane, so large sets are slow: the
-default selection covers every tie style at 9, 33, 65 and 129 entries and two
-257-entry sets (tiers 0-3, fast and exact paths). JAM_EMU_FULL=1 runs all 209
-sets in both launch orders (tens of minutes)."""
+default selection covers every tie style at 9, 33, 65 and 129 entries, two
+257-entry sets (tiers 0-3, fast and exact paths) and, in the batch launch
+order, one 1,025-entry set (the LDS exact pass of 1,025-8,192 entries).
+JAM_EMU_FULL=1 runs every set up to 8,192 entries in both launch orders (an
+hour or more)."""
 
 import os
 import shutil
@@ -43,13 +45,16 @@ def vectors():
     return z["style"], z["off"], z["q"], z["order"]
 
 
-def selection(full: bool):
+def selection(full: bool, one_molecule: bool):
     style, off, _, _ = vectors()
     n = np.diff(off)
     if full:
-        return list(range(len(n)))
+        return [i for i in range(len(n)) if int(n[i]) <= 8192]
     sel = [i for i in range(len(n)) if int(n[i]) in (9, 33, 65, 129)]
     sel += [i for i in range(len(n)) if int(n[i]) == 257 and style[i] in ("int5", "dup_pairs")]
+    if one_molecule:
+        return [i for i in sel if int(n[i]) >= 129]
+    sel += [i for i in range(len(n)) if int(n[i]) == 1025 and style[i] == "int5"]
     return sel
 
 
@@ -71,9 +76,7 @@ def run(binary, sets, one_molecule, tmp_path):
 @pytest.mark.parametrize("one_molecule", [False, True])
 def test_emulated_kd_kernels_give_the_reference_leaf_order(emulator, tmp_path, one_molecule):
     style, off, q, order = vectors()
-    sel = selection(os.environ.get("JAM_EMU_FULL") == "1")
-    if one_molecule and os.environ.get("JAM_EMU_FULL") != "1":
-        sel = [i for i in sel if int(off[i + 1] - off[i]) >= 129]
+    sel = selection(os.environ.get("JAM_EMU_FULL") == "1", one_molecule)
     sets = [q[off[i]:off[i + 1]].astype(np.float64) / 8.0 for i in sel]
     got, kd_err = run(emulator, sets, one_molecule, tmp_path)
     assert kd_err == 0

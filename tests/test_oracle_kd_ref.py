@@ -35,7 +35,10 @@ def mol_of(xyz):
 def test_vectors_cover_ties_and_sizes():
     sizes = np.diff(VEC["off"])
     assert len(sizes) >= 200
-    assert sizes.min() == 2 and sizes.max() == 2000
+    assert sizes.min() == 2 and sizes.max() == 8193
+    # both sides of the device's size limits: one position per thread up to
+    # 1,024 entries, LDS up to 4,096 and 8,192, global scratch above
+    assert {1024, 1025, 4096, 4097, 8192, 8193} <= set(sizes.tolist())
     styles = set(str(s) for s in VEC["style"])
     assert {"int5", "grid2", "int_chain", "flat_x", "dup_pairs", "same", "binary", "distinct"} <= styles
 
