@@ -137,6 +137,10 @@ struct jam_context {
     // fast passes of the other tiers (all streams at the default priority)
     hipStream_t kd_side[KD_TIERS] = {};
     hipEvent_t kd_fast[KD_TIERS] = {}, kd_joined[KD_TIERS] = {};
+    // the largest tier's sets of up to 8 * KD_LDS_SMALL_THREADS entries run
+    // on a stream of their own, beside its larger sets
+    hipStream_t kd_lds = nullptr;
+    hipEvent_t kd_lds_done = nullptr;
     bool kd_pending = false;    // side-stream work of the last kd launch may still run
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2];
@@ -332,6 +336,8 @@ extern "C" int32_t jam_context_create(int32_t device, jam_context **out)
         HIP_TRY(hipEventCreateWithFlags(&c->kd_fast[k], hipEventDisableTiming));
         HIP_TRY(hipEventCreateWithFlags(&c->kd_joined[k], hipEventDisableTiming));
     }
+    HIP_TRY(hipStreamCreateWithFlags(&c->kd_lds, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->kd_lds_done, hipEventDisableTiming));
     *out = c.release();
     return JAM_OK;
 }
@@ -363,6 +369,9 @@ extern "C" void jam_context_free(jam_context *ctx)
         if (ctx->kd_joined[k]) (void)hipEventDestroy(ctx->kd_joined[k]);
         if (ctx->kd_side[k]) (void)hipStreamDestroy(ctx->kd_side[k]);
     }
+    if (ctx->kd_lds) (void)hipStreamSynchronize(ctx->kd_lds);
+    if (ctx->kd_lds_done) (void)hipEventDestroy(ctx->kd_lds_done);
+    if (ctx->kd_lds) (void)hipStreamDestroy(ctx->kd_lds);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1352,11 +1361,10 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
             hipLaunchKernelGGL((jam::k_kdexact<1024>), grid(10), dim3(1024), 0, x, dm, dt, st, lst, lc);
             break;
         default:
-            // every set of the tier up to KD_LDS_MAX entries (most of them
-            // have a tie) in LDS, then the larger sets the fast pass listed
-            // in global scratch
-            hipLaunchKernelGGL((jam::k_kdexact_lds<KD_LDS_SMALL_THREADS, 8>), grid(12), dim3(KD_LDS_SMALL_THREADS), 0,
-                               x, dm, dt, st, bucket + o, nb + 5, KD_BIG);
+            // every set of the tier of 8 * KD_LDS_SMALL_THREADS + 1 ...
+            // KD_LDS_MAX entries (most of them have a tie) in LDS (the
+            // smaller ones run on kd_lds), then the larger sets the fast
+            // pass listed in global scratch
             hipLaunchKernelGGL((jam::k_kdexact_lds<KD_LDS_MAX / 8, 8>), grid(13), dim3(KD_LDS_MAX / 8), 0, x, dm, dt,
                                st, bucket + o, nb + 5, KD_LDS_SMALL_THREADS * 8);
             hipLaunchKernelGGL(jam::k_kdbig_exact, grid(11), dim3(KD_EXACT_THREADS), 0, x, dm, dt, st, scr, lst, lc,
@@ -1377,7 +1385,14 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
         if (!all) launch_fast(k, s);
         HIP_TRY(hipEventRecord(c->kd_fast[k], s));
         HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_fast[k], 0));
+        if (k == 5) {
+            HIP_TRY(hipStreamWaitEvent(c->kd_lds, c->kd_fast[k], 0));
+            hipLaunchKernelGGL((jam::k_kdexact_lds<KD_LDS_SMALL_THREADS, 8>), grid(12), dim3(KD_LDS_SMALL_THREADS), 0,
+                               c->kd_lds, dm, dt, st, bucket + (size_t)k * ns, nb + k, KD_BIG);
+            HIP_TRY(hipEventRecord(c->kd_lds_done, c->kd_lds));
+        }
         launch_exact(k, c->kd_side[k], all);
+        if (k == 5) HIP_TRY(hipStreamWaitEvent(c->kd_side[k], c->kd_lds_done, 0));
         HIP_TRY(hipEventRecord(c->kd_joined[k], c->kd_side[k]));
     }
     HIP_TRY(hipGetLastError());

@@ -624,13 +624,16 @@ __global__ void __launch_bounds__(NT) k_kdexact(DevMols ms, DevTpls tp, DevSets 
 
 // ---------------------------------------------------------------------------
 // exact path, sets of (min_n, NT * P] entries: the partition step of
-// k_kdexact with P consecutive positions per thread, so that a set of a few
-// thousand entries stays in LDS. Positions, elements and prefix counts are
-// 16-bit, and only the coordinate of the current depth's axis is kept
-// (reloaded from HBM at every depth): 18 bytes per entry, 144 KB for 8,192
-// entries. A thread keeps the element, range and window of each of its
-// positions in registers; consecutive positions of one range share its pivot
-// and prefix reads.
+// k_kdexact with P positions per thread, so that a set of a few thousand
+// entries stays in LDS. Wave w holds positions w * 64P ... (w + 1) * 64P - 1,
+// its lane l positions w * 64P + k * 64 + l (k < P): P slices of 64
+// consecutive positions, so that the lanes of a wave touch consecutive LDS
+// entries, and prefix counts within a wave come from ballots. Positions,
+// elements and prefix counts are 16-bit, and only the coordinate of the
+// current depth's axis is kept (reloaded from HBM at every depth): 18 bytes
+// per entry, 144 KB for 8,192 entries. A thread keeps the element, range and
+// window of each of its positions in registers; its positions in one range
+// share the range's pivot and prefix reads.
 // ---------------------------------------------------------------------------
 
 // largest set of the LDS exact pass (1,024 threads); larger ones run in
@@ -657,7 +660,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
     const int t = threadIdx.x;
     const int lane = lane_id();
     const int wv = t / JAM_WAVE;
-    const int p0 = t * P;
+    const int p0 = wv * JAM_WAVE * P + lane;   // position of slice k: p0 + k * JAM_WAVE
     if (*st.total > st.cap) return;
     const int cnt = (int)*count;
     for (int qs = blockIdx.x; qs < cnt; qs += gridDim.x) {
@@ -674,11 +677,11 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
         int a[P], R[P], H[P], lw[P], rw[P];
 #pragma unroll
         for (int k = 0; k < P; k++) {
-            a[k] = p0 + k;
+            a[k] = p0 + k * JAM_WAVE;
             R[k] = 0;
             H[k] = n;
             lw[k] = 0;
-            rw[k] = (n >= 2 && p0 + k < n) ? n - 1 : -1;
+            rw[k] = (n >= 2 && p0 + k * JAM_WAVE < n) ? n - 1 : -1;
         }
         __syncthreads();
         int hmax = n;                      // largest range at this depth
@@ -700,7 +703,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                 double c_pv = 0.0;
 #pragma unroll
                 for (int k = 0; k < P; k++) {
-                    const int p = p0 + k;
+                    const int p = p0 + k * JAM_WAVE;
                     piv[k] = -1;
                     er[k] = 0;
                     if (rw[k] < 0) continue;
@@ -727,22 +730,24 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                     }
                 }
             }
-            // block exclusive scan of the "<=" counts, and whether any window is open
-            const int mine = __popc(lem);
-            int incl = mine;
+            // prefix counts of the "<=" flags: within the wave by slice
+            // ballots, across waves through LDS; whether any window is open
+            int p1[P];
+            int wtot = 0;
 #pragma unroll
-            for (int sh = 1; sh < JAM_WAVE; sh <<= 1) {
-                const int o = __shfl_up(incl, sh, JAM_WAVE);
-                if (lane >= sh) incl += o;
+            for (int k = 0; k < P; k++) {
+                const unsigned long long bal = ballot((lem >> k) & 1u);
+                p1[k] = wtot + rank_below(bal);
+                wtot += __popcll(bal);
             }
             const unsigned long long aball = ballot(act);
-            if (lane == JAM_WAVE - 1) {
-                wsum[wv] = incl;
+            if (lane == 0) {
+                wsum[wv] = wtot;
                 wact[wv] = aball != 0ull;
             }
             __syncthreads();
             bool any = false;
-            int base = incl - mine;
+            int base = 0;
             for (int w = 0; w < NW; w++) {
                 any |= wact[w] != 0;
                 if (w < wv) base += wsum[w];
@@ -755,10 +760,10 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                 hmax -= hmax / 2;
 #pragma unroll
                 for (int k = 0; k < P; k++) {
-                    const bool live = p0 + k < n;
+                    const bool live = p0 + k * JAM_WAVE < n;
                     if (live && H[k] - R[k] >= 2) {
                         const int m = R[k] + (H[k] - R[k]) / 2;
-                        if (p0 + k < m) H[k] = m;
+                        if (p0 + k * JAM_WAVE < m) H[k] = m;
                         else R[k] = m;
                     }
                     lw[k] = R[k];
@@ -772,21 +777,20 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                 __syncthreads();
                 continue;
             }
-            // prefix counts; the first ">" of each window (a thread's first
-            // one there, its positions ascending)
+            // prefix counts; the first ">" of each window: per slice, the
+            // lowest ">" lane of the window among this wave's lanes
             int ones[P];
-            {
-                int run = base, lastR = -1;
 #pragma unroll
-                for (int k = 0; k < P; k++) {
-                    const int p = p0 + k;
-                    ones[k] = run;
-                    if (p < n) xp1[p] = (unsigned short)run;
-                    run += (lem >> k) & 1u;
-                    if (((gtm >> k) & 1u) && R[k] != lastR) {
-                        atomicMin(&wf[R[k]], p);
-                        lastR = R[k];
-                    }
+            for (int k = 0; k < P; k++) {
+                const int p = p0 + k * JAM_WAVE;
+                ones[k] = base + p1[k];
+                if (p < n) xp1[p] = (unsigned short)ones[k];
+                const bool g = (gtm >> k) & 1u;
+                const unsigned long long gball = ballot(g);
+                if (g) {
+                    const int lo = max(lw[k] - (p - lane), 0);
+                    const unsigned long long below = gball & ((1ull << lane) - 1ull) & ~((1ull << lo) - 1ull);
+                    if (below == 0ull) atomicMin(&wf[R[k]], p);
                 }
             }
             __syncthreads();
@@ -798,7 +802,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                 int cR = -1, c_p1l = 0, c_p1r = 0, c_f = 0;
 #pragma unroll
                 for (int k = 0; k < P; k++) {
-                    const int p = p0 + k;
+                    const int p = p0 + k * JAM_WAVE;
                     k1v[k] = 0;
                     fv[k] = 0;
                     if (rw[k] < 0) continue;
@@ -844,7 +848,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
                 again = false;
 #pragma unroll
                 for (int k = 0; k < P; k++) {
-                    const int p = p0 + k;
+                    const int p = p0 + k * JAM_WAVE;
                     if (rw[k] < 0 || p < lw[k] || p - lw[k] >= rw[k] - fv[k]) continue;
                     const unsigned v = src[p];
                     if (v & KD_LINK) {
@@ -859,7 +863,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
             // queue from its second element on, its front at the right end
 #pragma unroll
             for (int k = 0; k < P; k++) {
-                const int p = p0 + k;
+                const int p = p0 + k * JAM_WAVE;
                 if (rw[k] < 0) continue;
                 const int l = lw[k], r = rw[k], k1 = k1v[k];
                 if (p == r) {
@@ -875,7 +879,7 @@ __global__ void __launch_bounds__(NT, 4) k_kdexact_lds(DevMols ms, DevTpls tp, D
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < P; k++) {
-                if (p0 + k < n) a[k] = xa[p0 + k];
+                if (p0 + k * JAM_WAVE < n) a[k] = xa[p0 + k * JAM_WAVE];
                 if (rw[k] < 0) continue;
                 // next window of the range's QuickSelect
                 const int pos = lw[k] + k1v[k];
