@@ -289,6 +289,30 @@ cdef class HitList:
         return f"<HitList of {len(self)} hits>"
 
 
+def hit_lists(block, const double[:] rmsd, const int[:] pair, const long long[:] events,
+              const long long[:] bounds, int n_templates, list templates, list molecules):
+    """One `HitList` per molecule, molecule m holding events[bounds[m]:bounds[m + 1]]
+    (the result arrays are taken once for all the lists)."""
+    cdef Py_ssize_t m, n = len(molecules)
+    cdef HitList h
+    cdef list out = []
+    if bounds.shape[0] < n + 1:
+        raise ValueError("bounds must hold one more entry than there are molecules")
+    for m in range(n):
+        if not (0 <= bounds[m] <= bounds[m + 1] <= events.shape[0]):
+            raise ValueError("bounds out of range")
+        h = HitList.__new__(HitList)
+        h._block = block
+        h._rmsd = rmsd
+        h._pair = pair
+        h._events = events[bounds[m]:bounds[m + 1]]
+        h._n_templates = n_templates
+        h._templates = templates
+        h._molecule = molecules[m]
+        out.append(h)
+    return out
+
+
 cdef class _HitListIter:
     cdef HitList _list
     cdef Py_ssize_t _k

@@ -33,7 +33,8 @@ def _warn_nan():
 
 
 from ._hits import Block as _Block, Hit, HitIterator as _HitIterator, build_hits as _build_hits  # noqa: E402
-from ._hits import HitList, select_events as _select_events, split_events as _split_events  # noqa: E402
+from ._hits import HitList, hit_lists as _hit_lists, select_events as _select_events  # noqa: E402
+from ._hits import split_events as _split_events  # noqa: E402
 
 import collections.abc as _abc  # noqa: E402
 
@@ -140,9 +141,8 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         hit_ev, bounds, nan = _split_events(pair, np.ascontiguousarray(res.flag, dtype=np.int32), chosen, T,
                                             len(molecules))
         rmsd = np.ascontiguousarray(res.rmsd, dtype=np.float64)
-        templates = list(jess._templates)
-        out = [HitList(block, rmsd, pair, hit_ev[bounds[m]:bounds[m + 1]], T, templates, mol)
-               for m, mol in enumerate(molecules)]
+        out = _hit_lists(block, rmsd, pair, np.ascontiguousarray(hit_ev, dtype=np.int64),
+                         np.ascontiguousarray(bounds, dtype=np.int64), T, list(jess._templates), list(molecules))
         return out, nan
     nan = _build_hits(out, chosen, (pair // T).astype(np.int32), (pair % T).astype(np.int32),
                       np.ascontiguousarray(res.flag, dtype=np.int32),

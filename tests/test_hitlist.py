@@ -65,3 +65,32 @@ def test_split_events_cuts_by_molecule_and_drops_nan():
     assert ev.tolist() == [3, 6]
     assert bounds.tolist() == [0, 0, 1, 1, 2, 2]
     assert nan.tolist() == [1, 0, 0, 0, 0]
+
+
+def test_hit_lists_equal_one_hitlist_per_molecule():
+    """`hit_lists` (all molecules' lists from one pass over the result
+    arrays) gives the lists `HitList` gives one by one."""
+    from pyjess_amd._hits import hit_lists, split_events
+
+    mol = Molecule.load(golden("1AMY.pdb"))
+    mols = [mol, mol, mol, mol]
+    tpls = [Template.load(golden("1.3.3.tpl")), Template.load(golden("4.1.2.tpl"))]
+    T = 2
+    pair = np.array([0, 1, 1, 4, 5, 6, 7, 7], dtype=np.int32)   # molecules 0, 0, 0, 2, 2, 3, 3, 3
+    flag = np.array([1, 1, 2, 1, 1, 1, 2, 1], dtype=np.int32)
+    n_ev = len(pair)
+    block = _Block(np.zeros((n_ev, 9)), np.zeros((n_ev, 3)), np.zeros((n_ev, 3)), np.zeros((n_ev, 8), np.int32))
+    rmsd = np.linspace(0.1, 0.8, n_ev)
+    ev, bounds, nan = split_events(pair, flag, None, T, len(mols))
+    got = hit_lists(block, rmsd, pair, ev, bounds, T, tpls, mols)
+    want = [HitList(block, rmsd, pair, ev[bounds[m]:bounds[m + 1]], T, tpls, mols[m]) for m in range(len(mols))]
+    assert [len(x) for x in got] == [2, 0, 2, 2]
+    assert got == want
+    for g, w in zip(got, want):
+        assert [(h.rmsd, h.template()) for h in g] == [(h.rmsd, h.template()) for h in w]
+    with pytest.raises(ValueError):
+        hit_lists(block, rmsd, pair, ev, bounds[:-1], T, tpls, mols)
+    bad = bounds.copy()
+    bad[2] = len(ev) + 1
+    with pytest.raises(ValueError):
+        hit_lists(block, rmsd, pair, ev, bad, T, tpls, mols)
