@@ -212,15 +212,19 @@ def build_hits(list out, const long long[:] chosen, const int[:] mol_of, const i
 
 cdef class HitList:
     """The hits of one molecule of `Jess.query_batch`, in pyjess order: a
-    read-only sequence over rows of the batch's shared result arrays. A `Hit`
-    object is made when an element is read (a library-scale batch has tens of
-    millions of hits, most of which a caller never touches); reading the same
-    element twice gives two equal-valued `Hit` objects."""
+    read-only sequence over rows of the batch's shared result arrays (a list
+    of rows, or a contiguous range of them). A `Hit` object is made when an
+    element is read (a library-scale batch has tens of millions of hits, most
+    of which a caller never touches); reading the same element twice gives two
+    equal-valued `Hit` objects."""
 
     cdef object _block
     cdef const double[:] _rmsd
     cdef const int[:] _pair
-    cdef const long long[:] _events
+    cdef const long long[:] _events   # the rows, unless _range
+    cdef bint _range                   # rows _start ... _start + _n - 1
+    cdef Py_ssize_t _start
+    cdef Py_ssize_t _n
     cdef int _n_templates
     cdef list _templates
     cdef object _molecule
@@ -231,12 +235,18 @@ cdef class HitList:
         self._rmsd = rmsd
         self._pair = pair
         self._events = events
+        self._range = False
+        self._start = 0
+        self._n = events.shape[0]
         self._n_templates = n_templates
         self._templates = templates
         self._molecule = molecule
 
+    cdef inline Py_ssize_t _row(self, Py_ssize_t k):
+        return self._start + k if self._range else self._events[k]
+
     cdef Hit _hit(self, Py_ssize_t k):
-        cdef Py_ssize_t e = self._events[k]
+        cdef Py_ssize_t e = self._row(k)
         cdef Hit h = Hit.__new__(Hit)
         h.rmsd = self._rmsd[e]
         h._block = self._block
@@ -246,10 +256,10 @@ cdef class HitList:
         return h
 
     def __len__(self):
-        return self._events.shape[0]
+        return self._n
 
     def __getitem__(self, index):
-        cdef Py_ssize_t n = self._events.shape[0], k
+        cdef Py_ssize_t n = self._n, k
         if isinstance(index, slice):
             return [self._hit(k) for k in range(*index.indices(n))]
         k = index
@@ -263,7 +273,7 @@ cdef class HitList:
         return _HitListIter(self)
 
     def __bool__(self):
-        return self._events.shape[0] > 0
+        return self._n > 0
 
     def __eq__(self, other):
         try:
@@ -280,7 +290,7 @@ cdef class HitList:
             if o._block is not self._block:
                 return False
             for k in range(n):
-                if o._events[k] != self._events[k]:
+                if o._row(k) != self._row(k):
                     return False
             return True
         return all(a == b for a, b in zip(self, other))
@@ -306,9 +316,165 @@ def hit_lists(block, const double[:] rmsd, const int[:] pair, const long long[:]
         h._rmsd = rmsd
         h._pair = pair
         h._events = events[bounds[m]:bounds[m + 1]]
+        h._range = False
+        h._start = 0
+        h._n = bounds[m + 1] - bounds[m]
         h._n_templates = n_templates
         h._templates = templates
         h._molecule = molecules[m]
+        out.append(h)
+    return out
+
+
+cdef Py_ssize_t _lower_bound(const long long[:] v, Py_ssize_t lo, Py_ssize_t hi, long long x):
+    """First index in [lo, hi) of the sorted v whose value is >= x."""
+    cdef Py_ssize_t mid
+    while lo < hi:
+        mid = lo + (hi - lo) // 2
+        if v[mid] < x:
+            lo = mid + 1
+        else:
+            hi = mid
+    return lo
+
+
+@cython.cdivision(True)
+def split_ranges(const int[:] pair, const int[:] flag, int n_templates, int n_mols):
+    """Every event's molecule range when no policy drops events: the events
+    come grouped by molecule in order (pair // n_templates never decreases),
+    so molecule m's events are bounds[m]:bounds[m+1], found by bisection
+    (the first and last event of every range are checked to belong to it;
+    events out of that order elsewhere are not detected). Also the
+    NaN-rotation events (flag 2, which are left out of the hit lists), in
+    order, and their number per molecule."""
+    cdef Py_ssize_t n = pair.shape[0], i, lo, hi, mid, k = 0, cap = 256
+    cdef int m
+    if n_templates <= 0:
+        raise ValueError("no templates")
+    bounds = np.zeros(n_mols + 1, dtype=np.int64)
+    cdef long long[:] b = bounds
+    for m in range(n_mols + 1):
+        lo, hi = 0, n
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            if pair[mid] // n_templates < m:
+                lo = mid + 1
+            else:
+                hi = mid
+        b[m] = lo
+    b[n_mols] = n
+    if n > 0 and b[0] != 0:
+        raise ValueError("events are not grouped by molecule")
+    for m in range(n_mols):
+        if b[m + 1] > b[m] and (pair[b[m]] // n_templates != m or pair[b[m + 1] - 1] // n_templates != m):
+            raise ValueError("events are not grouped by molecule")
+    nan_ev = np.empty(cap, dtype=np.int64)
+    cdef long long[:] ne = nan_ev
+    for i in range(n):
+        if flag[i] == 2:
+            if k == cap:
+                cap *= 2
+                nan_ev = np.resize(nan_ev, cap)
+                ne = nan_ev
+            ne[k] = i
+            k += 1
+    nan_ev = nan_ev[:k]
+    ne = nan_ev
+    nan = np.zeros(n_mols, dtype=np.int64)
+    cdef long long[:] nn = nan
+    for m in range(n_mols):
+        nn[m] = _lower_bound(ne, 0, k, b[m + 1]) - _lower_bound(ne, 0, k, b[m])
+    return bounds, nan, nan_ev
+
+
+@cython.cdivision(True)
+def pair_ranges(const int[:] pair, int n_templates, const long long[:] pairs):
+    """[starts[j], ends[j]): the events of pair pairs[j] in a stream ordered
+    by molecule, then template in scan order (reverse index): the key
+    pair - t + (n_templates - 1 - t), t = pair % n_templates, never decreases,
+    so each pair's events are found by bisection (an empty range when it has
+    none; ValueError when the ends of a range belong to another pair)."""
+    cdef Py_ssize_t n = pair.shape[0], j, lo, hi, mid, k = pairs.shape[0]
+    cdef long long target, key, p
+    cdef int t
+    if n_templates <= 0:
+        raise ValueError("no templates")
+    starts = np.zeros(k, dtype=np.int64)
+    ends = np.zeros(k, dtype=np.int64)
+    cdef long long[:] st = starts
+    cdef long long[:] en = ends
+    for j in range(k):
+        p = pairs[j]
+        if p < 0:
+            raise ValueError("negative pair")
+        target = p - p % n_templates + (n_templates - 1 - p % n_templates)
+        lo, hi = 0, n
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            t = pair[mid] % n_templates
+            key = pair[mid] - t + (n_templates - 1 - t)
+            if key < target:
+                lo = mid + 1
+            else:
+                hi = mid
+        st[j] = lo
+        hi = n
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            t = pair[mid] % n_templates
+            key = pair[mid] - t + (n_templates - 1 - t)
+            if key <= target:
+                lo = mid + 1
+            else:
+                hi = mid
+        en[j] = lo
+        if en[j] > st[j] and (pair[st[j]] != p or pair[en[j] - 1] != p):
+            raise ValueError("events are not in report order")
+    return starts, ends
+
+
+def hit_list_ranges(block, const double[:] rmsd, const int[:] pair, const long long[:] bounds,
+                    const long long[:] nan_events, int n_templates, list templates, list molecules):
+    """One `HitList` per molecule holding its events bounds[m]:bounds[m + 1]
+    except the NaN-rotation ones (sorted ``nan_events``): a range of rows
+    when it has none, else the list of its other rows."""
+    cdef Py_ssize_t m, n = len(molecules), b0, b1, j0, j1, e, w, k = nan_events.shape[0]
+    cdef HitList h
+    cdef list out = []
+    cdef long long[:] r
+    if bounds.shape[0] < n + 1:
+        raise ValueError("bounds must hold one more entry than there are molecules")
+    for m in range(n):
+        b0, b1 = bounds[m], bounds[m + 1]
+        if not (0 <= b0 <= b1 <= rmsd.shape[0] and b1 <= pair.shape[0]):
+            raise ValueError("bounds out of range")
+        j0 = _lower_bound(nan_events, 0, k, b0)
+        j1 = _lower_bound(nan_events, j0, k, b1)
+        h = HitList.__new__(HitList)
+        h._block = block
+        h._rmsd = rmsd
+        h._pair = pair
+        h._n_templates = n_templates
+        h._templates = templates
+        h._molecule = molecules[m]
+        if j1 == j0:
+            h._range = True
+            h._start = b0
+            h._n = b1 - b0
+        else:
+            rows = np.empty(b1 - b0 - (j1 - j0), dtype=np.int64)
+            r = rows
+            w = 0
+            for e in range(b0, b1):
+                if j0 < j1 and nan_events[j0] == e:
+                    j0 += 1
+                    continue
+                r[w] = e
+                w += 1
+            h._events = rows
+            h._range = False
+            h._start = 0
+            h._n = w
         out.append(h)
     return out
 
@@ -325,7 +491,7 @@ cdef class _HitListIter:
         return self
 
     def __next__(self):
-        if self._k >= self._list._events.shape[0]:
+        if self._k >= self._list._n:
             raise StopIteration
         self._k += 1
         return self._list._hit(self._k - 1)

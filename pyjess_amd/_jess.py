@@ -33,8 +33,9 @@ def _warn_nan():
 
 
 from ._hits import Block as _Block, Hit, HitIterator as _HitIterator, build_hits as _build_hits  # noqa: E402
-from ._hits import HitList, hit_lists as _hit_lists, select_events as _select_events  # noqa: E402
-from ._hits import split_events as _split_events  # noqa: E402
+from ._hits import HitList, hit_list_ranges as _hit_list_ranges, hit_lists as _hit_lists  # noqa: E402
+from ._hits import select_events as _select_events, split_events as _split_events  # noqa: E402
+from ._hits import pair_ranges as _pair_ranges, split_ranges as _split_ranges  # noqa: E402
 
 import collections.abc as _abc  # noqa: E402
 
@@ -96,6 +97,35 @@ class Query(_HitIterator):
         return _policy(self.jess, [self.molecule], res, self.best_match, self._max_candidates)[0][0]
 
 
+def _cut_events(res, pair, flag, rmsd, T: int, mc: int, nan_ev, nan):
+    """The max_candidates cap without best_match, for lazy hit lists: the
+    counter of ``Query.__next__`` counts the complete matches since the last
+    yielded hit and can only pass ``mc + 1`` in a pair with more complete
+    matches than that, so only those pairs go through `select_events`. Returns
+    the events to leave out of the hit lists (every NaN-rotation event, and
+    the hits the cap drops) and the NaN warnings per molecule (a NaN event
+    the cap drops gives none)."""
+    pairs = np.flatnonzero(np.asarray(res.pair_matches) > mc + 1).astype(np.int64)
+    if len(pairs) == 0:
+        return nan_ev, nan
+    starts, ends = _pair_ranges(pair, T, pairs)
+    if not np.any(ends > starts):
+        return nan_ev, nan
+    rank = np.ascontiguousarray(res.ranks() if hasattr(res, "ranks") else res.rank, dtype=np.int64)
+    nan = nan.copy()
+    leave = [nan_ev]
+    for p, a, b in zip(pairs.tolist(), starts.tolist(), ends.tolist()):
+        if a == b:
+            continue
+        kept = _select_events(pair[a:b], rank[a:b], flag[a:b], rmsd[a:b], False, mc)
+        mask = np.ones(b - a, dtype=bool)
+        mask[kept] = False
+        dropped = np.flatnonzero(mask) + a
+        nan[p // T] -= int(np.count_nonzero(flag[dropped] == 2))
+        leave.append(dropped[flag[dropped] != 2])
+    return np.unique(np.concatenate(leave)).astype(np.int64), nan
+
+
 def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", best_match: bool,
             max_candidates: int, keep_none: bool = True):
     """Hits per molecule, in pyjess order, from the device events.
@@ -122,13 +152,19 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
     # matches, so a cap at or above the largest one never cuts anything
     if mc != -1 and len(res.pair_matches) and mc >= int(res.pair_matches.max()):
         mc = -1
-    if not best_match and mc == -1:
+    # without best_match, lazy lists apply a candidate cap to the few pairs it
+    # can cut, after cutting the events into molecules (_cut_events)
+    cut_late = not best_match and mc != -1 and not keep_none
+    if (not best_match and mc == -1) or cut_late:
         # no policy can drop an event: every event is reported, in order
         chosen = None if not keep_none else np.arange(n, dtype=np.int64)
     else:
-        chosen = _select_events(pair, np.ascontiguousarray(res.rank, dtype=np.int64),
+        rank = res.ranks() if hasattr(res, "ranks") else res.rank
+        chosen = _select_events(pair, np.ascontiguousarray(rank, dtype=np.int64),
                                 np.ascontiguousarray(res.flag, dtype=np.int32),
                                 np.ascontiguousarray(res.rmsd, dtype=np.float64), best_match, mc)
+    if cut_late and hasattr(res, "ranks") and len(res.pair_matches) and int(np.max(res.pair_matches)) > mc + 1:
+        res.ranks()   # the cap reads them (_cut_events): before the device copy is handed on
     if getattr(res, "kept", None) is not None:
         # the block owns the device copy from here (released once read)
         block = _native.LazyBlock(res.kept)
@@ -137,10 +173,17 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         block = _Block(res.rotation, res.c0, res.c1, res.atoms)
     if not keep_none:
         # lazy lists: the chosen hits (NaN rotations dropped and counted), cut
-        # by molecule (events are grouped by molecule, in order)
-        hit_ev, bounds, nan = _split_events(pair, np.ascontiguousarray(res.flag, dtype=np.int32), chosen, T,
-                                            len(molecules))
+        # by molecule (events are grouped by molecule, in order); with every
+        # event chosen, each molecule's hits are a range of the events
+        flag = np.ascontiguousarray(res.flag, dtype=np.int32)
         rmsd = np.ascontiguousarray(res.rmsd, dtype=np.float64)
+        if chosen is None:
+            bounds, nan, nan_ev = _split_ranges(pair, flag, T, len(molecules))
+            if cut_late:
+                nan_ev, nan = _cut_events(res, pair, flag, rmsd, T, mc, nan_ev, nan)
+            out = _hit_list_ranges(block, rmsd, pair, bounds, nan_ev, T, list(jess._templates), list(molecules))
+            return out, nan
+        hit_ev, bounds, nan = _split_events(pair, flag, chosen, T, len(molecules))
         out = _hit_lists(block, rmsd, pair, np.ascontiguousarray(hit_ev, dtype=np.int64),
                          np.ascontiguousarray(bounds, dtype=np.int64), T, list(jess._templates), list(molecules))
         return out, nan

@@ -419,6 +419,13 @@ class KeptEvents:
         self.n = n
         self.stride = stride
 
+    def ranks(self):
+        """The rank of every event, copied to host memory."""
+        rank = np.empty(max(self.n, 1), dtype=np.int64)
+        with self.ctx.lock:
+            _check(lib().jam_events_copy(self.handle, None, _ptr(rank), None, None, None, None, None, None))
+        return rank[:self.n]
+
     def heavy(self):
         """(rotation, c0, c1, atoms) of every event, copied to host memory."""
         m = max(self.n, 1)
@@ -478,8 +485,9 @@ class LazyBlock:
 
 class Result:
     """Events of one device query, as numpy arrays. With ``keep``, only the
-    fields the hit policy reads (pair, rank, flag, rmsd) are copied now; the
-    others stay in a device copy of the events (`LazyBlock`)."""
+    fields every hit list reads (pair, flag, rmsd) are copied now; the others
+    stay in a device copy of the events (`LazyBlock`), the ranks (which only
+    a policy that can drop events reads) until `ranks` is called."""
 
     __slots__ = ("pair", "rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms", "pair_matches",
                  "timings_ms", "join_kernel_ms", "join_launches", "level_partials", "n_pairs", "stride", "kept")
@@ -492,17 +500,18 @@ class Result:
         m = max(n, 1)
         # every row is written by the copy
         self.pair = np.empty(m, dtype=np.int32)
-        self.rank = np.empty(m, dtype=np.int64)
         self.flag = np.empty(m, dtype=np.int32)
         self.rmsd = np.empty(m, dtype=np.float64)
         self.kept = None
         if keep:
             self.kept = KeptEvents(ctx, handle, n, self.stride)
             self.rotation = self.c0 = self.c1 = self.atoms = None
-            _check(L.jam_result_copy(handle, _ptr(self.pair), _ptr(self.rank), _ptr(self.flag),
-                                     _ptr(self.rmsd), None, None, None, None))
-            names = ("pair", "rank", "flag", "rmsd")
+            self.rank = None
+            _check(L.jam_result_copy(handle, _ptr(self.pair), None, _ptr(self.flag), _ptr(self.rmsd), None, None,
+                                     None, None))
+            names = ("pair", "flag", "rmsd")
         else:
+            self.rank = np.empty(m, dtype=np.int64)
             self.rotation = np.empty((m, 9), dtype=np.float64)
             self.c0 = np.empty((m, 3), dtype=np.float64)
             self.c1 = np.empty((m, 3), dtype=np.float64)
@@ -525,6 +534,15 @@ class Result:
         self.join_launches = nl.value
         self.level_partials = np.zeros(32, dtype=np.int64)   # partials expanded per join level
         _check(L.jam_result_level_partials(handle, _ptr(self.level_partials)))
+
+    def ranks(self) -> np.ndarray:
+        """The events' ranks (from the device copy when they were not copied
+        with the other fields; read them before the copy is handed on)."""
+        if self.rank is None:
+            if self.kept is None:
+                raise RuntimeError("the ranks were not copied and the device copy is gone")
+            self.rank = self.kept.ranks()
+        return self.rank
 
 
 def query(tset: TemplateSet, mset: MoleculeSet, rmsd_threshold: float, distance_cutoff: float,

@@ -148,17 +148,25 @@ def test_multichain_flexible(ignore_chain):
     assert_same_hits(mol, tpls, got, want)
 
 
-def test_query_batch_equals_single_queries():
+@pytest.mark.parametrize("max_candidates,best_match", [(None, False), (3, False), (1, False), (3, True)])
+def test_query_batch_equals_single_queries(max_candidates, best_match):
+    """query_batch (lazy lists; a candidate cap without best_match applied to
+    the pairs it can cut only) equals the oracle and single queries, also
+    with caps small enough to cut pairs."""
     mols, tpls, kw = synthetic.workload("batch-4k", seed=3, scale=6 / 4096)
     args = (kw["rmsd_threshold"], kw["distance_cutoff"], kw["max_dynamic_distance"])
+    mc = kw["max_candidates"] if max_candidates is None else max_candidates
     jess = Jess(tpls)
-    batch = jess.query_batch(mols, *args, max_candidates=kw["max_candidates"])
+    batch = jess.query_batch(mols, *args, max_candidates=mc, best_match=best_match)
     assert len(batch) == len(mols)
+    if max_candidates is not None:
+        # some pair has more complete matches than the cap lets through
+        assert int(jess._last_result.pair_matches.max()) > mc + 1
     total = 0
     for mol, got in zip(mols, batch):
-        want = oracle.hits(mol, tpls, *args, max_candidates=kw["max_candidates"])
+        want = oracle.hits(mol, tpls, *args, max_candidates=mc, best_match=best_match)
         assert_same_hits(mol, tpls, got, want)
-        single = list(jess.query(mol, *args, max_candidates=kw["max_candidates"]))
+        single = list(jess.query(mol, *args, max_candidates=mc, best_match=best_match))
         assert [key(mol, h) for h in single] == [key(mol, h) for h in got]
         total += len(want)
     assert total > 0

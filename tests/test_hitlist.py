@@ -94,3 +94,53 @@ def test_hit_lists_equal_one_hitlist_per_molecule():
     bad[2] = len(ev) + 1
     with pytest.raises(ValueError):
         hit_lists(block, rmsd, pair, ev, bad, T, tpls, mols)
+
+
+def test_hit_list_ranges_equal_the_explicit_rows():
+    """With every event chosen, `split_ranges` + `hit_list_ranges` (a range
+    of rows per molecule, explicit rows only where NaN-rotation events are
+    left out) give the lists `split_events` + `hit_lists` give."""
+    from pyjess_amd._hits import hit_list_ranges, hit_lists, split_events, split_ranges
+
+    mol = Molecule.load(golden("1AMY.pdb"))
+    mols = [mol] * 5
+    tpls = [Template.load(golden("1.3.3.tpl")), Template.load(golden("4.1.2.tpl")),
+            Template.load(golden("1.3.3.tpl"))]
+    T = 3
+    rng = np.random.default_rng(7)
+    m_of = np.sort(rng.choice([0, 1, 3, 4], size=60)).astype(np.int32)   # molecule 2: no events
+    pair = (m_of * T + rng.integers(0, T, size=60)).astype(np.int32)
+    flag = np.ones(60, dtype=np.int32)
+    flag[[0, 5, 6, 59]] = 2                                              # NaN rotations
+    flag[m_of == 3] = 1                                                  # molecule 3: none
+    n_ev = len(pair)
+    block = _Block(np.zeros((n_ev, 9)), np.zeros((n_ev, 3)), np.zeros((n_ev, 3)), np.zeros((n_ev, 8), np.int32))
+    rmsd = rng.random(n_ev)
+    ev, bounds, nan = split_events(pair, flag, None, T, len(mols))
+    want = hit_lists(block, rmsd, pair, ev, bounds, T, tpls, mols)
+    b2, nan2, nan_ev = split_ranges(pair, flag, T, len(mols))
+    got = hit_list_ranges(block, rmsd, pair, b2, nan_ev, T, tpls, mols)
+    assert np.array_equal(nan, nan2)
+    assert list(nan_ev) == [i for i in range(n_ev) if flag[i] == 2]
+    assert [len(x) for x in got] == [len(x) for x in want]
+    assert len(got[2]) == 0 and not got[2]
+    assert got == want and want == got
+    for g, w in zip(got, want):
+        # the RMSDs are distinct: they name the rows
+        assert [(h.rmsd, h.template()) for h in g] == [(h.rmsd, h.template()) for h in w]
+        if len(w):
+            assert g[-1].rmsd == w[-1].rmsd and [h.rmsd for h in g[1:]] == [h.rmsd for h in w[1:]]
+        with pytest.raises(IndexError):
+            g[len(w)]
+
+
+def test_split_ranges_rejects_events_out_of_molecule_order():
+    from pyjess_amd._hits import split_ranges
+
+    flag = np.ones(4, dtype=np.int32)
+    with pytest.raises(ValueError):
+        split_ranges(np.array([3, 0, 4, 5], dtype=np.int32), flag, 3, 2)   # molecules 1, 0, 1, 1
+    with pytest.raises(ValueError):
+        split_ranges(np.array([0, 1, 3, 9], dtype=np.int32), flag, 3, 2)   # molecule 3 of 2
+    bounds, nan, nan_ev = split_ranges(np.zeros(0, dtype=np.int32), np.zeros(0, dtype=np.int32), 3, 2)
+    assert list(bounds) == [0, 0, 0] and list(nan) == [0, 0] and len(nan_ev) == 0

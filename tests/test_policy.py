@@ -78,3 +78,34 @@ def test_policy_matches_query_loop(seed, best_match, max_candidates):
     assert [(jess._templates.index(h.template()), tuple(int(a.serial) - 1 for a in h.atoms(False)))
             for h in got_hits] == [(m.template, m.atoms) for m in want]
     assert [h.rmsd for h in got_hits] == [m.rmsd for m in want]
+
+
+@pytest.mark.parametrize("seed", range(30))
+@pytest.mark.parametrize("best_match", [False, True])
+@pytest.mark.parametrize("max_candidates", [None, 0, 1, 2, 3, 7])
+def test_lazy_hit_lists_match_query_loop(seed, best_match, max_candidates):
+    """The lazy lists of `query_batch` (keep_none=False: NaN rotations left
+    out and counted; without best_match the candidate cap is applied only to
+    the pairs with more than max_candidates + 1 complete matches) against the
+    same loop, for two molecules in one batch."""
+    rng = np.random.default_rng(1000 + seed)
+    thr = 2.0
+    n_t = int(rng.integers(1, 6))
+    streams = [random_stream(rng, n_t, thr) for _ in range(2)]
+    parts = [events_of(s, thr, n_t) for s in streams]
+    res = types.SimpleNamespace()
+    for name in ("rank", "flag", "rmsd", "rotation", "c0", "c1", "atoms"):
+        setattr(res, name, np.concatenate([getattr(r, name) for r in parts]))
+    res.pair = np.concatenate([r.pair + m * n_t for m, r in enumerate(parts)]).astype(np.int32)
+    res.pair_matches = np.concatenate([r.pair_matches for r in parts])
+    jess = Jess([two_atom_template(i) for i in range(n_t)])
+    mol = synthetic.protein(np.random.Generator(np.random.PCG64(0)), 400)
+    mc = -1 if max_candidates is None else max_candidates
+    lists, nan = _policy(jess, [mol, mol], res, best_match, mc, keep_none=False)
+    for m, stream in enumerate(streams):
+        want, warns = oracle.replay(stream, thr, best_match, max_candidates)
+        got = list(lists[m])
+        assert int(nan[m]) == warns
+        assert [(jess._templates.index(h.template()), tuple(int(a.serial) - 1 for a in h.atoms(False)))
+                for h in got] == [(w.template, w.atoms) for w in want]
+        assert [h.rmsd for h in got] == [w.rmsd for w in want]
