@@ -337,6 +337,11 @@ struct PairCache {
     int flags[NE];     // its template atom's chainID2 (bit 0) and resSeq (bit 1) equal the new one's
 };
 
+// levels up to which a team's lanes read its placed atoms back from LDS to
+// form the join box, instead of combining partial boxes by shuffles
+#ifndef JOIN_BOX_DIRECT
+#define JOIN_BOX_DIRECT 12
+#endif
 template <int MAXN, int TW>
 __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const DevSets &st, const DevPairs &pr,
                                  const DevJoin &jn, const QueryParams &qp, JoinWaveLds<MAXN, TW> &S,
@@ -415,8 +420,10 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         M.px[j] = en.x;
         M.py[j] = en.y;
         M.pz[j] = en.z;
-        M.pk[j] = make_int4(en.ch, en.res, pc.flags[jj], 0);
         const double b = pc.b[jj];
+        // .w: the outer radius rounded up to float (the join box is a
+        // prefilter: a box a little larger finds the same candidates)
+        M.pk[j] = make_int4(en.ch, en.res, pc.flags[jj], __float_as_int(__double2float_ru(b)));
         lo[0] = fmax(lo[0], en.x - b);
         lo[1] = fmax(lo[1], en.y - b);
         lo[2] = fmax(lo[2], en.z - b);
@@ -424,12 +431,36 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         hi[1] = fmin(hi[1], en.y + b);
         hi[2] = fmin(hi[2], en.z + b);
     }
-    // join box of the team (Join_computeBox, patches/Join.c.patch:154-196)
-    for (int sh = TW / 2; sh >= 1; sh >>= 1) {
+    // join box of the team (Join_computeBox, patches/Join.c.patch:154-196):
+    // with few placed atoms (the level is the same for every row of a
+    // launch) every lane reads them back from the team's LDS, else the lanes
+    // combine their partial boxes by shuffles
+    if (L <= JOIN_BOX_DIRECT) {
+        wave_sync();
+        if (live) {
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
-            hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
+            for (int d = 0; d < 3; d++) {
+                lo[d] = -INFINITY;
+                hi[d] = INFINITY;
+            }
+            for (int j = 0; j < L; j++) {
+                const double x = M.px[j], y = M.py[j], z = M.pz[j];
+                const double b = (double)__int_as_float(M.pk[j].w);
+                lo[0] = fmax(lo[0], x - b);
+                lo[1] = fmax(lo[1], y - b);
+                lo[2] = fmax(lo[2], z - b);
+                hi[0] = fmin(hi[0], x + b);
+                hi[1] = fmin(hi[1], y + b);
+                hi[2] = fmin(hi[2], z + b);
+            }
+        }
+    } else {
+        for (int sh = TW / 2; sh >= 1; sh >>= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fmax(lo[d], __shfl_xor(lo[d], sh, JAM_WAVE));
+                hi[d] = fmin(hi[d], __shfl_xor(hi[d], sh, JAM_WAVE));
+            }
         }
     }
     wave_sync();
