@@ -337,6 +337,15 @@ struct PairCache {
     int flags[NE];     // its template atom's chainID2 (bit 0) and resSeq (bit 1) equal the new one's
 };
 
+// Row of item `it` when each row is shared by `parts` teams: with one part
+// (every frontier of at least one row per team slot) the item is the row;
+// otherwise parts <= slots / rows, so the items number at most the team
+// slots of the grid and fit in 32 bits: no 64-bit division is needed.
+__device__ static inline long long row_of(long long it, int parts)
+{
+    return parts == 1 ? it : (long long)((unsigned int)it / (unsigned int)parts);
+}
+
 // levels up to which a team's lanes read its placed atoms back from LDS to
 // form the join box, instead of combining partial boxes by shuffles
 #ifndef JOIN_BOX_DIRECT
@@ -353,7 +362,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
     const int team = lane / TW, tl = lane % TW;
     JoinTeamLds<MAXN> &M = S.tm[team];
     const int L = jn.level;
-    const long long i = active ? it / parts : 0;
+    const long long i = active ? row_of(it, parts) : 0;
     const int part = active ? (int)(it - i * parts) : 0;
     constexpr int NE = (MAXN + TW - 1) / TW;
     if (active && p != pc.p) {
@@ -620,10 +629,11 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
     // R over consecutive rows (so that a team meets rows of one pair one
     // after the other), when the frontier is large enough for every wave to
     // have several runs
-    const long long R = items >= 4 * slots * (long long)JOIN_RUN ? JOIN_RUN : 1;
+    static_assert((JOIN_RUN & (JOIN_RUN - 1)) == 0, "runs of a power of two rounds");
+    const int rsh = items >= 4 * slots * (long long)JOIN_RUN ? __builtin_ctz(JOIN_RUN) : 0;   // log2 of the run
     auto round_base = [&](long long i) {
-        const long long k = i / R, j = i - k * R;
-        return ((k * xcds + xcd) * per + tslot) * R + j * NTEAM;
+        const long long k = i >> rsh, j = i & ((1ll << rsh) - 1);
+        return (((k * xcds + xcd) * per + tslot) << rsh) + j * NTEAM;
     };
     // the pair and this lane's entries of the team's next row are loaded
     // while the current one is processed
@@ -635,7 +645,7 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
 #pragma unroll
         for (int jj = 0; jj < NE; jj++) e_next[jj] = 0;
         if (it >= items) return;
-        const long long r = it / parts;
+        const long long r = row_of(it, parts);
         p_next = jn.in_pair[r];
 #pragma unroll
         for (int jj = 0; jj < NE; jj++) {
