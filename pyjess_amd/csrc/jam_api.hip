@@ -1289,8 +1289,9 @@ int32_t launch_kd(jam_context *c, const DevMols &dm, const DevTpls &dt, const De
     unsigned int *nt = nb + KD_TIERS;
     const size_t ns = (size_t)n_sets;
     hipLaunchKernelGGL(k_kd_setup, dim3(1), dim3(64), 0, s, nb, 2 * KD_TIERS, st.kd_err);
-    hipLaunchKernelGGL(jam::k_kdbucket, dim3(std::min((n_sets + 255) / 256, 1024)), dim3(256), 0, s, st, bucket,
-                       nb);
+    hipLaunchKernelGGL(jam::k_kdbucket,
+                       dim3(std::max(1, std::min((n_sets + KD_BUCKET_SPAN - 1) / KD_BUCKET_SPAN, 1024))), dim3(256),
+                       0, s, st, bucket, nb);
     HIP_TRY(hipGetLastError());
     if (c->kd_grid[0] == 0) {
         const int d = c->device;

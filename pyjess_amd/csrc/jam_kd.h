@@ -86,23 +86,47 @@ __device__ static inline bool key_less(double a, int ia, double b, int ib)
 }
 
 // Buckets the sets by size tier: bucket[k * n_sets + i] is the i-th set of
-// tier k (in no particular order), nb[k] their number.
+// tier k (in no particular order), nb[k] their number. Each workgroup takes
+// one contiguous span of sets: it counts its sets per tier in LDS (one LDS
+// atomic per wave and tier), reserves room in each tier's list with one
+// global atomic per tier, then places its sets there.
+#define KD_BUCKET_SPAN 4096
 __global__ void __launch_bounds__(256) k_kdbucket(DevSets st, int *bucket, unsigned int *nb)
 {
+    __shared__ unsigned int cnt[KD_TIERS], base[KD_TIERS];
     const int lane = lane_id();
-    const int stride = gridDim.x * blockDim.x;
-    for (int s0 = blockIdx.x * blockDim.x + (threadIdx.x & ~(JAM_WAVE - 1)); s0 < st.n_sets; s0 += stride) {
-        const int s = s0 + lane;
-        const int tier = s < st.n_sets ? kd_tier(st.cnt[s]) : -1;
-        for (int k = 0; k < KD_TIERS; k++) {
-            const unsigned long long mask = ballot(tier == k);
-            if (mask == 0ull) continue;
-            const int leader = first_bit(mask);
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(&nb[k], (unsigned int)__popcll(mask));
-            base = (unsigned int)shfl_int((int)base, leader);
-            if (tier == k) bucket[(size_t)k * st.n_sets + base + rank_below(mask)] = s;
+    const int t = threadIdx.x;
+    for (long long s0 = (long long)blockIdx.x * KD_BUCKET_SPAN; s0 < st.n_sets;
+         s0 += (long long)gridDim.x * KD_BUCKET_SPAN) {
+        const long long s1 = min(s0 + (long long)KD_BUCKET_SPAN, (long long)st.n_sets);
+        if (t < KD_TIERS) cnt[t] = 0u;
+        __syncthreads();
+        for (long long s = s0 + t; s - lane < s1; s += blockDim.x) {
+            const int tier = s < s1 ? kd_tier(st.cnt[s]) : -1;
+            for (int k = 0; k < KD_TIERS; k++) {
+                const unsigned long long mask = ballot(tier == k);
+                if (mask != 0ull && lane == first_bit(mask)) atomicAdd(&cnt[k], (unsigned int)__popcll(mask));
+            }
         }
+        __syncthreads();
+        if (t < KD_TIERS) {
+            base[t] = cnt[t] ? atomicAdd(&nb[t], cnt[t]) : 0u;
+            cnt[t] = 0u;
+        }
+        __syncthreads();
+        for (long long s = s0 + t; s - lane < s1; s += blockDim.x) {
+            const int tier = s < s1 ? kd_tier(st.cnt[s]) : -1;
+            for (int k = 0; k < KD_TIERS; k++) {
+                const unsigned long long mask = ballot(tier == k);
+                if (mask == 0ull) continue;
+                const int leader = first_bit(mask);
+                unsigned int off = 0u;
+                if (lane == leader) off = atomicAdd(&cnt[k], (unsigned int)__popcll(mask));
+                off = (unsigned int)shfl_int((int)off, leader);
+                if (tier == k) bucket[(size_t)k * st.n_sets + base[k] + off + rank_below(mask)] = (int)s;
+            }
+        }
+        __syncthreads();
     }
 }
 
