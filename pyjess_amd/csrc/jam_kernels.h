@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
         // a one-atom template steps past list entry 0: skip its kd position
         if (pr.n1[p] && k >= st.l2k[off0]) k++;
         jn.out_pair[i] = p;
-        jn.out_ent[i * jn.stride] = off0 + k;
+        jn.out_ent[i] = off0 + k;   // level 0's frontier: one entry per row
     }
 }
 
@@ -283,7 +283,7 @@ __device__ static void join_write(const DevJoin &jn, const int *par, const int *
         const int k = idx / L, l = idx - k * L;
         const long long r = base + k;
         if (r >= cap) continue;
-        out_ent[r * jn.stride + l] = jn.in_ent[(long long)par[k] * jn.stride + l];
+        out_ent[r * jn.stride + l] = jn.in_ent[(long long)par[k] * jn.in_stride + l];
     }
     for (int k = lane; k < staged; k += JAM_WAVE) {
         const long long r = base + k;
@@ -412,7 +412,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         const unsigned long long r = atomicAdd(jn.n_done, 1ull);
         if ((long long)r < jn.done_cap) {
             jn.done_pair[r] = p;
-            jn.done_ent[r * jn.stride] = jn.in_ent[i * jn.stride];
+            jn.done_ent[r * jn.stride] = jn.in_ent[i * jn.in_stride];
         } else {
             atomicOr(jn.overflow, 1u);
         }
@@ -650,7 +650,7 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
 #pragma unroll
         for (int jj = 0; jj < NE; jj++) {
             const int j = tl + jj * TW;
-            if (j < L) e_next[jj] = jn.in_ent[r * jn.stride + j];
+            if (j < L) e_next[jj] = jn.in_ent[r * jn.in_stride + j];
         }
     };
     PairCache<NE> pc;
