@@ -1533,7 +1533,8 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         unsigned int *overflow = (unsigned int *)(cnts + CNT_OVF);
         DevJoin jn{};
         jn.stride = c->front_stride;
-        jn.in_stride = c->front_stride;
+        jn.in_stride = 1;
+        jn.out_stride = 1;
         jn.n_done = n_done_dev;
         jn.done_cap = c->done_cap;
         jn.done_pair = c->d_pair.as<int>();
@@ -1560,7 +1561,11 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
                 jn.in_cap = c->front_cap;
                 jn.in_pair = c->j_pair[bi].as<int>();
                 jn.in_ent = c->j_ent[bi].as<int>();
-                jn.in_stride = L == 1 ? 1 : jn.stride;   // k_level0 writes one entry per row
+                // the frontier of level L - 1 holds L entries per row (a
+                // child of level L exists only for templates of more than
+                // L + 1 atoms, so L + 1 never exceeds the row buffers' width)
+                jn.in_stride = L;
+                jn.out_stride = std::min(L + 1, jn.stride);
                 jn.n_out = fcount + L;
                 jn.out_cap = c->front_cap;
                 jn.out_pair = c->j_pair[bo].as<int>();

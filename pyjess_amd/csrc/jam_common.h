@@ -111,8 +111,9 @@ struct DevPairs {
 // complete matches are sorted into the reference order at the end.
 struct DevJoin {
     int level;                          // L: the level being added
-    int stride;                         // ints per row of the output frontier and the complete matches
-    int in_stride;                      // ints per row of the input frontier (1 for level 0's)
+    int stride;                         // ints per row of the complete matches
+    int in_stride;                      // ints per row of the input frontier: L (level L-1's rows)
+    int out_stride;                     // ints per row of the output frontier: L + 1
     const unsigned long long *n_in;     // rows of the input frontier (may exceed in_cap)
     long long in_cap;
     const int *in_pair;

@@ -265,9 +265,10 @@ __global__ void __launch_bounds__(256) k_level0(DevSets st, DevPairs pr, DevJoin
 #endif
 
 // Writes `staged` rows (parent row par[k] + the new entry ent[k]) to a row
-// buffer: one atomic for all of them, then the rows by all lanes.
+// buffer of `ostride` ints per row: one atomic for all of them, then the rows
+// by all lanes.
 __device__ static void join_write(const DevJoin &jn, const int *par, const int *ent, int &staged,
-                                  unsigned long long *count, long long cap, int *out_pair, int *out_ent)
+                                  unsigned long long *count, long long cap, int *out_pair, int *out_ent, int ostride)
 {
     if (staged == 0) return;
     wave_sync();
@@ -283,7 +284,7 @@ __device__ static void join_write(const DevJoin &jn, const int *par, const int *
         const int k = idx / L, l = idx - k * L;
         const long long r = base + k;
         if (r >= cap) continue;
-        out_ent[r * jn.stride + l] = jn.in_ent[(long long)par[k] * jn.in_stride + l];
+        out_ent[r * ostride + l] = jn.in_ent[(long long)par[k] * jn.in_stride + l];
     }
     for (int k = lane; k < staged; k += JAM_WAVE) {
         const long long r = base + k;
@@ -291,7 +292,7 @@ __device__ static void join_write(const DevJoin &jn, const int *par, const int *
             atomicOr(jn.overflow, 1u);
             continue;
         }
-        out_ent[r * jn.stride + L] = ent[k];
+        out_ent[r * ostride + L] = ent[k];
         out_pair[r] = jn.in_pair[par[k]];
     }
     wave_sync();
@@ -520,7 +521,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         if (cm) {
             const int cnt = __popcll(cm);
             if (staged + cnt > JOIN_STAGE) join_write(jn, S.sp, S.se, staged, jn.n_out, jn.out_cap, jn.out_pair,
-                                                      jn.out_ent);
+                                                      jn.out_ent, jn.out_stride);
             if (ok && !last) {
                 const int r = rank_below(cm);
                 S.sp[staged + r] = (int)i;
@@ -531,7 +532,7 @@ __device__ static void join_team(const DevMols &ms, const DevTpls &tp, const Dev
         if (dm) {
             const int cnt = __popcll(dm);
             if (dstaged + cnt > JOIN_STAGE) join_write(jn, S.dp, S.de, dstaged, jn.n_done, jn.done_cap,
-                                                       jn.done_pair, jn.done_ent);
+                                                       jn.done_pair, jn.done_ent, jn.stride);
             if (ok && last) {
                 const int r = rank_below(dm);
                 S.dp[dstaged + r] = (int)i;
@@ -667,8 +668,8 @@ __global__ void __launch_bounds__(JOIN_WAVES * JAM_WAVE, JOIN_MIN_WAVES) k_join(
         join_team<MAXN, TW>(ms, tp, st, pr, jn, qp, SW[w], it < items, it, parts, p, e, pc, staged, dstaged);
         base = next;
     }
-    join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent);
-    join_write(jn, SW[w].dp, SW[w].de, dstaged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent);
+    join_write(jn, SW[w].sp, SW[w].se, staged, jn.n_out, jn.out_cap, jn.out_pair, jn.out_ent, jn.out_stride);
+    join_write(jn, SW[w].dp, SW[w].de, dstaged, jn.n_done, jn.done_cap, jn.done_pair, jn.done_ent, jn.stride);
 }
 
 // ---------------------------------------------------------------------------
