@@ -144,6 +144,7 @@ struct jam_context {
     bool kd_pending = false;    // side-stream work of the last kd launch may still run
     // sort of the complete matches
     DBuf s_key[2], s_perm[2], s_pkey[2];
+    DBuf s_starts;   // per template size: complete matches of at most that many atoms
     // radix-sort histograms and their scan; scan temporaries of the query
     DBuf rs_hist, rs_off;
     std::vector<std::unique_ptr<DBuf>> q_scan;
@@ -1647,17 +1648,48 @@ extern "C" int32_t jam_query(jam_context *c, const jam_templates *tp, const jam_
         } else if (n_done > 0) {
             const int per = std::max(1, 64 / bits);
             const int nt = stride;
+            // the matches by template size (stably): a match's levels at or
+            // past its template's size are 0, so a group of levels starting
+            // at l0 needs sorting only over the matches of more than l0
+            // atoms, a suffix; that suffix's order is the only one to change,
+            // so both permutation buffers keep the rest as it was
             hipLaunchKernelGGL(jam::k_iota, dim3(gs), dim3(256), 0, s, c->s_perm[0].as<int>(), (long long)n_done);
+            hipLaunchKernelGGL(jam::k_sizekey, dim3(gs), dim3(256), 0, s, dt, jn, (long long)n_done,
+                               c->s_pkey[0].as<unsigned int>());
+            {
+                int which = 0;
+                if ((rc = radix_sort<unsigned int>(c->s_pkey[0].as<unsigned int>(), c->s_perm[0].as<int>(),
+                                                   c->s_pkey[1].as<unsigned int>(), c->s_perm[1].as<int>(),
+                                                   (long long)n_done, 6, c->rs_hist, c->rs_off, c->q_scan, s,
+                                                   which)) != JAM_OK)
+                    return rc;
+                cur = which;
+                HIP_TRY(c->s_starts.reserve(sizeof(long long) * (JAM_MAX_LEVELS + 1)));
+                hipLaunchKernelGGL(jam::k_size_starts, dim3(1), dim3(64), 0, s, c->s_pkey[which].as<const unsigned int>(),
+                                   (long long)n_done, nt, c->s_starts.as<long long>());
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipMemcpyAsync(c->s_perm[cur ^ 1].as<int>(), c->s_perm[cur].as<const int>(),
+                                       sizeof(int) * (size_t)n_done, hipMemcpyDeviceToDevice, s));
+                HIP_TRY(hipMemcpyAsync(c->hsmall, c->s_starts.as<const long long>(), sizeof(long long) * (size_t)(nt + 1),
+                                       hipMemcpyDeviceToHost, s));
+                HIP_TRY(hipStreamSynchronize(s));
+            }
+            long long size_le[JAM_MAX_LEVELS + 1];
+            std::memcpy(size_le, c->hsmall, sizeof(long long) * (size_t)(nt + 1));
             for (int l0 = ((nt - 1) / per) * per; l0 >= 0; l0 -= per) {
                 const int cnt_l = std::min(per, nt - l0);
+                const long long a = l0 == 0 ? 0 : size_le[l0];   // matches of at most l0 atoms stay put
+                const long long m = (long long)n_done - a;
+                if (m <= 1) continue;
                 // levels [l0, l0 + cnt_l) packed into cnt_l * bits bits
-                hipLaunchKernelGGL(jam::k_sortkey, dim3(gs), dim3(256), 0, s, st, dt, pr, jn,
-                                   c->s_perm[cur].as<const int>(), (long long)n_done, l0, cnt_l, bits,
-                                   c->s_key[0].as<unsigned long long>());
+                hipLaunchKernelGGL(jam::k_sortkey, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, st, dt, pr, jn,
+                                   c->s_perm[cur].as<const int>() + a, m, l0, cnt_l, bits,
+                                   c->s_key[0].as<unsigned long long>() + a);
                 int which = 0;
-                if ((rc = radix_sort<unsigned long long>(c->s_key[0].as<unsigned long long>(), c->s_perm[cur].as<int>(),
-                                                         c->s_key[1].as<unsigned long long>(),
-                                                         c->s_perm[cur ^ 1].as<int>(), (long long)n_done, cnt_l * bits,
+                if ((rc = radix_sort<unsigned long long>(c->s_key[0].as<unsigned long long>() + a,
+                                                         c->s_perm[cur].as<int>() + a,
+                                                         c->s_key[1].as<unsigned long long>() + a,
+                                                         c->s_perm[cur ^ 1].as<int>() + a, m, cnt_l * bits,
                                                          c->rs_hist, c->rs_off, c->q_scan, s, which)) != JAM_OK)
                     return rc;
                 cur ^= which;

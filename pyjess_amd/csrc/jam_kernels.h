@@ -726,6 +726,30 @@ __global__ void __launch_bounds__(256) k_sortkey(DevSets st, DevTpls tp, DevPair
     key[s] = level_key(st, tp, pr, jn, perm[s], l0, per, bits);
 }
 
+// Template size (atoms) of complete match s, the key that orders the matches
+// by how many level groups their sort keys reach.
+__global__ void __launch_bounds__(256) k_sizekey(DevTpls tp, DevJoin jn, long long n, unsigned int *key)
+{
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int t = jn.done_pair[s] % tp.n_templates;
+    key[s] = (unsigned int)(tp.atom_off[t + 1] - tp.atom_off[t]);
+}
+
+// starts[l] (l = 0 ... maxl): how many of the n sorted sizes are at most l.
+__global__ void k_size_starts(const unsigned int *sorted, long long n, int maxl, long long *starts)
+{
+    const int l = (int)threadIdx.x;
+    if (l > maxl) return;
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = lo + (hi - lo) / 2;
+        if (sorted[mid] <= (unsigned int)l) lo = mid + 1;
+        else hi = mid;
+    }
+    starts[l] = lo;
+}
+
 __global__ void __launch_bounds__(256) k_pairkey(DevJoin jn, int n_templates, const int *perm, long long n,
                                                  unsigned int *key)
 {
