@@ -434,11 +434,12 @@ def pair_ranges(const int[:] pair, int n_templates, const long long[:] pairs):
 
 
 def hit_list_ranges(block, const double[:] rmsd, const int[:] pair, const long long[:] bounds,
-                    const long long[:] nan_events, int n_templates, list templates, list molecules):
+                    const long long[:] skip, int n_templates, list templates, list molecules):
     """One `HitList` per molecule holding its events bounds[m]:bounds[m + 1]
-    except the NaN-rotation ones (sorted ``nan_events``): a range of rows
-    when it has none, else the list of its other rows."""
-    cdef Py_ssize_t m, n = len(molecules), b0, b1, j0, j1, e, w, k = nan_events.shape[0]
+    except those in ``skip`` (sorted: the NaN-rotation events, and any hit a
+    policy drops): a range of rows when it has none of them, else the list of
+    its other rows."""
+    cdef Py_ssize_t m, n = len(molecules), b0, b1, j0, j1, e, w, k = skip.shape[0]
     cdef HitList h
     cdef list out = []
     cdef long long[:] r
@@ -448,8 +449,8 @@ def hit_list_ranges(block, const double[:] rmsd, const int[:] pair, const long l
         b0, b1 = bounds[m], bounds[m + 1]
         if not (0 <= b0 <= b1 <= rmsd.shape[0] and b1 <= pair.shape[0]):
             raise ValueError("bounds out of range")
-        j0 = _lower_bound(nan_events, 0, k, b0)
-        j1 = _lower_bound(nan_events, j0, k, b1)
+        j0 = _lower_bound(skip, 0, k, b0)
+        j1 = _lower_bound(skip, j0, k, b1)
         h = HitList.__new__(HitList)
         h._block = block
         h._rmsd = rmsd
@@ -466,7 +467,7 @@ def hit_list_ranges(block, const double[:] rmsd, const int[:] pair, const long l
             r = rows
             w = 0
             for e in range(b0, b1):
-                if j0 < j1 and nan_events[j0] == e:
+                if j0 < j1 and skip[j0] == e:
                     j0 += 1
                     continue
                 r[w] = e

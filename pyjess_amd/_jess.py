@@ -178,10 +178,10 @@ def _policy(jess: "Jess", molecules: Sequence[Molecule], res: "_native.Result", 
         flag = np.ascontiguousarray(res.flag, dtype=np.int32)
         rmsd = np.ascontiguousarray(res.rmsd, dtype=np.float64)
         if chosen is None:
-            bounds, nan, nan_ev = _split_ranges(pair, flag, T, len(molecules))
+            bounds, nan, skip = _split_ranges(pair, flag, T, len(molecules))
             if cut_late:
-                nan_ev, nan = _cut_events(res, pair, flag, rmsd, T, mc, nan_ev, nan)
-            out = _hit_list_ranges(block, rmsd, pair, bounds, nan_ev, T, list(jess._templates), list(molecules))
+                skip, nan = _cut_events(res, pair, flag, rmsd, T, mc, skip, nan)
+            out = _hit_list_ranges(block, rmsd, pair, bounds, skip, T, list(jess._templates), list(molecules))
             return out, nan
         hit_ev, bounds, nan = _split_events(pair, flag, chosen, T, len(molecules))
         out = _hit_lists(block, rmsd, pair, np.ascontiguousarray(hit_ev, dtype=np.int64),
