@@ -303,11 +303,13 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
                                                 const unsigned int *count, int *tied, unsigned int *n_tied)
 {
     constexpr int NW = NT / JAM_WAVE;
-    __shared__ double xkey[NT];
-    __shared__ int xid[NT];
-    __shared__ int xpre[NT];
+    // the two axes other than a level's are partitioned together: one
+    // exchange buffer and prefix array each
+    __shared__ double xkey[NT], xkey2[NT];
+    __shared__ int xid[NT], xid2[NT];
+    __shared__ int xpre[NT], xpre2[NT];
     __shared__ int side[NT];
-    __shared__ int wsum[NW];
+    __shared__ int wsum[NW], wsum2[NW];
     const int t = threadIdx.x;
     const int lane = lane_id();
     const int wv = t / JAM_WAVE;
@@ -361,51 +363,78 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
             key[a] = k0;
             id[a] = i0;
         }
+        // Per level, five barriers: the split side of every element; the tie
+        // test folded into the publication of both axes' ballot counts;
+        // their prefix counts; their stable partitions; and whether any
+        // range is still open folded into the last.
         int sb = 0, se = n;
         bool tie = false;
-        for (int d = 0;; d++) {
+        bool open = __syncthreads_or(t < n && n >= 2);
+        for (int d = 0; open; d++) {
             const int a = d % 3;
+            const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;   // the two other axes
             const bool act = t < n && se - sb >= 2;
             const int m = sb + (se - sb) / 2;
-            if (!__syncthreads_or(act)) break;
             const double ka = (a == 0) ? key[0] : (a == 1) ? key[1] : key[2];
             const int ia = (a == 0) ? id[0] : (a == 1) ? id[1] : id[2];
             xkey[t] = ka;
             side[ia] = (act && t < m) ? 1 : 0;
             __syncthreads();
             const bool my_tie = act && t == m && xkey[t - 1] == ka;
+            const int i1 = (b1 == 0) ? id[0] : id[1];
+            const int i2 = (b2 == 1) ? id[1] : id[2];
+            const double k1 = (b1 == 0) ? key[0] : key[1];
+            const double k2 = (b2 == 1) ? key[1] : key[2];
+            const int sd1 = side[i1], sd2 = side[i2];
+            const unsigned long long bal1 = ballot(act && sd1), bal2 = ballot(act && sd2);
+            if (lane == 0) {
+                wsum[wv] = __popcll(bal1);
+                wsum2[wv] = __popcll(bal2);
+            }
             if (__syncthreads_or(my_tie)) {
                 tie = true;
                 break;
             }
-#pragma unroll
-            for (int b = 0; b < 3; b++) {
-                if (b == a) continue;
-                const int sd = side[id[b]];
-                const bool f = act && sd;
-                const unsigned long long bal = ballot(f);
-                if (lane == 0) wsum[wv] = __popcll(bal);
-                __syncthreads();
-                int pre = rank_below(bal);
-                for (int w = 0; w < wv; w++) pre += wsum[w];
-                xpre[t] = pre;
-                __syncthreads();
-                int np = t;
-                if (act) {
-                    const int lc = pre - xpre[sb];
-                    np = sd ? sb + lc : m + (t - sb - lc);
-                }
-                xkey[np] = key[b];
-                xid[np] = id[b];
-                __syncthreads();
-                key[b] = xkey[t];
-                id[b] = xid[t];
-                __syncthreads();
+            int pre1 = rank_below(bal1), pre2 = rank_below(bal2);
+            for (int w = 0; w < wv; w++) {
+                pre1 += wsum[w];
+                pre2 += wsum2[w];
+            }
+            xpre[t] = pre1;
+            xpre2[t] = pre2;
+            __syncthreads();
+            int np1 = t, np2 = t;
+            if (act) {
+                const int lc1 = pre1 - xpre[sb], lc2 = pre2 - xpre2[sb];
+                np1 = sd1 ? sb + lc1 : m + (t - sb - lc1);
+                np2 = sd2 ? sb + lc2 : m + (t - sb - lc2);
+            }
+            xkey[np1] = k1;
+            xid[np1] = i1;
+            xkey2[np2] = k2;
+            xid2[np2] = i2;
+            __syncthreads();
+            const double n1 = xkey[t], n2 = xkey2[t];
+            const int j1 = xid[t], j2 = xid2[t];
+            if (b1 == 0) {
+                key[0] = n1;
+                id[0] = j1;
+            } else {
+                key[1] = n1;
+                id[1] = j1;
+            }
+            if (b2 == 1) {
+                key[1] = n2;
+                id[1] = j2;
+            } else {
+                key[2] = n2;
+                id[2] = j2;
             }
             if (act) {
                 if (t < m) se = m;
                 else sb = m;
             }
+            open = __syncthreads_or(t < n && se - sb >= 2);
         }
         if (tie) {
             if (t == 0) tied[atomicAdd(n_tied, 1u)] = s;
