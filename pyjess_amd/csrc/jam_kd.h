@@ -331,6 +331,10 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
         // positions n ... NT-1 hold padding elements with infinite keys
         double key[3];
         int id[3];
+        // exchanges across waves go through LDS, the steps taking the two
+        // buffer pairs in turn: a buffer is written again only after a
+        // barrier that every read of it precedes, so one barrier a step
+        bool second = false;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             double k0 = c[a];
@@ -340,12 +344,14 @@ __global__ void __launch_bounds__(NT) k_kdblock(DevMols ms, DevTpls tp, DevSets 
                     double ok;
                     int oi;
                     if (j >= JAM_WAVE) {
-                        xkey[t] = k0;
-                        xid[t] = i0;
+                        double *bk = second ? xkey2 : xkey;
+                        int *bi = second ? xid2 : xid;
+                        second = !second;
+                        bk[t] = k0;
+                        bi[t] = i0;
                         __syncthreads();
-                        ok = xkey[t ^ j];
-                        oi = xid[t ^ j];
-                        __syncthreads();
+                        ok = bk[t ^ j];
+                        oi = bi[t ^ j];
                     } else {
                         ok = __shfl_xor(k0, j, JAM_WAVE);
                         oi = __shfl_xor(i0, j, JAM_WAVE);
